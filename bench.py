@@ -1,0 +1,234 @@
+"""Benchmark of the prover hot path on one MI355X.
+
+Default workload (the headline metric's configuration): Basefold commit-and-open of one
+multilinear polynomial with 2^22 uniform Goldilocks evaluations, rate 1/2 (2^23 codeword),
+Monolith-w12 Merkle trees, 22 fold rounds with Ext2 challenges from the Poseidon2-w8
+transcript, 100 query openings. A step is one full commit + open with the evaluations
+already resident in HBM; the folding table (setup) is built once before timing.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload basefold|sumcheck]
+
+Prints one JSON line. `value` is proved elements per second (2^22 per step).
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "proved Goldilocks elems/s (sumcheck+Basefold, 2^22); proofs/s + p99 latency"
+HBM_PEAK_GBS = 8000.0
+# 32-bit VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz (AMD's 157.3 TFLOP/s
+# FP32 vector figure counts an FMA as 2 flops on 2 packed lanes: 157.3 / 4 = 39.3).
+VALU_PEAK_TOPS = 39.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["basefold", "sumcheck"], default="basefold")
+    ap.add_argument("--nv", type=int, default=None)
+    ap.add_argument("--queries", type=int, default=100)
+    ap.add_argument("--cpu-sample-nv", type=int, default=None)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--valu-ops-per-perm", type=float, default=float(os.environ.get("MAM_VALU_OPS_PER_PERM", "0")))
+    return ap.parse_args()
+
+
+def gen_inputs(nv, seed=1):
+    """Uniform Goldilocks words from a seeded generator (rejection of values >= p)."""
+    P = 0xFFFFFFFF00000001
+    rng = np.random.default_rng(seed)
+    x = rng.integers(0, 1 << 64, 1 << nv, dtype=np.uint64)
+    bad = x >= np.uint64(P)
+    while bad.any():
+        x[bad] = rng.integers(0, 1 << 64, int(bad.sum()), dtype=np.uint64)
+        bad = x >= np.uint64(P)
+    return x
+
+
+def profile_collect(lib):
+    ms = (ctypes.c_double * 4)()
+    cnt = (ctypes.c_long * 4)()
+    work = (ctypes.c_double * 4)()
+    assert lib.mam_profile_collect(ms, cnt, work) == 0
+    return [(ms[i], cnt[i], work[i]) for i in range(4)]
+
+
+def cpu_baseline_basefold(nv, lr, nq):
+    """Times the oracle (single-threaded C restatement) on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    ev = gen_inputs(nv, seed=3)
+    pt = np.resize(gen_inputs(6, seed=5), 2 * nv)
+    table = O.basefold_table(nv, lr, nv, 42)
+    t0 = time.perf_counter()
+    cw, bh, tree = O.basefold_commit(ev, nv, lr, O.MONOLITH)
+    O.basefold_open_ext2(cw, bh, tree[-4:], nv, lr, O.MONOLITH, pt, table, nv, O.Transcript(b"bench"), nq)
+    dt = time.perf_counter() - t0
+    return {"value": (1 << nv) / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+            "sample": f"oracle commit+open_ext2 of 2^{nv} evaluations (rate 1/2, Monolith, {nq} queries), "
+                      f"single thread, {dt:.2f} s"}
+
+
+def cpu_baseline_sumcheck(nv):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    f = gen_inputs(nv + 1, seed=6)
+    ext = np.zeros((2, 1 << nv, 2), dtype=np.uint64)
+    ext[:, :, 0] = f.reshape(2, -1)
+    t0 = time.perf_counter()
+    O.sumcheck_prove_ext2(O.Transcript(b"bench"), ext, 2, nv)
+    dt = time.perf_counter() - t0
+    return {"value": (2 << nv) / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+            "sample": f"oracle Ext2 sumcheck of f*g, 2 x 2^{nv} evaluations, single thread, {dt:.2f} s"}
+
+
+def bench_basefold(args, M):
+    nv = args.nv or 22
+    lr, nq, hash_kind = 1, args.queries, M.HASH_MONOLITH
+    lib = M.lib
+    ev = gen_inputs(nv)
+    pt = np.resize(gen_inputs(8, seed=2), 2 * nv)
+    table = M.BasefoldTable.generate(nv, lr, nv, 42)
+    d_table = table.upload()
+    d_ev = M.DeviceBuffer.from_numpy(ev)
+    cw_len = 1 << (nv + lr)
+    d_cw, d_bh, d_tree = M.DeviceBuffer(cw_len), M.DeviceBuffer(1 << nv), M.DeviceBuffer((cw_len - 1) * 4)
+    root = np.zeros(4, dtype=np.uint64)
+    U = ctypes.POINTER(ctypes.c_uint64)
+    proof = np.zeros(M.proof_len(nv, lr, nq), dtype=np.uint64)
+
+    def step():
+        M.check(lib.mam_basefold_commit_into(d_ev.ptr, nv, lr, hash_kind, d_cw.ptr, d_bh.ptr, d_tree.ptr,
+                                             root.ctypes.data_as(U)), "commit")
+        h = ctypes.c_void_p()
+        M.check(lib.mam_basefold_commitment_wrap(d_cw.ptr, d_bh.ptr, d_tree.ptr, root.ctypes.data_as(U), nv, lr,
+                                                 hash_kind, ctypes.byref(h)), "wrap")
+        t = M.Transcript(b"bench")
+        M.check(lib.mam_basefold_open_ext2(h.value, pt.ctypes.data_as(U), d_table.ptr, nv, ctypes.byref(t.t), nq,
+                                           proof.ctypes.data_as(U)), "open")
+        lib.mam_basefold_commitment_free(h.value)
+
+    for _ in range(args.warmup):
+        step()
+    M.synchronize()
+    lib.mam_profile_enable(1)
+    lat = []
+    M.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - s0)
+    M.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = profile_collect(lib)
+    lib.mam_profile_enable(0)
+    verified = M.verify_ext2(root, pt, nv, lr, hash_kind, proof, nq, table, M.Transcript(b"bench"))
+    # dominant kernel group: Merkle builds (Monolith permutations)
+    mk_ms, mk_n, mk_perm = prof[0]
+    perm_rate = mk_perm / (mk_ms / 1e3) if mk_ms > 0 else 0.0
+    roof = {"kernel": "k_tree_levels<Monolith> (Merkle builds: leaves + in-LDS levels)",
+            "bound": "valu", "unit": "Tops/s", "peak": VALU_PEAK_TOPS,
+            "achieved": None, "frac": None, "traffic": None,
+            "perms_per_s": perm_rate, "avg_build_ms": mk_ms / max(mk_n, 1), "builds": mk_n,
+            "perms_per_build": mk_perm / max(mk_n, 1)}
+    if args.valu_ops_per_perm > 0:
+        # each thread runs one permutation's instruction stream: lane-ops = perms x instructions
+        ach = perm_rate * args.valu_ops_per_perm / 1e12
+        roof["achieved"] = ach
+        roof["frac"] = ach / VALU_PEAK_TOPS
+        roof["valu_ops_per_perm"] = args.valu_ops_per_perm
+    hbm = {}
+    for cat, name in ((1, "open_fold_oracle"), (2, "commit_transform")):
+        ms, n, by = prof[cat]
+        if ms > 0:
+            gbs = by / (ms / 1e3) / 1e9
+            hbm[name] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "avg_ms": ms / n, "launch_groups": n}
+    lat_ms = np.array(lat) * 1e3
+    out = {"metric": METRIC, "value": args.steps * (1 << nv) / elapsed, "unit": "elems/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+           "config": {"workload": "basefold_commit_open", "num_vars": nv, "log_rate": lr, "codeword_len": cw_len,
+                      "merkle_hash": "monolith-w12", "transcript": "poseidon2-w8", "fold_rounds": nv,
+                      "num_queries": nq, "table_seed": 42, "parallelism": "single-gpu"},
+           "proofs_per_s": args.steps / elapsed, "p50_ms": float(np.percentile(lat_ms, 50)),
+           "p99_ms": float(np.percentile(lat_ms, 99)), "verified": verified, "roofline": roof,
+           "hbm_kernels": hbm}
+    return out
+
+
+def bench_sumcheck(args, M):
+    nv = args.nv or 20
+    f = gen_inputs(nv + 1)
+    d = M.DeviceBuffer.from_numpy(f)
+    lib = M.lib
+
+    def step():
+        return M.sumcheck_prove(M.Transcript(b"bench"), d, 2, nv, base=True)
+
+    for _ in range(args.warmup):
+        step()
+    M.synchronize()
+    lib.mam_profile_enable(1)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        step()
+        lat.append(time.perf_counter() - s0)
+    M.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = profile_collect(lib)
+    lib.mam_profile_enable(0)
+    ms, n, by = prof[3]
+    gbs = by / (ms / 1e3) / 1e9 if ms > 0 else 0.0
+    lat_ms = np.array(lat) * 1e3
+    return {"metric": METRIC, "value": args.steps * (2 << nv) / elapsed, "unit": "elems/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+            "data": "synthetic",
+            "config": {"workload": "sumcheck_product_f_g", "num_vars": nv, "num_polys": 2,
+                       "transcript": "poseidon2-w8", "parallelism": "single-gpu"},
+            "proofs_per_s": args.steps / elapsed, "p50_ms": float(np.percentile(lat_ms, 50)),
+            "p99_ms": float(np.percentile(lat_ms, 99)),
+            "roofline": {"kernel": "k_sc_fold_msg (fold + next message)", "bound": "hbm", "achieved": gbs,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                         "avg_ms": ms / max(n, 1), "launches": n}}
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}, WORLD_SIZE {world}); "
+              "this build has no multi-GPU path", flush=True)
+        sys.exit(2)
+    import morpheus_amd as M
+    if M.lib.goldilocks_cuda_init() != 0:
+        print("no usable GPU", file=sys.stderr)
+        sys.exit(1)
+    out = bench_basefold(args, M) if args.workload == "basefold" else bench_sumcheck(args, M)
+    if not args.no_cpu:
+        if args.workload == "basefold":
+            out["cpu_baseline"] = cpu_baseline_basefold(args.cpu_sample_nv or 18, 1, args.queries)
+        else:
+            out["cpu_baseline"] = cpu_baseline_sumcheck(args.cpu_sample_nv or 20)
+        out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
+        out["cpu_baseline"]["nproc"] = os.cpu_count()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

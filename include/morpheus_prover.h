@@ -1,0 +1,112 @@
+/*
+ * morpheus_prover.h -- prover-level C-ABI of libmorpheus_amd.so.
+ *
+ * In the reference these operations are Rust methods of goldilocks-cuda-rs and
+ * zk-torch-3 layered over the raw FFI; this library implements them natively (C++ host
+ * code over HIP kernels) and exports them as C functions, so the Rust types become thin
+ * shells over these calls. Each declaration cites the Rust interface it replaces.
+ *
+ * Conventions: d_* device pointers, h_* / unprefixed pointers host memory; Ext2 values
+ * are [c0, c1] word pairs; returns 0 / -1 (runtime error) / -2 (rejected argument) unless
+ * stated; every function completes before returning.
+ */
+#ifndef MORPHEUS_PROVER_H
+#define MORPHEUS_PROVER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Fiat-Shamir transcript: zk-torch-3/src/transcript.rs:167-330 (Transcript) ---- */
+typedef struct {
+    uint64_t state[8];
+    uint64_t in_buf[4];
+    uint64_t out_buf[4];
+    int64_t in_count;
+    int64_t out_count;
+} mam_transcript;
+
+void mam_transcript_init(mam_transcript *t, const uint8_t *label, long len);        /* Transcript::new, :182 */
+void mam_transcript_append_u64(mam_transcript *t, const uint8_t *label, long len,
+                               uint64_t v);                                         /* append_u64, :225 */
+void mam_transcript_append_scalars(mam_transcript *t, const uint8_t *label, long len, const uint64_t *v,
+                                   long n);                                         /* append_scalars, :217 */
+void mam_transcript_append_ext2(mam_transcript *t, const uint8_t *label, long len,
+                                const uint64_t *v2);                                /* append_ext2, :255 */
+uint64_t mam_transcript_challenge_scalar(mam_transcript *t, const uint8_t *label,
+                                         long len);                                 /* challenge_scalar, :231 */
+void mam_transcript_challenge_ext2(mam_transcript *t, const uint8_t *label, long len,
+                                   uint64_t *out2);                                 /* challenge_ext2, :247 */
+void mam_poseidon2_permute_host(uint64_t *state8);                                  /* poseidon2_permute, :136 */
+void mam_monolith_permute_host(uint64_t *state12);       /* goldilocks-cuda-rs/src/cpu_monolith.rs:141 */
+
+/* ---- linear sumcheck over Ext2 ----
+ * Whole proof: zk-torch-3/src/sumcheck/gpu_prover.rs:21-89 (GpuLinearSumcheckProver::new
+ * + prove / prove_ext2). d_polys holds num_polys polynomials of 2^num_vars elements back
+ * to back, base words (input_is_base = 1) or interleaved Ext2; it is not modified.
+ * Outputs: round_msgs[num_vars][num_polys + 1][2], challenges[num_vars][2],
+ * final_evals[num_polys][2], final_eval[2]. num_polys <= 8. */
+int mam_sumcheck_prove(mam_transcript *t, const uint64_t *d_polys, int num_polys, int num_vars, int input_is_base,
+                       uint64_t *round_msgs, uint64_t *challenges, uint64_t *final_evals, uint64_t *final_eval);
+/* GpuSumcheckStateExt2 (goldilocks-cuda-rs/src/sumcheck_prover.rs:205-423):
+ * new / new_from_base (:218, :262), compute_round_message (:325), fold (:373),
+ * final_evaluations (:401). The state reads d_polys until its first fold is applied. */
+int mam_sc_state_new(const uint64_t *d_polys, int num_polys, int num_vars, int input_is_base, void **handle);
+int mam_sc_state_round_message(void *handle, uint64_t *msg_out /* (num_polys + 1) * 2 */);
+int mam_sc_state_fold(void *handle, uint64_t challenge_c0, uint64_t challenge_c1);
+int mam_sc_state_final_evaluations(void *handle, uint64_t *out /* num_polys * 2 */);
+int mam_sc_state_free(void *handle);
+
+/* ---- Basefold PCS: goldilocks-cuda-rs/src/basefold.rs ---- */
+#define MAM_HASH_MONOLITH 0  /* default Merkle hash (goldilocks-cuda-rs/Cargo.toml:22-24) */
+#define MAM_HASH_POSEIDON2 1
+/* BasefoldTable::generate (:281-321): entries as (point, weight) pairs, host memory */
+long mam_basefold_table_len(int num_vars, int log_rate, int num_rounds);
+int mam_basefold_table_generate(int num_vars, int log_rate, int num_rounds, uint64_t seed, uint64_t *entries);
+/* BasefoldCommitment::commit_device (:382-414); handle owns its device buffers */
+int mam_basefold_commit(const uint64_t *d_evals, int num_vars, int log_rate, int hash, void **handle,
+                        uint64_t *root_out /* 4 */);
+/* Same into caller buffers: codeword 2^(nv+lr), bh_evals 2^nv, tree (2^(nv+lr) - 1) * 4 words */
+int mam_basefold_commit_into(const uint64_t *d_evals, int num_vars, int log_rate, int hash, uint64_t *d_codeword,
+                             uint64_t *d_bh_evals, uint64_t *d_tree, uint64_t *root_out);
+/* HostCommitmentCache re-upload (:873-900): wrap caller-owned buffers as a commitment */
+int mam_basefold_commitment_wrap(uint64_t *d_codeword, uint64_t *d_bh_evals, uint64_t *d_tree,
+                                 const uint64_t *root4, int num_vars, int log_rate, int hash, void **handle);
+int mam_basefold_commitment_buffers(void *handle, uint64_t **d_codeword, uint64_t **d_bh_evals, uint64_t **d_tree,
+                                    uint64_t *root_out);
+int mam_basefold_commitment_free(void *handle);
+/* BasefoldCommitment::open_ext2 (:578-863) with extract_ext2_queries (:2310-2402).
+ * point2: num_vars Ext2 (host); d_table: device table generated for table_nv >= num_vars;
+ * proof: mam_basefold_proof_len() words in the flat layout
+ *   eval[2] | oracles[nv][3][2] | folded_roots[nv-1][4] | final_cw[2^lr][2] |
+ *   queries x { index | values[nv][2][2] | initial path[nv+lr-1][4] | folded paths } */
+long mam_basefold_proof_len(int num_vars, int log_rate, int num_queries);
+int mam_basefold_open_ext2(void *handle, const uint64_t *point2, const uint64_t *d_table, int table_nv,
+                           mam_transcript *t, int num_queries, uint64_t *proof);
+/* BasefoldVerifier::verify_ext2 (:1759-1925); host only; returns 1 accept, 0 reject */
+int mam_basefold_verify_ext2(const uint64_t *root4, const uint64_t *point2, int num_vars, int log_rate, int hash,
+                             const uint64_t *proof, int num_queries, const uint64_t *table, int table_nv,
+                             mam_transcript *t);
+
+/* ---- device primitives without a reference symbol of their own ---- */
+/* Monolith permutation of n 12-word states in place (the kernel behind the reference's
+ * monolith_merkle_tree_* builders, cuda/monolith.cuh:94-216) */
+int mam_monolith_permute_batch(uint64_t *d_states, int n);
+/* Merkle build on the calling thread's stream without synchronizing (ffi.rs:549-576 semantics) */
+int mam_merkle_tree_async(const uint64_t *d_codeword, uint64_t *d_tree, size_t num_leaves, int hash,
+                          int ext2_leaves);
+
+/* Opt-in timing of kernel groups with HIP events on their launch stream (no reference
+ * counterpart; used by bench.py). Categories: 0 Merkle builds (work = permutations),
+ * 1 opening fold+oracle (bytes), 2 commit transform (bytes), 3 sumcheck fold+message
+ * (bytes). collect() fills 4 entries of each array and clears the record. */
+int mam_profile_enable(int on);
+int mam_profile_collect(double *ms, long *count, double *work);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MORPHEUS_PROVER_H */

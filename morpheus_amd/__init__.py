@@ -1,0 +1,12 @@
+"""MI355X-native prover hot path: Goldilocks/Ext2 sumcheck, eq tables, Basefold with
+Monolith/Poseidon2 Merkle trees, behind the goldilocks-cuda-rs C-ABI.
+
+Importing requires the built HIP library (morpheus_amd/libmorpheus_amd.so); there is no
+CPU fallback.
+"""
+from ._lib import EXPORTED, LIB_PATH, lib  # noqa: F401
+from .device import DeviceBuffer, GpuError, check, device_count, synchronize  # noqa: F401
+from .prover import (HASH_MONOLITH, HASH_POSEIDON2, BasefoldCommitment, BasefoldTable,  # noqa: F401
+                     GpuSumcheckStateExt2, Transcript, proof_len, split_proof, sumcheck_prove, verify_ext2)
+
+P = 0xFFFFFFFF00000001

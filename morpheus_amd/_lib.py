@@ -1,0 +1,148 @@
+"""Loader for libmorpheus_amd.so and the ctypes signatures of its C-ABI.
+
+The library is built in-tree (``morpheus_amd/libmorpheus_amd.so``) by
+``morpheus_amd.build.build()``. There is no fallback: importing this module without the
+built library raises ``ImportError``.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmorpheus_amd.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()' "
+        "or make -C morpheus_amd/csrc)")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+u64 = ctypes.c_uint64
+u64p = ctypes.POINTER(ctypes.c_uint64)
+vp = ctypes.c_void_p
+ci = ctypes.c_int
+cl = ctypes.c_long
+csz = ctypes.c_size_t
+u8p = ctypes.c_char_p
+
+
+class Transcript_t(ctypes.Structure):
+    _fields_ = [("state", u64 * 8), ("in_buf", u64 * 4), ("out_buf", u64 * 4),
+                ("in_count", ctypes.c_int64), ("out_count", ctypes.c_int64)]
+
+
+trp = ctypes.POINTER(Transcript_t)
+
+_SIGS = {
+    # reference-named (morpheus_gl.h)
+    "goldilocks_cuda_init": (ci, []),
+    "poseidon2_cuda_init": (ci, []),
+    "monolith_cuda_init": (ci, []),
+    "cuda_malloc": (ci, [ctypes.POINTER(vp), csz]),
+    "cuda_free": (ci, [vp]),
+    "cuda_memcpy_htod": (ci, [vp, vp, csz]),
+    "cuda_memcpy_dtoh": (ci, [vp, vp, csz]),
+    "cuda_memcpy_dtod": (ci, [vp, vp, csz]),
+    "cuda_device_synchronize": (ci, []),
+    "cuda_get_last_error": (ci, []),
+    "cuda_peek_at_last_error": (ci, []),
+    "cuda_mem_get_info": (ci, [ctypes.POINTER(csz), ctypes.POINTER(csz)]),
+    "cuda_set_device": (ci, [ci]),
+    "cuda_get_device": (ci, [ctypes.POINTER(ci)]),
+    "cuda_get_device_count": (ci, [ctypes.POINTER(ci)]),
+    "cuda_get_device_name": (ci, [ci, ctypes.c_char_p, ci]),
+    "gl_memset_zero": (ci, [vp, csz]),
+    "gl_zero_buffer": (ci, [vp, ci]),
+    "gl_batch_add": (ci, [vp, vp, vp, ci]),
+    "gl_batch_sub": (ci, [vp, vp, vp, ci]),
+    "gl_batch_mul": (ci, [vp, vp, vp, ci]),
+    "gl_batch_div": (ci, [vp, vp, vp, ci]),
+    "gl_batch_inverse": (ci, [vp, vp, ci]),
+    "gl_batch_neg": (ci, [vp, vp, ci]),
+    "gl_batch_square": (ci, [vp, vp, ci]),
+    "gl_batch_double": (ci, [vp, vp, ci]),
+    "gl_batch_exp": (ci, [vp, u64, vp, ci]),
+    "gl_batch_mul_scalar": (ci, [u64, vp, vp, ci]),
+    "ext2_batch_add_ffi": (ci, [vp, vp, vp, ci]),
+    "ext2_batch_sub_ffi": (ci, [vp, vp, vp, ci]),
+    "ext2_batch_mul_ffi": (ci, [vp, vp, vp, ci]),
+    "ext2_batch_inverse_ffi": (ci, [vp, vp, ci]),
+    "ext2_batch_mul_scalar_ffi": (ci, [u64, vp, vp, ci]),
+    "ext2_batch_neg_ffi": (ci, [vp, vp, ci]),
+    "ext2_batch_square_ffi": (ci, [vp, vp, ci]),
+    "ext2_batch_frobenius_ffi": (ci, [vp, vp, ci]),
+    "ext2_batch_conjugate_ffi": (ci, [vp, vp, ci]),
+    "ext2_batch_exp_ffi": (ci, [vp, u64, vp, ci]),
+    "gl_to_ext2_batch_ffi": (ci, [vp, vp, ci]),
+    "ext2_to_gl_batch_ffi": (ci, [vp, vp, ci]),
+    "poseidon2_hash_batch_ffi": (ci, [vp, vp, ci]),
+    "poseidon2_compress_batch_ffi": (ci, [vp, vp, vp, ci]),
+    "poseidon2_merkle_layer_ffi": (ci, [vp, vp, ci]),
+    "partial_eval_gl_ffi": (ci, [vp, vp, ci, ci]),
+    "partial_eval_ext2_from_gl_ffi": (ci, [vp, vp, vp, ci, ci]),
+    "eq_dp_all_ffi": (ci, [vp, vp, vp, ci, ctypes.POINTER(vp)]),
+    "ext2_eq_dp_all_ffi": (ci, [vp, vp, vp, ci, ctypes.POINTER(vp)]),
+    "basefold_bit_reverse_gl_ffi": (ci, [vp, ci]),
+    "basefold_bit_reverse_ext2_ffi": (ci, [vp, ci]),
+    "basefold_bhc_interpolate_ffi": (ci, [vp, vp, vp, ci]),
+    "basefold_encode_ffi": (ci, [vp, vp, ci, ci]),
+    "basefold_fold_gl_ffi": (ci, [vp, vp, u64, vp, ci]),
+    "basefold_fold_mixed_ffi": (ci, [vp, vp, u64, u64, vp, ci]),
+    "basefold_fold_ext2_ffi": (ci, [vp, vp, u64, u64, vp, ci]),
+    "basefold_sumcheck_interp_gl_ffi": (ci, [vp, ci]),
+    "basefold_sumcheck_interp_ext2_ffi": (ci, [vp, ci]),
+    "basefold_sumcheck_product_gl_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci]),
+    "basefold_sumcheck_product_mixed_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci]),
+    "basefold_sumcheck_product_ext2_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci]),
+    "basefold_sumcheck_eval_gl_ffi": (ci, [vp, u64, vp, ci]),
+    "basefold_sumcheck_eval_mixed_ffi": (ci, [vp, u64, u64, vp, ci]),
+    "basefold_sumcheck_eval_ext2_ffi": (ci, [vp, u64, u64, vp, ci]),
+    "fused_sumcheck_round_ext2_ffi": (ci, [vp, vp, u64, u64, vp, vp, vp, vp, vp, ci, ci]),
+    "basefold_dot_product_gl_ffi": (ci, [vp, vp, vp, ci, ci]),
+    "basefold_dot_product_mixed_ffi": (ci, [vp, vp, vp, ci, ci]),
+    "poseidon2_merkle_tree_gl_ffi": (ci, [vp, vp, ci]),
+    "poseidon2_merkle_tree_ext2_ffi": (ci, [vp, vp, ci]),
+    "monolith_merkle_tree_gl_ffi": (ci, [vp, vp, ci]),
+    "monolith_merkle_tree_ext2_ffi": (ci, [vp, vp, ci]),
+    "monolith_permute_test_ffi": (ci, [u64p, u64p]),
+    "sumcheck_round_message_ffi": (ci, [vp, vp, ci, csz, csz, ci]),
+    "sumcheck_fold_ffi": (ci, [vp, vp, u64, ci, csz, csz]),
+    "sumcheck_round_message_ext2_ffi": (ci, [vp, vp, ci, csz, csz, ci]),
+    "sumcheck_fold_ext2_ffi": (ci, [vp, vp, u64, u64, ci, csz, csz]),
+    # prover level (morpheus_prover.h)
+    "mam_transcript_init": (None, [trp, u8p, cl]),
+    "mam_transcript_append_u64": (None, [trp, u8p, cl, u64]),
+    "mam_transcript_append_scalars": (None, [trp, u8p, cl, u64p, cl]),
+    "mam_transcript_append_ext2": (None, [trp, u8p, cl, u64p]),
+    "mam_transcript_challenge_scalar": (u64, [trp, u8p, cl]),
+    "mam_transcript_challenge_ext2": (None, [trp, u8p, cl, u64p]),
+    "mam_poseidon2_permute_host": (None, [u64p]),
+    "mam_monolith_permute_host": (None, [u64p]),
+    "mam_sumcheck_prove": (ci, [trp, vp, ci, ci, ci, u64p, u64p, u64p, u64p]),
+    "mam_sc_state_new": (ci, [vp, ci, ci, ci, ctypes.POINTER(vp)]),
+    "mam_sc_state_round_message": (ci, [vp, u64p]),
+    "mam_sc_state_fold": (ci, [vp, u64, u64]),
+    "mam_sc_state_final_evaluations": (ci, [vp, u64p]),
+    "mam_sc_state_free": (ci, [vp]),
+    "mam_basefold_table_len": (cl, [ci, ci, ci]),
+    "mam_basefold_table_generate": (ci, [ci, ci, ci, u64, u64p]),
+    "mam_basefold_commit": (ci, [vp, ci, ci, ci, ctypes.POINTER(vp), u64p]),
+    "mam_basefold_commit_into": (ci, [vp, ci, ci, ci, vp, vp, vp, u64p]),
+    "mam_basefold_commitment_wrap": (ci, [vp, vp, vp, u64p, ci, ci, ci, ctypes.POINTER(vp)]),
+    "mam_basefold_commitment_buffers": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), u64p]),
+    "mam_basefold_commitment_free": (ci, [vp]),
+    "mam_basefold_proof_len": (cl, [ci, ci, ci]),
+    "mam_basefold_open_ext2": (ci, [vp, u64p, vp, ci, trp, ci, u64p]),
+    "mam_basefold_verify_ext2": (ci, [u64p, u64p, ci, ci, ci, u64p, ci, u64p, ci, trp]),
+    "mam_monolith_permute_batch": (ci, [vp, ci]),
+    "mam_merkle_tree_async": (ci, [vp, vp, csz, ci, ci]),
+    "mam_profile_enable": (ci, [ci]),
+    "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),
+}
+
+for _name, (_res, _args) in _SIGS.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+EXPORTED = tuple(_SIGS)

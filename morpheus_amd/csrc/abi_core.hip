@@ -1,0 +1,300 @@
+// Initialization, memory management, device selection and elementwise field batches.
+//
+// Entry points replace goldilocks-cuda-rs/src/ffi.rs:11-26, 30-53, 306-317, 159-296
+// (the `gl_batch_*`, `ext2_batch_*` and conversion symbols) with the same names,
+// arguments and return codes; implementations follow goldilocks-cuda-rs/cuda/wrapper.cu:27-97.
+#include <string.h>
+
+#include "common.h"
+
+// ---------------------------------------------------------------------------
+// Initialization (wrapper.cu:27-45). Constants live in code, so the hash inits only
+// check that a device is usable.
+// ---------------------------------------------------------------------------
+MAM_API int goldilocks_cuda_init() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return -1;
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) return -1;
+    return hipSetDevice(cur) == hipSuccess ? 0 : -1;
+}
+
+MAM_API int poseidon2_cuda_init() { return goldilocks_cuda_init(); }
+MAM_API int monolith_cuda_init() { return goldilocks_cuda_init(); }
+
+// ---------------------------------------------------------------------------
+// Memory management (wrapper.cu:51-97); copies are ordered on the per-thread stream and
+// complete before returning, as the reference's synchronous cudaMemcpy does.
+// ---------------------------------------------------------------------------
+MAM_API int cuda_malloc(void **ptr, size_t size) {
+    if (!ptr) return -1;
+    if (size == 0) size = 1;
+    return hipMalloc(ptr, size) == hipSuccess ? 0 : -1;
+}
+
+MAM_API int cuda_free(void *ptr) {
+    if (!ptr) return 0;
+    return hipFree(ptr) == hipSuccess ? 0 : -1;
+}
+
+static int copy_sync(void *dst, const void *src, size_t size, hipMemcpyKind kind) {
+    if (size == 0) return 0;
+    if (hipMemcpyAsync(dst, src, size, kind, mam_stream()) != hipSuccess) return -1;
+    return hipStreamSynchronize(mam_stream()) == hipSuccess ? 0 : -1;
+}
+
+MAM_API int cuda_memcpy_htod(void *dst, const void *src, size_t size) {
+    return copy_sync(dst, src, size, hipMemcpyHostToDevice);
+}
+MAM_API int cuda_memcpy_dtoh(void *dst, const void *src, size_t size) {
+    return copy_sync(dst, src, size, hipMemcpyDeviceToHost);
+}
+MAM_API int cuda_memcpy_dtod(void *dst, const void *src, size_t size) {
+    return copy_sync(dst, src, size, hipMemcpyDeviceToDevice);
+}
+
+MAM_API int cuda_device_synchronize() { return hipDeviceSynchronize() == hipSuccess ? 0 : -1; }
+MAM_API int cuda_get_last_error() { return (int)hipGetLastError(); }
+MAM_API int cuda_peek_at_last_error() { return (int)hipPeekAtLastError(); }
+MAM_API int cuda_mem_get_info(size_t *free_b, size_t *total_b) {
+    return hipMemGetInfo(free_b, total_b) == hipSuccess ? 0 : -1;
+}
+
+MAM_API int cuda_set_device(int device) { return hipSetDevice(device) == hipSuccess ? 0 : -1; }
+MAM_API int cuda_get_device(int *device) { return hipGetDevice(device) == hipSuccess ? 0 : -1; }
+MAM_API int cuda_get_device_count(int *count) {
+    return hipGetDeviceCount(count) == hipSuccess ? 0 : -1;
+}
+MAM_API int cuda_get_device_name(int device, char *name, int max_len) {
+    if (!name || max_len <= 0) return -1;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return -1;
+    strncpy(name, prop.name, (size_t)max_len - 1);
+    name[max_len - 1] = 0;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise batches. Inputs may be non-canonical; outputs are canonical.
+// ---------------------------------------------------------------------------
+__global__ void k_fill_zero(u64 *__restrict__ d, size_t n) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) d[i] = 0;
+}
+
+MAM_API int gl_memset_zero(u64 *d_buf, size_t n_u64) {
+    if (n_u64 == 0) return 0;
+    k_fill_zero<<<grid_1d(n_u64), MAM_BLOCK, 0, mam_stream()>>>(d_buf, n_u64);
+    return mam_launch_status();
+}
+
+MAM_API int gl_zero_buffer(u64 *d_buf, int n) { return n <= 0 ? 0 : gl_memset_zero(d_buf, (size_t)n); }
+
+enum BinOp { OP_ADD, OP_SUB, OP_MUL, OP_DIV };
+enum UnOp { OP_NEG, OP_SQR, OP_DBL, OP_INV, OP_EXP, OP_SCALE, OP_FROB };
+
+template <int OP>
+__global__ void k_gl_bin(const u64 *__restrict__ a, const u64 *__restrict__ b, u64 *__restrict__ r, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 x = gl_canon(a[i]), y = gl_canon(b[i]);
+    u64 z;
+    if (OP == OP_ADD) z = gl_add(x, y);
+    else if (OP == OP_SUB) z = gl_sub(x, y);
+    else if (OP == OP_MUL) z = gl_mul(x, y);
+    else z = gl_mul(x, gl_inv(y));
+    r[i] = z;
+}
+
+template <int OP>
+__global__ void k_gl_un(const u64 *__restrict__ a, u64 *__restrict__ r, u64 arg, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 x = gl_canon(a[i]);
+    u64 z;
+    if (OP == OP_NEG) z = gl_neg(x);
+    else if (OP == OP_SQR) z = gl_sqr(x);
+    else if (OP == OP_DBL) z = gl_double(x);
+    else if (OP == OP_INV) z = gl_inv(x);
+    else if (OP == OP_EXP) z = gl_pow(x, arg);
+    else z = gl_mul(x, gl_canon(arg));
+    r[i] = z;
+}
+
+template <int OP>
+__global__ void k_e2_bin(const u64 *__restrict__ a, const u64 *__restrict__ b, u64 *__restrict__ r, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ext2 x = e2_load(a, i), y = e2_load(b, i);
+    ext2 z;
+    if (OP == OP_ADD) z = e2_add(x, y);
+    else if (OP == OP_SUB) z = e2_sub(x, y);
+    else z = e2_mul(x, y);
+    e2_store(r, i, z);
+}
+
+template <int OP>
+__global__ void k_e2_un(const u64 *__restrict__ a, u64 *__restrict__ r, u64 arg, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ext2 x = e2_load(a, i);
+    ext2 z;
+    if (OP == OP_NEG) z = e2_neg(x);
+    else if (OP == OP_SQR) z = e2_sqr(x);
+    else if (OP == OP_INV) z = e2_inv(x);
+    else if (OP == OP_EXP) z = e2_pow(x, arg);
+    else if (OP == OP_SCALE) z = e2_scale(gl_canon(arg), x);
+    else z = e2_frobenius(x);
+    e2_store(r, i, z);
+}
+
+__global__ void k_gl_to_e2(const u64 *__restrict__ a, u64 *__restrict__ r, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    e2_store(r, i, e2_base(gl_canon(a[i])));
+}
+
+__global__ void k_e2_to_gl(const u64 *__restrict__ a, u64 *__restrict__ r, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    r[i] = gl_canon(a[2 * (size_t)i]);
+}
+
+#define GL_BIN(name, op)                                                                   \
+    MAM_API int name(const u64 *d_a, const u64 *d_b, u64 *d_result, int n) {               \
+        if (n <= 0) return 0;                                                              \
+        k_gl_bin<op><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_b, d_result, n);   \
+        return mam_launch_status();                                                        \
+    }
+#define GL_UN(name, op)                                                                    \
+    MAM_API int name(const u64 *d_a, u64 *d_result, int n) {                               \
+        if (n <= 0) return 0;                                                              \
+        k_gl_un<op><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_result, 0, n);      \
+        return mam_launch_status();                                                        \
+    }
+#define E2_BIN(name, op)                                                                   \
+    MAM_API int name(const u64 *d_a, const u64 *d_b, u64 *d_result, int n) {               \
+        if (n <= 0) return 0;                                                              \
+        k_e2_bin<op><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_b, d_result, n);   \
+        return mam_launch_status();                                                        \
+    }
+#define E2_UN(name, op)                                                                    \
+    MAM_API int name(const u64 *d_a, u64 *d_result, int n) {                               \
+        if (n <= 0) return 0;                                                              \
+        k_e2_un<op><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_result, 0, n);      \
+        return mam_launch_status();                                                        \
+    }
+
+GL_BIN(gl_batch_add, OP_ADD)
+GL_BIN(gl_batch_sub, OP_SUB)
+GL_BIN(gl_batch_mul, OP_MUL)
+GL_BIN(gl_batch_div, OP_DIV)
+GL_UN(gl_batch_inverse, OP_INV)
+GL_UN(gl_batch_neg, OP_NEG)
+GL_UN(gl_batch_square, OP_SQR)
+GL_UN(gl_batch_double, OP_DBL)
+
+MAM_API int gl_batch_exp(const u64 *d_a, u64 exp, u64 *d_result, int n) {
+    if (n <= 0) return 0;
+    k_gl_un<OP_EXP><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_result, exp, n);
+    return mam_launch_status();
+}
+MAM_API int gl_batch_mul_scalar(u64 scalar, const u64 *d_a, u64 *d_result, int n) {
+    if (n <= 0) return 0;
+    k_gl_un<OP_SCALE><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_result, scalar, n);
+    return mam_launch_status();
+}
+
+E2_BIN(ext2_batch_add_ffi, OP_ADD)
+E2_BIN(ext2_batch_sub_ffi, OP_SUB)
+E2_BIN(ext2_batch_mul_ffi, OP_MUL)
+E2_UN(ext2_batch_inverse_ffi, OP_INV)
+E2_UN(ext2_batch_neg_ffi, OP_NEG)
+E2_UN(ext2_batch_square_ffi, OP_SQR)
+E2_UN(ext2_batch_frobenius_ffi, OP_FROB)
+E2_UN(ext2_batch_conjugate_ffi, OP_FROB)
+
+MAM_API int ext2_batch_exp_ffi(const u64 *d_a, u64 exp, u64 *d_result, int n) {
+    if (n <= 0) return 0;
+    k_e2_un<OP_EXP><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_result, exp, n);
+    return mam_launch_status();
+}
+MAM_API int ext2_batch_mul_scalar_ffi(u64 scalar, const u64 *d_a, u64 *d_result, int n) {
+    if (n <= 0) return 0;
+    k_e2_un<OP_SCALE><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_a, d_result, scalar, n);
+    return mam_launch_status();
+}
+MAM_API int gl_to_ext2_batch_ffi(const u64 *d_input, u64 *d_output, int n) {
+    if (n <= 0) return 0;
+    k_gl_to_e2<<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_input, d_output, n);
+    return mam_launch_status();
+}
+MAM_API int ext2_to_gl_batch_ffi(const u64 *d_input, u64 *d_output, int n) {
+    if (n <= 0) return 0;
+    k_e2_to_gl<<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_input, d_output, n);
+    return mam_launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Opt-in kernel-group timing (profile.h)
+// ---------------------------------------------------------------------------
+#include <mutex>
+#include <vector>
+
+#include "profile.h"
+
+namespace {
+struct ProfRec {
+    int cat;
+    double work;
+    hipEvent_t a, b;
+};
+std::mutex g_prof_mu;
+std::vector<ProfRec> g_prof_recs;
+bool g_prof_on = false;
+}  // namespace
+
+bool mam_prof_enabled() { return g_prof_on; }
+
+void mam_prof_push(int cat, double work, hipEvent_t a, hipEvent_t b) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    g_prof_recs.push_back({cat, work, a, b});
+}
+
+static void prof_clear() {
+    for (auto &r : g_prof_recs) {
+        (void)hipEventDestroy(r.a);
+        (void)hipEventDestroy(r.b);
+    }
+    g_prof_recs.clear();
+}
+
+// Enable (1) or disable (0) timing; either call discards what was recorded.
+MAM_API int mam_profile_enable(int on) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    prof_clear();
+    g_prof_on = on != 0;
+    return 0;
+}
+
+// Totals per category (PROF_NCAT entries each): milliseconds, scopes, work units.
+// Categories: 0 Merkle builds (work = permutations), 1 opening fold+oracle (bytes),
+// 2 commit transform (bytes), 3 sumcheck fold+message (bytes).
+MAM_API int mam_profile_collect(double *ms, long *count, double *work) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    for (int c = 0; c < PROF_NCAT; c++) {
+        ms[c] = 0;
+        count[c] = 0;
+        work[c] = 0;
+    }
+    for (auto &r : g_prof_recs) {
+        if (hipEventSynchronize(r.b) != hipSuccess) return -1;
+        float t = 0;
+        if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) return -1;
+        ms[r.cat] += t;
+        count[r.cat] += 1;
+        work[r.cat] += r.work;
+    }
+    prof_clear();
+    return 0;
+}

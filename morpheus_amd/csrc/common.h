@@ -1,0 +1,51 @@
+// Shared launch helpers for the HIP kernels of libmorpheus_amd.
+//
+// Streams: every entry point enqueues on the calling host thread's per-thread stream
+// (hipStreamPerThread), the HIP counterpart of the reference's `--default-stream
+// per-thread` build (goldilocks-cuda-rs/build.rs:125). Entry points that the reference
+// documents as synchronizing (encode, interpolation, Merkle builds, anything returning
+// a host value) synchronize that stream before returning.
+//
+// Return convention (goldilocks-cuda-rs/cuda/wrapper.cu:27-97): 0 on success, -1 on a
+// runtime error; -2 for an argument the entry point rejects.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "field.h"
+
+#define MAM_API extern "C" __attribute__((visibility("default")))
+
+static constexpr int MAM_BLOCK = 256;
+
+static inline hipStream_t mam_stream() { return hipStreamPerThread; }
+
+static inline unsigned grid_1d(size_t n, int block = MAM_BLOCK) {
+    size_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    if (g > 0x7FFFFFFFull) g = 0x7FFFFFFFull;
+    return (unsigned)g;
+}
+
+// Grid for grid-stride reduction kernels: enough workgroups to fill 256 CUs several
+// times over, never more than the work needs.
+static inline unsigned grid_stride(size_t n, int block = MAM_BLOCK, unsigned cap = 2048) {
+    size_t g = (n + block - 1) / block;
+    if (g == 0) g = 1;
+    if (g > cap) g = cap;
+    return (unsigned)g;
+}
+
+static inline int mam_launch_status() { return hipGetLastError() == hipSuccess ? 0 : -1; }
+
+static inline int mam_sync_status() {
+    if (hipGetLastError() != hipSuccess) return -1;
+    return hipStreamSynchronize(mam_stream()) == hipSuccess ? 0 : -1;
+}
+
+#define MAM_CHECK(x)                         \
+    do {                                     \
+        if ((x) != hipSuccess) return -1;    \
+    } while (0)

@@ -1,0 +1,752 @@
+// Basefold commitment and opening kernels, eq tables and partial evaluation.
+//
+// Reference semantics:
+//   bit reversal            cuda/basefold.cuh:57-90
+//   BHC interpolation       cuda/basefold.cuh:103-147, goldilocks-cuda-rs/cuda/wrapper.cu:1242-1274
+//   encode                  cuda/basefold.cuh:157-205, wrapper.cu:1277-1312
+//   codeword folds          cuda/basefold.cuh:397-419, 522-548, 739-761
+//   opening sumcheck pieces cuda/basefold.cuh:287-383, 427-735, 966-1006
+//   eq tables               cuda/eq_lagrange.cuh:22-158 (bit i of x <-> r_i, LSB first)
+//   partial evaluation      cuda/partial_eval.cuh:31-219
+//
+// MI355X design.
+// * The interpolation (Moebius transform: hi -= lo per bit) and the encode (Walsh-Hadamard
+//   butterflies lo+hi, lo-hi per bit over the repetition code, then bit reversal) are each
+//   a tensor product of one 2x2 matrix per bit, so each runs as ceil(bits/8) passes of an
+//   LDS-tiled radix-256 transform (k_bfly_pass) instead of one strided launch per bit.
+// * Bit reversal commutes with any transform that applies the same 2x2 matrix to every bit.
+//   Hence encode(coeffs) = replicate(WHT(bitrev(coeffs))), and the whole commit codeword
+//   equals replicate(T(bitrev(evals))) with T = (lo, hi) -> (hi, 2 lo - hi) per bit
+//   (the Moebius matrix followed by the Hadamard one). The fused commit path uses that
+//   single transform on bh_evals = bitrev(evals), which the commitment keeps anyway.
+// * Bit reversal is out of place through 32x32 LDS tiles, so reads and writes are both
+//   256-byte coalesced.
+#include "basefold.h"
+#include "common.h"
+#include "profile.h"
+#include "reduce.h"
+
+// ---------------------------------------------------------------------------
+// Radix-256 butterfly transform passes
+// ---------------------------------------------------------------------------
+enum { BF_MOBIUS = 0, BF_WHT = 1, BF_T = 2 };
+
+template <int OP>
+__device__ __forceinline__ void bfly(u64 &lo, u64 &hi) {
+    if (OP == BF_MOBIUS) {
+        hi = gl_sub(hi, lo);
+    } else if (OP == BF_WHT) {
+        u64 a = gl_add(lo, hi), b = gl_sub(lo, hi);
+        lo = a;
+        hi = b;
+    } else {
+        u64 a = hi, b = gl_sub(gl_double(lo), hi);
+        lo = a;
+        hi = b;
+    }
+}
+
+static constexpr int BF_COLS = 16;                 // columns per workgroup
+static constexpr int BF_MAXKB = 8;                 // row bits per pass
+static constexpr int BF_TILE = BF_COLS << BF_MAXKB;  // 4096 words = 32 KiB of LDS
+
+// One pass over bits [b0, b0 + kb) of an array of 2^nbits words. Column c of the pass is
+// (high = c >> b0, low = c & (2^b0 - 1)); element (row, c) sits at
+// high * 2^(b0+kb) + row * 2^b0 + low. A workgroup owns 16 consecutive columns. The last
+// pass may write `reps` copies, the r-th at offset r * rep_stride.
+template <int OP>
+// src may alias dst: every element is read into LDS before any is written back.
+__global__ __launch_bounds__(256) void k_bfly_pass(const u64 *src, u64 *dst, int b0, int kb,
+                                                   int reps, size_t rep_stride) {
+    __shared__ u64 t[BF_TILE + BF_TILE / 32];
+    const int R = 1 << kb;
+    const int T = R * BF_COLS;
+    const size_t c0 = (size_t)blockIdx.x * BF_COLS;
+    // padded LDS index: one extra word every 32 breaks power-of-two strides
+#define LIDX(e) ((e) + ((e) >> 5))
+    if (b0 == 0) {
+        // columns are contiguous runs of R words: the tile is one contiguous span
+        const size_t base = c0 << kb;
+        for (int e = threadIdx.x; e < T; e += blockDim.x) t[LIDX(e)] = gl_canon(src[base + e]);
+    } else {
+        const size_t lowmask = ((size_t)1 << b0) - 1;
+        const size_t high = c0 >> b0, low0 = c0 & lowmask;
+        const size_t base = (high << (b0 + kb)) + low0;
+        for (int e = threadIdx.x; e < T; e += blockDim.x) {
+            int col = e & (BF_COLS - 1), row = e >> 4;
+            t[LIDX(row * BF_COLS + col)] = gl_canon(src[base + ((size_t)row << b0) + col]);
+        }
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; j++) {
+        for (int p = threadIdx.x; p < T / 2; p += blockDim.x) {
+            int col, rp;
+            if (b0 == 0) {
+                col = p >> (kb - 1);
+                rp = p & ((R >> 1) - 1);
+            } else {
+                col = p & (BF_COLS - 1);
+                rp = p >> 4;
+            }
+            int rlo = ((rp >> j) << (j + 1)) | (rp & ((1 << j) - 1));
+            int rhi = rlo | (1 << j);
+            int elo, ehi;
+            if (b0 == 0) {
+                elo = col * R + rlo;
+                ehi = col * R + rhi;
+            } else {
+                elo = rlo * BF_COLS + col;
+                ehi = rhi * BF_COLS + col;
+            }
+            u64 lo = t[LIDX(elo)], hi = t[LIDX(ehi)];
+            bfly<OP>(lo, hi);
+            t[LIDX(elo)] = lo;
+            t[LIDX(ehi)] = hi;
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < reps; r++) {
+        u64 *d = dst + r * rep_stride;
+        if (b0 == 0) {
+            const size_t base = c0 << kb;
+            for (int e = threadIdx.x; e < T; e += blockDim.x) d[base + e] = t[LIDX(e)];
+        } else {
+            const size_t lowmask = ((size_t)1 << b0) - 1;
+            const size_t high = c0 >> b0, low0 = c0 & lowmask;
+            const size_t base = (high << (b0 + kb)) + low0;
+            for (int e = threadIdx.x; e < T; e += blockDim.x) {
+                int col = e & (BF_COLS - 1), row = e >> 4;
+                d[base + ((size_t)row << b0) + col] = t[LIDX(row * BF_COLS + col)];
+            }
+        }
+    }
+#undef LIDX
+}
+
+__global__ void k_bfly_small(const u64 *src, u64 *dst, int nbits, int op, int reps,
+                             size_t rep_stride) {
+    // whole array (at most 2^12 words) in one workgroup
+    __shared__ u64 t[4096];
+    const int n = 1 << nbits;
+    for (int e = threadIdx.x; e < n; e += blockDim.x) t[e] = gl_canon(src[e]);
+    __syncthreads();
+    for (int j = 0; j < nbits; j++) {
+        for (int p = threadIdx.x; p < n / 2; p += blockDim.x) {
+            int lo_i = ((p >> j) << (j + 1)) | (p & ((1 << j) - 1));
+            int hi_i = lo_i | (1 << j);
+            u64 lo = t[lo_i], hi = t[hi_i];
+            if (op == BF_MOBIUS) bfly<BF_MOBIUS>(lo, hi);
+            else if (op == BF_WHT) bfly<BF_WHT>(lo, hi);
+            else bfly<BF_T>(lo, hi);
+            t[lo_i] = lo;
+            t[hi_i] = hi;
+        }
+        __syncthreads();
+    }
+    for (int r = 0; r < reps; r++)
+        for (int e = threadIdx.x; e < n; e += blockDim.x) dst[r * rep_stride + e] = t[e];
+}
+
+// Transform src (2^nbits words) into dst; the final result is also replicated `reps`
+// times at stride 2^nbits in dst. src may equal dst.
+static int bfly_transform(int op, const u64 *src, u64 *dst, int nbits, int reps, hipStream_t s) {
+    const size_t n = (size_t)1 << nbits;
+    if (nbits <= 12) {
+        k_bfly_small<<<1, 256, 0, s>>>(src, dst, nbits, op, reps, n);
+        return hipGetLastError() == hipSuccess ? 0 : -1;
+    }
+    int b0 = 0;
+    const u64 *in = src;
+    int passes = (nbits + BF_MAXKB - 1) / BF_MAXKB;
+    while (b0 < nbits) {
+        // split the bits evenly over the remaining passes (each at most 8 bits)
+        int kb = (nbits - b0 + passes - 1) / passes;
+        passes--;
+        bool last = b0 + kb == nbits;
+        unsigned grid = (unsigned)((n >> kb) / BF_COLS);
+        int r = last ? reps : 1;
+        if (op == BF_MOBIUS) k_bfly_pass<BF_MOBIUS><<<grid, 256, 0, s>>>(in, dst, b0, kb, r, n);
+        else if (op == BF_WHT) k_bfly_pass<BF_WHT><<<grid, 256, 0, s>>>(in, dst, b0, kb, r, n);
+        else k_bfly_pass<BF_T><<<grid, 256, 0, s>>>(in, dst, b0, kb, r, n);
+        if (hipGetLastError() != hipSuccess) return -1;
+        in = dst;
+        b0 += kb;
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Bit reversal (out of place), W words per element
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u32 brev_bits(u32 x, int bits) { return bits == 0 ? 0 : (__brev(x) >> (32 - bits)); }
+
+template <int W>
+__global__ __launch_bounds__(256) void k_bitrev_tiled(const u64 *__restrict__ src, u64 *__restrict__ dst, int nb) {
+    // element index = a (top 5 bits) | m (middle nb - 10 bits) | b (low 5 bits)
+    __shared__ u64 tile[32][33 * W];
+    const u32 m = blockIdx.x;
+    const int mb = nb - 10;
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        u32 a = e >> 5, b = e & 31;
+        size_t i = ((size_t)a << (nb - 5)) | ((size_t)m << 5) | b;
+#pragma unroll
+        for (int w = 0; w < W; w++) tile[a][b * W + w] = gl_canon(src[i * W + w]);
+    }
+    __syncthreads();
+    const u32 rm = brev_bits(m, mb);
+    for (int e = threadIdx.x; e < 1024; e += 256) {
+        u32 b = e >> 5, ap = e & 31;
+        u32 a = brev_bits(ap, 5);
+        size_t o = ((size_t)brev_bits(b, 5) << (nb - 5)) | ((size_t)rm << 5) | ap;
+#pragma unroll
+        for (int w = 0; w < W; w++) dst[o * W + w] = tile[a][b * W + w];
+    }
+}
+
+template <int W>
+__global__ void k_bitrev_simple(const u64 *__restrict__ src, u64 *__restrict__ dst, int nb) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= ((size_t)1 << nb)) return;
+    size_t r = nb == 0 ? 0 : (size_t)(__brevll((unsigned long long)i) >> (64 - nb));
+#pragma unroll
+    for (int w = 0; w < W; w++) dst[r * W + w] = gl_canon(src[i * W + w]);
+}
+
+// Out-of-place bit reversal; src and dst must not overlap.
+int mam_bitrev(const u64 *src, u64 *dst, int nb, int words, hipStream_t s) {
+    size_t n = (size_t)1 << nb;
+    if (nb >= 10) {
+        unsigned grid = (unsigned)(n >> 10);
+        if (words == 1) k_bitrev_tiled<1><<<grid, 256, 0, s>>>(src, dst, nb);
+        else k_bitrev_tiled<2><<<grid, 256, 0, s>>>(src, dst, nb);
+    } else {
+        if (words == 1) k_bitrev_simple<1><<<grid_1d(n), 256, 0, s>>>(src, dst, nb);
+        else k_bitrev_simple<2><<<grid_1d(n), 256, 0, s>>>(src, dst, nb);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// In-place bit reversal through a temporary (reference entry points are in place).
+static int bitrev_inplace(u64 *data, int nb, int words, hipStream_t s) {
+    size_t bytes = ((size_t)words << nb) * sizeof(u64);
+    u64 *tmp = nullptr;
+    if (hipMallocAsync((void **)&tmp, bytes, s) != hipSuccess) return -1;
+    int rc = mam_bitrev(data, tmp, nb, words, s);
+    if (!rc && hipMemcpyAsync(data, tmp, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) rc = -1;
+    hipFreeAsync(tmp, s);
+    return rc;
+}
+
+MAM_API int basefold_bit_reverse_gl_ffi(u64 *d_data, int log_n) {
+    if (log_n < 0 || log_n > 40) return -1;
+    if (bitrev_inplace(d_data, log_n, 1, mam_stream())) return -1;
+    return mam_launch_status();
+}
+MAM_API int basefold_bit_reverse_ext2_ffi(u64 *d_data, int log_n) {
+    if (log_n < 0 || log_n > 40) return -1;
+    if (bitrev_inplace(d_data, log_n, 2, mam_stream())) return -1;
+    return mam_launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Commit: interpolation and encode (reference-named), and the fused commit transform
+// ---------------------------------------------------------------------------
+
+// coeffs = Moebius(evals) (natural order); bh_evals = bitrev(evals).
+MAM_API int basefold_bhc_interpolate_ffi(const u64 *d_evals, u64 *d_coeffs, u64 *d_bh_evals, int num_vars) {
+    if (num_vars < 1 || num_vars > 34) return -1;
+    hipStream_t s = mam_stream();
+    if (bfly_transform(BF_MOBIUS, d_evals, d_coeffs, num_vars, 1, s)) return -1;
+    if (mam_bitrev(d_evals, d_bh_evals, num_vars, 1, s)) return -1;
+    return mam_sync_status();
+}
+
+// codeword = replicate(WHT(bitrev(coeffs))), 2^log_rate copies of 2^num_vars words.
+MAM_API int basefold_encode_ffi(const u64 *d_coeffs, u64 *d_codeword, int num_vars, int log_rate) {
+    if (num_vars < 1 || log_rate < 0 || num_vars + log_rate > 36) return -1;
+    hipStream_t s = mam_stream();
+    if (mam_bitrev(d_coeffs, d_codeword, num_vars, 1, s)) return -1;
+    if (bfly_transform(BF_WHT, d_codeword, d_codeword, num_vars, 1 << log_rate, s)) return -1;
+    return mam_sync_status();
+}
+
+// Fused commit codeword: bh_evals = bitrev(evals); codeword = replicate(T(bh_evals)).
+int mam_commit_codeword(const u64 *d_evals, u64 *d_bh_evals, u64 *d_codeword, int nv, int lr, hipStream_t s) {
+    // algorithmic bytes: read evals, write bh_evals, write 2^lr codeword copies
+    ProfScope prof(PROF_COMMIT_TRANSFORM, 8.0 * (double)((size_t)1 << nv) * (2.0 + (double)(1 << lr)), s);
+    if (mam_bitrev(d_evals, d_bh_evals, nv, 1, s)) return -1;
+    return bfly_transform(BF_T, d_bh_evals, d_codeword, nv, 1 << lr, s);
+}
+
+// ---------------------------------------------------------------------------
+// Codeword folds: out[i] = v0 + (alpha - x0_i) * (v1 - v0) * w_i
+// ---------------------------------------------------------------------------
+template <int KIND>  // 0: base cw, base alpha; 1: base cw, ext2 alpha; 2: ext2 cw, ext2 alpha
+__global__ void k_cw_fold(const u64 *__restrict__ cw, const u64 *__restrict__ table, ext2 alpha,
+                          u64 *__restrict__ out, size_t pairs) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= pairs) return;
+    const u64 x0 = gl_canon(table[2 * i]), w = gl_canon(table[2 * i + 1]);
+    if (KIND == 0) {
+        u64 v0 = gl_canon(cw[2 * i]), v1 = gl_canon(cw[2 * i + 1]);
+        out[i] = gl_add(v0, gl_mul(gl_mul(gl_sub(alpha.c0, x0), gl_sub(v1, v0)), w));
+    } else {
+        ext2 v0, v1;
+        if (KIND == 1) {
+            v0 = e2_base(gl_canon(cw[2 * i]));
+            v1 = e2_base(gl_canon(cw[2 * i + 1]));
+        } else {
+            v0 = e2_load(cw, 2 * i);
+            v1 = e2_load(cw, 2 * i + 1);
+        }
+        ext2 cx = e2_sub(alpha, e2_base(x0));
+        e2_store(out, i, mam_fold_value(v0, v1, cx, w));
+    }
+}
+
+MAM_API int basefold_fold_gl_ffi(const u64 *d_codeword, const u64 *d_table, u64 challenge, u64 *d_output,
+                                 int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_cw_fold<0><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_codeword, d_table,
+                                                                       e2_base(gl_canon(challenge)), d_output,
+                                                                       (size_t)pair_count);
+    return mam_launch_status();
+}
+MAM_API int basefold_fold_mixed_ffi(const u64 *d_codeword, const u64 *d_table, u64 c0, u64 c1, u64 *d_output,
+                                    int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_cw_fold<1><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_codeword, d_table,
+                                                                       e2(gl_canon(c0), gl_canon(c1)), d_output,
+                                                                       (size_t)pair_count);
+    return mam_launch_status();
+}
+MAM_API int basefold_fold_ext2_ffi(const u64 *d_codeword, const u64 *d_table, u64 c0, u64 c1, u64 *d_output,
+                                   int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_cw_fold<2><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_codeword, d_table,
+                                                                       e2(gl_canon(c0), gl_canon(c1)), d_output,
+                                                                       (size_t)pair_count);
+    return mam_launch_status();
+}
+
+int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s) {
+    if (base_cw)
+        k_cw_fold<1><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs);
+    else
+        k_cw_fold<2><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------
+// Opening-sumcheck pieces (reference-named; partial sums per workgroup)
+// ---------------------------------------------------------------------------
+template <bool EXT>
+__global__ void k_interp(u64 *__restrict__ d, size_t pairs) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= pairs) return;
+    if (EXT) {
+        ext2 a = e2_load(d, 2 * i), b = e2_load(d, 2 * i + 1);
+        e2_store(d, 2 * i, a);
+        e2_store(d, 2 * i + 1, e2_sub(b, a));
+    } else {
+        u64 a = gl_canon(d[2 * i]), b = gl_canon(d[2 * i + 1]);
+        d[2 * i] = a;
+        d[2 * i + 1] = gl_sub(b, a);
+    }
+}
+
+template <int KIND>  // 0: base data, base challenge; 1: base data, ext2 challenge; 2: ext2 data
+__global__ void k_eval(const u64 *__restrict__ d, ext2 ch, u64 *__restrict__ out, size_t pairs) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= pairs) return;
+    if (KIND == 0) {
+        out[i] = gl_add(gl_canon(d[2 * i]), gl_mul(ch.c0, gl_canon(d[2 * i + 1])));
+    } else if (KIND == 1) {
+        u64 c = gl_canon(d[2 * i]), df = gl_canon(d[2 * i + 1]);
+        e2_store(out, i, e2_add(e2_base(c), e2_scale(df, ch)));
+    } else {
+        e2_store(out, i, e2_add(e2_load(d, 2 * i), e2_mul(ch, e2_load(d, 2 * i + 1))));
+    }
+}
+
+// eq and bh in coefficient form (value, difference): c0 += e0 b0, c1 += e0 b1 + e1 b0, c2 += e1 b1
+template <int KIND>  // 0: both base; 1: eq ext2, bh base; 2: both ext2
+__global__ __launch_bounds__(MAM_BLOCK) void k_product(const u64 *__restrict__ eq, const u64 *__restrict__ bh,
+                                                       u64 *pc0, u64 *pc1, u64 *pc2, size_t pairs) {
+    __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
+    u64 acc[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < pairs; i += (size_t)gridDim.x * blockDim.x) {
+        ext2 e0, e1, b0, b1;
+        if (KIND == 0) {
+            e0 = e2_base(gl_canon(eq[2 * i]));
+            e1 = e2_base(gl_canon(eq[2 * i + 1]));
+        } else {
+            e0 = e2_load(eq, 2 * i);
+            e1 = e2_load(eq, 2 * i + 1);
+        }
+        if (KIND == 2) {
+            b0 = e2_load(bh, 2 * i);
+            b1 = e2_load(bh, 2 * i + 1);
+        } else {
+            b0 = e2_base(gl_canon(bh[2 * i]));
+            b1 = e2_base(gl_canon(bh[2 * i + 1]));
+        }
+        mam_oracle_accumulate(acc, e0, e1, b0, b1);
+    }
+    block_sum<6>(acc, lds);
+    if (threadIdx.x == 0) {
+        if (KIND == 0) {
+            pc0[blockIdx.x] = acc[0];
+            pc1[blockIdx.x] = acc[2];
+            pc2[blockIdx.x] = acc[4];
+        } else {
+            pc0[2 * blockIdx.x] = acc[0];
+            pc0[2 * blockIdx.x + 1] = acc[1];
+            pc1[2 * blockIdx.x] = acc[2];
+            pc1[2 * blockIdx.x + 1] = acc[3];
+            pc2[2 * blockIdx.x] = acc[4];
+            pc2[2 * blockIdx.x + 1] = acc[5];
+        }
+    }
+}
+
+// Fused eval + interp + product over 4*pairs inputs (cuda/basefold.cuh:654-735).
+__global__ __launch_bounds__(MAM_BLOCK) void k_fused_round_ref(const u64 *__restrict__ eq_in,
+                                                               const u64 *__restrict__ bh_in, ext2 ch,
+                                                               u64 *__restrict__ eq_out, u64 *__restrict__ bh_out,
+                                                               u64 *pc0, u64 *pc1, u64 *pc2, size_t pairs) {
+    __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
+    u64 acc[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < pairs; i += (size_t)gridDim.x * blockDim.x) {
+        ext2 ea = e2_add(e2_load(eq_in, 4 * i), e2_mul(ch, e2_load(eq_in, 4 * i + 1)));
+        ext2 eb = e2_add(e2_load(eq_in, 4 * i + 2), e2_mul(ch, e2_load(eq_in, 4 * i + 3)));
+        ext2 ba = e2_add(e2_load(bh_in, 4 * i), e2_mul(ch, e2_load(bh_in, 4 * i + 1)));
+        ext2 bb = e2_add(e2_load(bh_in, 4 * i + 2), e2_mul(ch, e2_load(bh_in, 4 * i + 3)));
+        ext2 e1 = e2_sub(eb, ea), b1 = e2_sub(bb, ba);
+        mam_oracle_accumulate(acc, ea, e1, ba, b1);
+        e2_store(eq_out, 2 * i, ea);
+        e2_store(eq_out, 2 * i + 1, e1);
+        e2_store(bh_out, 2 * i, ba);
+        e2_store(bh_out, 2 * i + 1, b1);
+    }
+    block_sum<6>(acc, lds);
+    if (threadIdx.x == 0) {
+        pc0[2 * blockIdx.x] = acc[0];
+        pc0[2 * blockIdx.x + 1] = acc[1];
+        pc1[2 * blockIdx.x] = acc[2];
+        pc1[2 * blockIdx.x + 1] = acc[3];
+        pc2[2 * blockIdx.x] = acc[4];
+        pc2[2 * blockIdx.x + 1] = acc[5];
+    }
+}
+
+template <bool MIXED>
+__global__ __launch_bounds__(MAM_BLOCK) void k_dot(const u64 *__restrict__ a, const u64 *__restrict__ b,
+                                                   u64 *partial, size_t n) {
+    __shared__ u64 lds[2 * (MAM_BLOCK / WAVE)];
+    u64 acc[2] = {0, 0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        if (MIXED) {
+            ext2 p = e2_scale(gl_canon(a[i]), e2_load(b, i));
+            acc[0] = gl_add(acc[0], p.c0);
+            acc[1] = gl_add(acc[1], p.c1);
+        } else {
+            acc[0] = gl_add(acc[0], gl_mul(gl_canon(a[i]), gl_canon(b[i])));
+        }
+    }
+    block_sum<2>(acc, lds);
+    if (threadIdx.x == 0) {
+        if (MIXED) {
+            partial[2 * blockIdx.x] = acc[0];
+            partial[2 * blockIdx.x + 1] = acc[1];
+        } else {
+            partial[blockIdx.x] = acc[0];
+        }
+    }
+}
+
+MAM_API int basefold_sumcheck_interp_gl_ffi(u64 *d_data, int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_interp<false><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_data, (size_t)pair_count);
+    return mam_launch_status();
+}
+MAM_API int basefold_sumcheck_interp_ext2_ffi(u64 *d_data, int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_interp<true><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_data, (size_t)pair_count);
+    return mam_launch_status();
+}
+MAM_API int basefold_sumcheck_eval_gl_ffi(const u64 *d_data, u64 challenge, u64 *d_output, int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_eval<0><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_data, e2_base(gl_canon(challenge)), d_output,
+                                                                     (size_t)pair_count);
+    return mam_launch_status();
+}
+MAM_API int basefold_sumcheck_eval_mixed_ffi(const u64 *d_data, u64 c0, u64 c1, u64 *d_output, int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_eval<1><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_data, e2(gl_canon(c0), gl_canon(c1)), d_output,
+                                                                     (size_t)pair_count);
+    return mam_launch_status();
+}
+MAM_API int basefold_sumcheck_eval_ext2_ffi(const u64 *d_data, u64 c0, u64 c1, u64 *d_output, int pair_count) {
+    if (pair_count <= 0) return 0;
+    k_eval<2><<<grid_1d(pair_count), MAM_BLOCK, 0, mam_stream()>>>(d_data, e2(gl_canon(c0), gl_canon(c1)), d_output,
+                                                                     (size_t)pair_count);
+    return mam_launch_status();
+}
+
+#define PRODUCT_FFI(name, kind)                                                                              \
+    MAM_API int name(const u64 *d_eq, const u64 *d_bh, u64 *pc0, u64 *pc1, u64 *pc2, int pair_count,        \
+                     int num_blocks) {                                                                       \
+        if (num_blocks <= 0) return -1;                                                                      \
+        k_product<kind><<<num_blocks, MAM_BLOCK, 0, mam_stream()>>>(d_eq, d_bh, pc0, pc1, pc2,              \
+                                                                    pair_count > 0 ? (size_t)pair_count : 0); \
+        return mam_launch_status();                                                                          \
+    }
+PRODUCT_FFI(basefold_sumcheck_product_gl_ffi, 0)
+PRODUCT_FFI(basefold_sumcheck_product_mixed_ffi, 1)
+PRODUCT_FFI(basefold_sumcheck_product_ext2_ffi, 2)
+
+MAM_API int fused_sumcheck_round_ext2_ffi(const u64 *d_eq_in, const u64 *d_bh_in, u64 c0, u64 c1, u64 *d_eq_out,
+                                          u64 *d_bh_out, u64 *pc0, u64 *pc1, u64 *pc2, int pair_count,
+                                          int num_blocks) {
+    if (num_blocks <= 0) return -1;
+    k_fused_round_ref<<<num_blocks, MAM_BLOCK, 0, mam_stream()>>>(d_eq_in, d_bh_in, e2(gl_canon(c0), gl_canon(c1)),
+                                                                  d_eq_out, d_bh_out, pc0, pc1, pc2,
+                                                                  pair_count > 0 ? (size_t)pair_count : 0);
+    return mam_launch_status();
+}
+MAM_API int basefold_dot_product_gl_ffi(const u64 *d_a, const u64 *d_b, u64 *d_partial, int n, int num_blocks) {
+    if (num_blocks <= 0) return -1;
+    k_dot<false><<<num_blocks, MAM_BLOCK, 0, mam_stream()>>>(d_a, d_b, d_partial, n > 0 ? (size_t)n : 0);
+    return mam_launch_status();
+}
+MAM_API int basefold_dot_product_mixed_ffi(const u64 *d_a, const u64 *d_b, u64 *d_partial, int n, int num_blocks) {
+    if (num_blocks <= 0) return -1;
+    k_dot<true><<<num_blocks, MAM_BLOCK, 0, mam_stream()>>>(d_a, d_b, d_partial, n > 0 ? (size_t)n : 0);
+    return mam_launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// eq tables in one pass: a workgroup covers 2^EQ_LOW consecutive indices; it builds the
+// table of the low variables in LDS by doubling, multiplies its one high-variable factor
+// in, and streams the result out once.
+// ---------------------------------------------------------------------------
+static constexpr int EQ_LOW = 10;
+
+template <bool EXT>
+__global__ __launch_bounds__(256) void k_eq_table(const u64 *__restrict__ r, int nv, u64 *__restrict__ out) {
+    __shared__ u64 lt[2 << EQ_LOW];
+    __shared__ u64 hf[2];
+    const int low = nv < EQ_LOW ? nv : EQ_LOW;
+    const int L = 1 << low;
+    auto ld = [&](int i) { return EXT ? e2_load(r, i) : e2_base(gl_canon(r[i])); };
+    if (threadIdx.x == 0) {
+        // factor of the high variables for this workgroup's index range
+        ext2 h = e2_one();
+        size_t hb = blockIdx.x;
+        for (int i = low; i < nv; i++) {
+            ext2 ri = ld(i);
+            h = e2_mul(h, ((hb >> (i - low)) & 1) ? ri : e2_sub(e2_one(), ri));
+        }
+        hf[0] = h.c0;
+        hf[1] = h.c1;
+        lt[0] = 1;
+        lt[1] = 0;
+    }
+    __syncthreads();
+    for (int i = 0; i < low; i++) {
+        const int half = 1 << i;
+        ext2 ri = ld(i);
+        for (int j = threadIdx.x; j < half; j += blockDim.x) {
+            ext2 a = e2(lt[2 * j], lt[2 * j + 1]);
+            ext2 ar = e2_mul(a, ri);
+            ext2 lo = e2_sub(a, ar);
+            // entry j is read and rewritten only by the thread that owns j
+            lt[2 * (j + half)] = ar.c0;
+            lt[2 * (j + half) + 1] = ar.c1;
+            lt[2 * j] = lo.c0;
+            lt[2 * j + 1] = lo.c1;
+        }
+        __syncthreads();
+    }
+    const ext2 h = e2(hf[0], hf[1]);
+    const size_t base = (size_t)blockIdx.x << low;
+    for (int j = threadIdx.x; j < L; j += blockDim.x) {
+        ext2 v = e2_mul(e2(lt[2 * j], lt[2 * j + 1]), h);
+        if (EXT) e2_store(out, base + j, v);
+        else out[base + j] = v.c0;
+    }
+}
+
+int mam_eq_table(const u64 *d_r, int nv, u64 *d_out, bool ext, hipStream_t s) {
+    if (nv < 0 || nv > 40) return -2;
+    const int low = nv < EQ_LOW ? nv : EQ_LOW;
+    unsigned grid = (unsigned)((size_t)1 << (nv - low));
+    if (ext) k_eq_table<true><<<grid, 256, 0, s>>>(d_r, nv, d_out);
+    else k_eq_table<false><<<grid, 256, 0, s>>>(d_r, nv, d_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+MAM_API int eq_dp_all_ffi(const u64 *d_r, u64 *d_buf_a, u64 *d_buf_b, int log_n, u64 **d_result) {
+    (void)d_buf_b;
+    if (mam_eq_table(d_r, log_n, d_buf_a, false, mam_stream())) return -1;
+    if (d_result) *d_result = d_buf_a;
+    return mam_launch_status();
+}
+MAM_API int ext2_eq_dp_all_ffi(const u64 *d_r, u64 *d_buf_a, u64 *d_buf_b, int log_n, u64 **d_result) {
+    (void)d_buf_b;
+    if (mam_eq_table(d_r, log_n, d_buf_a, true, mam_stream())) return -1;
+    if (d_result) *d_result = d_buf_a;
+    return mam_launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Partial evaluation: fold the first m variables (LSB first)
+// ---------------------------------------------------------------------------
+template <int KIND>  // 0: base->base (base r); 1: base->ext2 (ext2 r); 2: ext2->ext2
+__global__ void k_peval_layer(const u64 *__restrict__ in, u64 *__restrict__ out, const u64 *__restrict__ r, int li,
+                              size_t pairs) {
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i >= pairs) return;
+    if (KIND == 0) {
+        u64 ri = gl_canon(r[li]);
+        u64 a = gl_canon(in[2 * i]), b = gl_canon(in[2 * i + 1]);
+        out[i] = gl_add(a, gl_mul(ri, gl_sub(b, a)));
+    } else if (KIND == 1) {
+        ext2 ri = e2_load(r, li);
+        u64 a = gl_canon(in[2 * i]), b = gl_canon(in[2 * i + 1]);
+        e2_store(out, i, e2_add(e2_base(a), e2_scale(gl_sub(b, a), ri)));
+    } else {
+        ext2 ri = e2_load(r, li);
+        ext2 a = e2_load(in, 2 * i), b = e2_load(in, 2 * i + 1);
+        e2_store(out, i, e2_add(a, e2_mul(ri, e2_sub(b, a))));
+    }
+}
+
+MAM_API int partial_eval_gl_ffi(u64 *d_data, const u64 *d_r, int log_n, int m) {
+    if (m <= 0) return 0;
+    if (m > log_n) return -1;
+    hipStream_t s = mam_stream();
+    size_t cur = (size_t)1 << log_n;
+    u64 *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, (cur / 2) * sizeof(u64), s) != hipSuccess) return -1;
+    u64 *src = d_data, *dst = scratch;
+    int rc = 0;
+    for (int i = 0; i < m && !rc; i++) {
+        size_t pairs = cur / 2;
+        k_peval_layer<0><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(src, dst, d_r, i, pairs);
+        if (hipGetLastError() != hipSuccess) rc = -1;
+        u64 *t = src;
+        src = dst;
+        dst = t;
+        cur = pairs;
+    }
+    if (!rc && src != d_data &&
+        hipMemcpyAsync(d_data, src, cur * sizeof(u64), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = -1;
+    hipFreeAsync(scratch, s);
+    return rc ? rc : mam_sync_status();
+}
+
+MAM_API int partial_eval_ext2_from_gl_ffi(const u64 *d_input, u64 *d_output, const u64 *d_r, int log_n, int m) {
+    if (m <= 0) return 0;
+    if (m > log_n) return -1;
+    hipStream_t s = mam_stream();
+    size_t cur = (size_t)1 << log_n;
+    size_t pairs = cur / 2;
+    k_peval_layer<1><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(d_input, d_output, d_r, 0, pairs);
+    if (hipGetLastError() != hipSuccess) return -1;
+    cur = pairs;
+    if (m == 1) return mam_sync_status();
+    u64 *scratch = nullptr;
+    if (hipMallocAsync((void **)&scratch, (cur / 2 > 0 ? cur / 2 : 1) * 2 * sizeof(u64), s) != hipSuccess) return -1;
+    u64 *src = d_output, *dst = scratch;
+    int rc = 0;
+    for (int i = 1; i < m && !rc; i++) {
+        pairs = cur / 2;
+        k_peval_layer<2><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(src, dst, d_r, i, pairs);
+        if (hipGetLastError() != hipSuccess) rc = -1;
+        u64 *t = src;
+        src = dst;
+        dst = t;
+        cur = pairs;
+    }
+    if (!rc && src != d_output &&
+        hipMemcpyAsync(d_output, src, cur * 2 * sizeof(u64), hipMemcpyDeviceToDevice, s) != hipSuccess)
+        rc = -1;
+    hipFreeAsync(scratch, s);
+    return rc ? rc : mam_sync_status();
+}
+
+// ---------------------------------------------------------------------------
+// Opening path kernels (fused): oracle of round 0 and fold + next oracle
+// ---------------------------------------------------------------------------
+
+// Round 0: eq (Ext2) and bh (base), natural order; oracle coefficients reduced in-launch.
+__global__ __launch_bounds__(MAM_BLOCK) void k_open_oracle0(const u64 *__restrict__ eq, const u64 *__restrict__ bh,
+                                                            size_t pairs, u64 *partials, u32 *counter, u64 *result) {
+    __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
+    u64 acc[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < pairs; i += (size_t)gridDim.x * blockDim.x) {
+        ext2 e0 = e2_load(eq, 2 * i), ehi = e2_load(eq, 2 * i + 1);
+        u64 b0 = gl_canon(bh[2 * i]), bhi = gl_canon(bh[2 * i + 1]);
+        mam_oracle_accumulate(acc, e0, e2_sub(ehi, e0), e2_base(b0), e2_base(gl_sub(bhi, b0)));
+    }
+    block_sum<6>(acc, lds);
+    grid_finish<6>(acc, lds, partials, counter, result);
+}
+
+// Fold eq (Ext2) and bh (base or Ext2) at ch: each output pair y comes from inputs
+// 4y..4y+3; writes the folded arrays and reduces the next oracle in-launch.
+template <bool BH_BASE>
+__global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(const u64 *__restrict__ eq, const u64 *__restrict__ bh,
+                                                                ext2 ch, u64 *__restrict__ eq_out,
+                                                                u64 *__restrict__ bh_out, size_t half_out,
+                                                                u64 *partials, u32 *counter, u64 *result) {
+    __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
+    u64 acc[6] = {0, 0, 0, 0, 0, 0};
+    for (size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x; y < half_out; y += (size_t)gridDim.x * blockDim.x) {
+        ext2 q0 = e2_load(eq, 4 * y), q1 = e2_load(eq, 4 * y + 1), q2 = e2_load(eq, 4 * y + 2),
+             q3 = e2_load(eq, 4 * y + 3);
+        ext2 ea = e2_add(q0, e2_mul(ch, e2_sub(q1, q0)));
+        ext2 eb = e2_add(q2, e2_mul(ch, e2_sub(q3, q2)));
+        ext2 ba, bb;
+        if (BH_BASE) {
+            u64 a0 = gl_canon(bh[4 * y]), a1 = gl_canon(bh[4 * y + 1]);
+            u64 a2 = gl_canon(bh[4 * y + 2]), a3 = gl_canon(bh[4 * y + 3]);
+            ba = e2_add(e2_base(a0), e2_scale(gl_sub(a1, a0), ch));
+            bb = e2_add(e2_base(a2), e2_scale(gl_sub(a3, a2), ch));
+        } else {
+            ext2 b0 = e2_load(bh, 4 * y), b1 = e2_load(bh, 4 * y + 1), b2 = e2_load(bh, 4 * y + 2),
+                 b3 = e2_load(bh, 4 * y + 3);
+            ba = e2_add(b0, e2_mul(ch, e2_sub(b1, b0)));
+            bb = e2_add(b2, e2_mul(ch, e2_sub(b3, b2)));
+        }
+        e2_store(eq_out, 2 * y, ea);
+        e2_store(eq_out, 2 * y + 1, eb);
+        e2_store(bh_out, 2 * y, ba);
+        e2_store(bh_out, 2 * y + 1, bb);
+        mam_oracle_accumulate(acc, ea, e2_sub(eb, ea), ba, e2_sub(bb, ba));
+    }
+    block_sum<6>(acc, lds);
+    grid_finish<6>(acc, lds, partials, counter, result);
+}
+
+int mam_open_oracle0(const u64 *eq, const u64 *bh, size_t pairs, const RedScratch &rs, hipStream_t s) {
+    unsigned g = grid_stride(pairs, MAM_BLOCK, RED_BLOCKS);
+    k_open_oracle0<<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.counter, rs.result);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,
+                         size_t half_out, const RedScratch &rs, hipStream_t s) {
+    unsigned g = grid_stride(half_out, MAM_BLOCK, RED_BLOCKS);
+    ProfScope prof(PROF_OPEN_SUMCHECK, (double)half_out * (64.0 + (bh_base ? 32.0 : 64.0) + 128.0), s);
+    if (bh_base)
+        k_open_fold_oracle<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, ch, eq_out, bh_out, half_out, rs.partials, rs.counter,
+                                                         rs.result);
+    else
+        k_open_fold_oracle<false><<<g, MAM_BLOCK, 0, s>>>(eq, bh, ch, eq_out, bh_out, half_out, rs.partials,
+                                                          rs.counter, rs.result);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -1,0 +1,290 @@
+// Merkle trees over Basefold codewords with Monolith-w12 (the reference default) or
+// Poseidon2-w8, and the Poseidon2 batch primitives.
+//
+// Tree layout (goldilocks-cuda-rs/src/merkle.rs:1-25): one buffer holding
+// [level 0: N digests][level 1: N/2]...[root], 4 u64 per digest, (2N - 1) * 4 u64 in all.
+// Leaf i of a base-field codeword hashes (cw[2i], cw[2i+1]); of an Ext2 codeword the four
+// words cw[4i..4i+4) (wrapper.cu:1604-1648, 1909-1932). Node = compress(left, right).
+//
+// MI355X design: the reference launches one kernel and synchronizes per tree level
+// (wrapper.cu:1653-1672, 1955-1988). Here one launch hashes a 256-wide slab of a level and
+// then keeps compressing inside LDS while every wave of the workgroup is still full
+// (256 -> 128 -> 64 nodes), so a 2^22-leaf tree is built by 8 launches with no host
+// synchronization in between, and each digest is written to HBM exactly once.
+#include "basefold.h"
+#include "common.h"
+#include "merkle.h"
+#include "monolith.h"
+#include "poseidon2.h"
+#include "profile.h"
+
+struct MonoHash {
+    static MAM_HD void gl_leaf(u64 a, u64 b, u64 *o) { mono::hash_gl_leaf(a, b, o); }
+    static MAM_HD void ext2_leaf(const u64 *ab, u64 *o) { mono::hash_ext2_leaf(ab, o); }
+    static MAM_HD void node(const u64 *l, const u64 *r, u64 *o) { mono::compress(l, r, o); }
+};
+struct P2Hash {
+    static MAM_HD void gl_leaf(u64 a, u64 b, u64 *o) { p2::hash_gl_leaf(a, b, o); }
+    static MAM_HD void ext2_leaf(const u64 *ab, u64 *o) { p2::hash_ext2_leaf(ab, o); }
+    static MAM_HD void node(const u64 *l, const u64 *r, u64 *o) { p2::compress(l, r, o); }
+};
+
+static constexpr int TREE_BLOCK = 256;
+
+// Source kinds for the first level a launch writes: codeword leaves (base or Ext2),
+// children nodes, or leaves of a codeword folded on the fly (base or Ext2 input).
+enum { SRC_GL = 0, SRC_EXT2 = 1, SRC_NODES = 2, SRC_FOLD_GL = 3, SRC_FOLD_EXT2 = 4 };
+
+struct FoldSrc {
+    const u64 *table;  // (x0, w) pairs of this fold level
+    ext2 alpha;
+    u64 *out;          // folded codeword (Ext2)
+};
+
+// Writes `nlev` consecutive levels. The first written level has `n_first` nodes and
+// starts at digest index `first_off` of `tree`; each next level follows the previous one.
+template <class H, int SRC>
+__global__ __launch_bounds__(TREE_BLOCK) void k_tree_levels(const u64 *__restrict__ src, u64 *__restrict__ tree,
+                                                            size_t first_off, size_t n_first, int nlev, FoldSrc fs) {
+    __shared__ u64 sh[TREE_BLOCK * 4];
+    const int t = threadIdx.x;
+    const size_t i = (size_t)blockIdx.x * TREE_BLOCK + t;
+    u64 d[4];
+    if (i < n_first) {
+        if (SRC == SRC_GL) {
+            H::gl_leaf(src[2 * i], src[2 * i + 1], d);
+        } else if (SRC == SRC_EXT2) {
+            u64 ab[4] = {src[4 * i], src[4 * i + 1], src[4 * i + 2], src[4 * i + 3]};
+            H::ext2_leaf(ab, d);
+        } else if (SRC == SRC_NODES) {
+            H::node(src + 8 * i, src + 8 * i + 4, d);
+        } else {
+            // folded elements 2i and 2i+1 come from input pairs (4i, 4i+1) and (4i+2, 4i+3)
+            ext2 f[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const size_t k = 2 * i + h;
+                ext2 v0, v1;
+                if (SRC == SRC_FOLD_GL) {
+                    v0 = e2_base(gl_canon(src[2 * k]));
+                    v1 = e2_base(gl_canon(src[2 * k + 1]));
+                } else {
+                    v0 = e2_load(src, 2 * k);
+                    v1 = e2_load(src, 2 * k + 1);
+                }
+                const u64 x0 = gl_canon(fs.table[2 * k]), w = gl_canon(fs.table[2 * k + 1]);
+                f[h] = mam_fold_value(v0, v1, e2_sub(fs.alpha, e2_base(x0)), w);
+                e2_store(fs.out, k, f[h]);
+            }
+            u64 ab[4] = {f[0].c0, f[0].c1, f[1].c0, f[1].c1};
+            H::ext2_leaf(ab, d);
+        }
+        u64 *o = tree + 4 * (first_off + i);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            o[k] = d[k];
+            sh[4 * t + k] = d[k];
+        }
+    }
+    size_t level_off = first_off;
+    size_t level_n = n_first;
+    int block_n = (int)(n_first < TREE_BLOCK ? n_first : TREE_BLOCK);
+    for (int l = 1; l < nlev; l++) {
+        __syncthreads();
+        level_off += level_n;
+        level_n >>= 1;
+        block_n >>= 1;
+        u64 r[4];
+        if (t < block_n) H::node(sh + 8 * t, sh + 8 * t + 4, r);
+        __syncthreads();
+        if (t < block_n) {
+            u64 *o = tree + 4 * (level_off + (size_t)blockIdx.x * (TREE_BLOCK >> l) + t);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                o[k] = r[k];
+                sh[4 * t + k] = r[k];
+            }
+        }
+    }
+}
+
+static int ilog2_sz(size_t n) {
+    int l = 0;
+    while ((size_t(1) << l) < n) l++;
+    return l;
+}
+
+// Levels written by one launch whose first level has n nodes: all remaining levels when
+// the whole rest fits one workgroup, else the levels that keep every wave full.
+static int levels_per_launch(size_t n) {
+    if (n <= (size_t)TREE_BLOCK) return ilog2_sz(n) + 1;
+    return 3;
+}
+
+template <class H>
+static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_src, const FoldSrc &fs,
+                      hipStream_t s) {
+    if (num_leaves == 0 || (num_leaves & (num_leaves - 1))) return -2;
+    ProfScope prof(PROF_MERKLE, 2.0 * (double)num_leaves - 1.0, s);
+    size_t n = num_leaves, off = 0;
+    int total_levels = ilog2_sz(num_leaves) + 1;
+    int done = 0;
+    bool first = true;
+    while (done < total_levels) {
+        int nl = levels_per_launch(n);
+        if (nl > total_levels - done) nl = total_levels - done;
+        unsigned grid = (unsigned)((n + TREE_BLOCK - 1) / TREE_BLOCK);
+        if (first) {
+            switch (leaf_src) {
+                case SRC_GL: k_tree_levels<H, SRC_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs); break;
+                case SRC_EXT2:
+                    k_tree_levels<H, SRC_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                    break;
+                case SRC_FOLD_GL:
+                    k_tree_levels<H, SRC_FOLD_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                    break;
+                default:
+                    k_tree_levels<H, SRC_FOLD_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                    break;
+            }
+        } else {
+            // children are the level ending right before `off`
+            const u64 *child = d_tree + 4 * (off - 2 * n);
+            k_tree_levels<H, SRC_NODES><<<grid, TREE_BLOCK, 0, s>>>(child, d_tree, off, n, nl, fs);
+        }
+        if (hipGetLastError() != hipSuccess) return -1;
+        for (int l = 0; l < nl; l++) {
+            off += n;
+            n >>= 1;
+        }
+        done += nl;
+        first = false;
+    }
+    return 0;
+}
+
+int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, int ext2_leaves, hipStream_t s) {
+    FoldSrc fs = {nullptr, e2_zero(), nullptr};
+    int src = ext2_leaves ? SRC_EXT2 : SRC_GL;
+    if (hash == MAM_HASH_MONOLITH) return build_tree<MonoHash>(d_cw, d_tree, num_leaves, src, fs, s);
+    if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(d_cw, d_tree, num_leaves, src, fs, s);
+    return -2;
+}
+
+int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
+                      size_t folded_len, int hash, hipStream_t s) {
+    if (folded_len < 2) return -2;
+    FoldSrc fs = {table, alpha, fold_out};
+    int src = base_cw ? SRC_FOLD_GL : SRC_FOLD_EXT2;
+    if (hash == MAM_HASH_MONOLITH) return build_tree<MonoHash>(cw, tree, folded_len / 2, src, fs, s);
+    if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(cw, tree, folded_len / 2, src, fs, s);
+    return -2;
+}
+
+size_t mam_merkle_root_offset(size_t num_leaves) { return (2 * num_leaves - 2) * 4; }
+
+// ---------------------------------------------------------------------------
+// Reference-named tree entry points (ffi.rs:543-570): build and synchronize.
+// ---------------------------------------------------------------------------
+MAM_API int monolith_merkle_tree_gl_ffi(const u64 *d_codeword, u64 *d_tree, int num_leaves) {
+    if (num_leaves <= 0) return -1;
+    int rc = mam_merkle_build(d_codeword, d_tree, (size_t)num_leaves, MAM_HASH_MONOLITH, 0, mam_stream());
+    return rc ? rc : mam_sync_status();
+}
+MAM_API int monolith_merkle_tree_ext2_ffi(const u64 *d_codeword, u64 *d_tree, int num_leaves) {
+    if (num_leaves <= 0) return -1;
+    int rc = mam_merkle_build(d_codeword, d_tree, (size_t)num_leaves, MAM_HASH_MONOLITH, 1, mam_stream());
+    return rc ? rc : mam_sync_status();
+}
+MAM_API int poseidon2_merkle_tree_gl_ffi(const u64 *d_codeword, u64 *d_tree, int num_leaves) {
+    if (num_leaves <= 0) return -1;
+    int rc = mam_merkle_build(d_codeword, d_tree, (size_t)num_leaves, MAM_HASH_POSEIDON2, 0, mam_stream());
+    return rc ? rc : mam_sync_status();
+}
+MAM_API int poseidon2_merkle_tree_ext2_ffi(const u64 *d_codeword, u64 *d_tree, int num_leaves) {
+    if (num_leaves <= 0) return -1;
+    int rc = mam_merkle_build(d_codeword, d_tree, (size_t)num_leaves, MAM_HASH_POSEIDON2, 1, mam_stream());
+    return rc ? rc : mam_sync_status();
+}
+
+// ---------------------------------------------------------------------------
+// Single permutations and Poseidon2 batches (wrapper.cu:864-928, 1990-2011)
+// ---------------------------------------------------------------------------
+__global__ void k_mono_permute_n(u64 *s, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 x[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) x[k] = s[12 * (size_t)i + k];
+    mono::permute(x);
+#pragma unroll
+    for (int k = 0; k < 12; k++) s[12 * (size_t)i + k] = x[k];
+}
+
+MAM_API int monolith_permute_test_ffi(const u64 *h_input, u64 *h_output) {
+    u64 *d = nullptr;
+    if (hipMalloc(&d, 12 * sizeof(u64)) != hipSuccess) return -1;
+    int rc = -1;
+    if (hipMemcpyAsync(d, h_input, 96, hipMemcpyHostToDevice, mam_stream()) == hipSuccess) {
+        k_mono_permute_n<<<1, 64, 0, mam_stream()>>>(d, 1);
+        if (hipGetLastError() == hipSuccess &&
+            hipMemcpyAsync(h_output, d, 96, hipMemcpyDeviceToHost, mam_stream()) == hipSuccess &&
+            hipStreamSynchronize(mam_stream()) == hipSuccess)
+            rc = 0;
+    }
+    hipFree(d);
+    return rc;
+}
+
+// Monolith permutation of n states of 12 words in place (device pointers).
+MAM_API int mam_monolith_permute_batch(u64 *d_states, int n) {
+    if (n <= 0) return 0;
+    k_mono_permute_n<<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_states, n);
+    return mam_launch_status();
+}
+
+__global__ void k_p2_permute(const u64 *__restrict__ in, u64 *__restrict__ out, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) x[k] = in[8 * (size_t)i + k];
+    p2::permute(x);
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[8 * (size_t)i + k] = x[k];
+}
+
+__global__ void k_p2_compress(const u64 *__restrict__ l, const u64 *__restrict__ r, u64 *__restrict__ out, int n,
+                              int interleaved) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    u64 d[4];
+    if (interleaved)
+        p2::compress(l + 8 * (size_t)i, l + 8 * (size_t)i + 4, d);
+    else
+        p2::compress(l + 4 * (size_t)i, r + 4 * (size_t)i, d);
+#pragma unroll
+    for (int k = 0; k < 4; k++) out[4 * (size_t)i + k] = d[k];
+}
+
+MAM_API int poseidon2_hash_batch_ffi(const u64 *d_input, u64 *d_output, int n) {
+    if (n <= 0) return 0;
+    k_p2_permute<<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_input, d_output, n);
+    return mam_launch_status();
+}
+MAM_API int poseidon2_compress_batch_ffi(const u64 *d_left, const u64 *d_right, u64 *d_output, int n) {
+    if (n <= 0) return 0;
+    k_p2_compress<<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_left, d_right, d_output, n, 0);
+    return mam_launch_status();
+}
+MAM_API int poseidon2_merkle_layer_ffi(const u64 *d_input, u64 *d_output, int n) {
+    if (n <= 0) return 0;
+    k_p2_compress<<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_input, nullptr, d_output, n, 1);
+    return mam_launch_status();
+}
+
+// Build a full tree from host-visible arguments on the calling thread's stream without
+// synchronizing (for callers composing their own pipelines).
+MAM_API int mam_merkle_tree_async(const u64 *d_codeword, u64 *d_tree, size_t num_leaves, int hash, int ext2_leaves) {
+    return mam_merkle_build(d_codeword, d_tree, num_leaves, hash, ext2_leaves, mam_stream());
+}

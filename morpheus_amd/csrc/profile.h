@@ -1,0 +1,36 @@
+// Opt-in timing of kernel groups with HIP events recorded on the launching stream.
+// Disabled by default (no events, no cost); bench.py enables it to measure the dominant
+// kernels inside its timed region. Each scope adds its algorithmic work (permutations
+// or bytes) to its category.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+enum MamProfCat { PROF_MERKLE = 0, PROF_OPEN_SUMCHECK = 1, PROF_COMMIT_TRANSFORM = 2, PROF_SUMCHECK = 3, PROF_NCAT = 4 };
+
+bool mam_prof_enabled();
+void mam_prof_push(int cat, double work, hipEvent_t a, hipEvent_t b);
+
+struct ProfScope {
+    int cat;
+    double work;
+    hipStream_t s;
+    hipEvent_t a = nullptr;
+    ProfScope(int c, double w, hipStream_t st) : cat(c), work(w), s(st) {
+        if (mam_prof_enabled() && hipEventCreate(&a) == hipSuccess) {
+            if (hipEventRecord(a, s) != hipSuccess) {
+                (void)hipEventDestroy(a);
+                a = nullptr;
+            }
+        }
+    }
+    ~ProfScope() {
+        if (!a) return;
+        hipEvent_t b;
+        if (hipEventCreate(&b) == hipSuccess && hipEventRecord(b, s) == hipSuccess) {
+            mam_prof_push(cat, work, a, b);
+        } else {
+            (void)hipEventDestroy(a);
+        }
+    }
+};

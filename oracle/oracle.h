@@ -1,0 +1,114 @@
+/*
+ * oracle/oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * The oracle: a scalar CPU restatement of the reference prover path
+ * (Vimalakirti/morpheus: zk-torch-3 transcript + CPU provers, goldilocks-cuda-rs
+ * Basefold / Merkle semantics).  It is the checker for the HIP library in
+ * morpheus_amd/ and the CPU baseline timed by bench.py.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+ * product never links or calls it.  All outputs are canonical residues.
+ *
+ * Pinning status of each function is recorded next to its declaration.
+ */
+#ifndef ORACLE_H
+#define ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_HASH_MONOLITH 0
+#define OR_HASH_POSEIDON2 1
+
+/* ---- field (pinned: brute force vs Python big ints, algebraic identities) ---- */
+uint64_t or_gl_add(uint64_t a, uint64_t b);
+uint64_t or_gl_sub(uint64_t a, uint64_t b);
+uint64_t or_gl_mul(uint64_t a, uint64_t b);
+uint64_t or_gl_inv(uint64_t a);
+void or_ext2_mul(const uint64_t *a, const uint64_t *b, uint64_t *out);
+void or_ext2_inv(const uint64_t *a, uint64_t *out);
+
+/* ---- hashes ---- */
+/* Poseidon2 width 8.  initial_linear_layer=1 is the Plonky3 schedule (pinned by
+ * the Plonky3 known-answer vector, cuda/poseidon2_kernels.cu:546-555);
+ * initial_linear_layer=0 is the reference code's schedule
+ * (zk-torch-3/src/transcript.rs:136-161), which shares every constant. */
+void or_poseidon2_permute(uint64_t *state8, int initial_linear_layer);
+/* Monolith width 12 (pinned: goldilocks-cuda-rs/src/cpu_monolith.rs:236-256) */
+void or_monolith_permute(uint64_t *state12);
+void or_monolith_hash_gl_leaf(uint64_t a, uint64_t b, uint64_t *out4);
+void or_monolith_hash_ext2_leaf(const uint64_t *a2, const uint64_t *b2, uint64_t *out4);
+void or_monolith_compress(const uint64_t *l4, const uint64_t *r4, uint64_t *out4);
+void or_poseidon2_hash_gl_leaf(uint64_t a, uint64_t b, uint64_t *out4);
+void or_poseidon2_hash_ext2_leaf(const uint64_t *a2, const uint64_t *b2, uint64_t *out4);
+void or_poseidon2_compress(const uint64_t *l4, const uint64_t *r4, uint64_t *out4);
+
+/* Merkle tree, layout [N leaves][N/2]...[root], 4 u64 per digest.
+ * tree must hold (2N-1)*4 u64. */
+void or_merkle_tree_gl(const uint64_t *cw, long num_leaves, int hash, uint64_t *tree);
+void or_merkle_tree_ext2(const uint64_t *cw, long num_leaves, int hash, uint64_t *tree);
+long or_merkle_root_offset(long num_leaves);
+int or_merkle_verify_path(const uint64_t *leaf4, const uint64_t *path, int path_len, long index,
+                          const uint64_t *root4, int hash);
+
+/* ---- Fiat-Shamir transcript (zk-torch-3/src/transcript.rs:167-330) ---- */
+typedef struct {
+    uint64_t state[8];
+    uint64_t in_buf[4];
+    uint64_t out_buf[4];
+    int64_t in_count;
+    int64_t out_count;
+} or_transcript;
+
+void or_tr_new(or_transcript *t, const uint8_t *label, long len);
+void or_tr_append_u64(or_transcript *t, const uint8_t *label, long len, uint64_t v);
+void or_tr_append_scalars(or_transcript *t, const uint8_t *label, long len, const uint64_t *v, long n);
+void or_tr_append_ext2(or_transcript *t, const uint8_t *label, long len, const uint64_t *v2);
+uint64_t or_tr_challenge_scalar(or_transcript *t, const uint8_t *label, long len);
+void or_tr_challenge_ext2(or_transcript *t, const uint8_t *label, long len, uint64_t *out2);
+
+/* ---- eq / Lagrange tables, LSB-first (zk-torch-3/src/poly/mod.rs:106-163) ---- */
+void or_eq_table(const uint64_t *r, int nv, uint64_t *out);
+void or_eq_table_ext2(const uint64_t *r2, int nv, uint64_t *out2);
+/* MLE evaluation by repeated LSB-first folding (closed-form check for partial eval) */
+void or_partial_eval_ext2(const uint64_t *in2, int nv, const uint64_t *r2, int m, uint64_t *out2);
+
+/* ---- linear sumcheck over Ext2 (zk-torch-3/src/sumcheck/cpu_ext2_prover.rs:21-148) ----
+ * polys: np polynomials, each 2^nv Ext2 values interleaved [c0,c1]; consumed.
+ * Writes round messages [nv][np+1][2], challenges [nv][2], final evals [np][2],
+ * final_eval [2] (product of final evals). */
+void or_sumcheck_prove_ext2(or_transcript *t, uint64_t *polys, int np, int nv,
+                            uint64_t *round_msgs, uint64_t *challenges,
+                            uint64_t *final_evals, uint64_t *final_eval);
+/* zk-torch-3/src/sumcheck/verifier.rs:17-65 */
+int or_sumcheck_verify_ext2(or_transcript *t, const uint64_t *claim2, int np, int nv,
+                            const uint64_t *round_msgs, const uint64_t *final_eval);
+
+/* ---- Basefold (goldilocks-cuda-rs/src/basefold.rs, zk-torch-3/src/commit/cpu_basefold.rs) ---- */
+/* number of table entries = sum over levels of 2^(nv+lr-i-1) */
+long or_basefold_table_len(int nv, int lr, int nrounds);
+/* BasefoldTable::generate (basefold.rs:281-321); entries as [x0, w] pairs */
+void or_basefold_table_generate(int nv, int lr, int nrounds, uint64_t seed, uint64_t *entries);
+/* commit (wrapper.cu:1242-1312 + merkle.rs:34-59): codeword 2^(nv+lr) (Type1),
+ * bh_evals 2^nv (Type1), tree (2N-1)*4 with N = 2^(nv+lr-1) leaves */
+void or_basefold_commit(const uint64_t *evals, int nv, int lr, int hash,
+                        uint64_t *codeword, uint64_t *bh_evals, uint64_t *tree);
+/* flat proof length in u64 for (nv, lr, nq) */
+long or_basefold_proof_len(int nv, int lr, int nq);
+/* cpu_full_open_ext2 (cpu_basefold.rs:235-425); table entries as generated for
+ * (table_nv, lr, table_nv).  Writes the flat proof layout. */
+void or_basefold_open_ext2(const uint64_t *codeword, const uint64_t *bh_evals, const uint64_t *root4,
+                           int nv, int lr, int hash, const uint64_t *point2,
+                           const uint64_t *table, int table_nv, or_transcript *t, int nq,
+                           uint64_t *proof);
+/* BasefoldVerifier::verify_ext2 (basefold.rs:1759-1925); returns 1 on accept */
+int or_basefold_verify_ext2(const uint64_t *root4, const uint64_t *point2, int nv, int lr, int hash,
+                            const uint64_t *proof, int nq, const uint64_t *table, int table_nv,
+                            or_transcript *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+"""ctypes wrapper of the oracle (liboracle.so, built from oracle/*.c by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg, as the checker. The product (morpheus_amd) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
+L = ctypes.CDLL(LIB_PATH)
+
+u64 = ctypes.c_uint64
+u64p = ctypes.POINTER(u64)
+ci = ctypes.c_int
+cl = ctypes.c_long
+P = 0xFFFFFFFF00000001
+MONOLITH = 0
+POSEIDON2 = 1
+
+
+class Transcript_t(ctypes.Structure):
+    _fields_ = [("state", u64 * 8), ("in_buf", u64 * 4), ("out_buf", u64 * 4),
+                ("in_count", ctypes.c_int64), ("out_count", ctypes.c_int64)]
+
+
+trp = ctypes.POINTER(Transcript_t)
+_sigs = {
+    "or_gl_add": (u64, [u64, u64]), "or_gl_sub": (u64, [u64, u64]), "or_gl_mul": (u64, [u64, u64]),
+    "or_gl_inv": (u64, [u64]), "or_ext2_mul": (None, [u64p, u64p, u64p]), "or_ext2_inv": (None, [u64p, u64p]),
+    "or_poseidon2_permute": (None, [u64p, ci]), "or_monolith_permute": (None, [u64p]),
+    "or_monolith_hash_gl_leaf": (None, [u64, u64, u64p]), "or_monolith_compress": (None, [u64p, u64p, u64p]),
+    "or_merkle_tree_gl": (None, [u64p, cl, ci, u64p]), "or_merkle_tree_ext2": (None, [u64p, cl, ci, u64p]),
+    "or_tr_new": (None, [trp, ctypes.c_char_p, cl]), "or_tr_append_u64": (None, [trp, ctypes.c_char_p, cl, u64]),
+    "or_tr_append_scalars": (None, [trp, ctypes.c_char_p, cl, u64p, cl]),
+    "or_tr_append_ext2": (None, [trp, ctypes.c_char_p, cl, u64p]),
+    "or_tr_challenge_scalar": (u64, [trp, ctypes.c_char_p, cl]),
+    "or_tr_challenge_ext2": (None, [trp, ctypes.c_char_p, cl, u64p]),
+    "or_eq_table": (None, [u64p, ci, u64p]), "or_eq_table_ext2": (None, [u64p, ci, u64p]),
+    "or_partial_eval_ext2": (None, [u64p, ci, u64p, ci, u64p]),
+    "or_sumcheck_prove_ext2": (None, [trp, u64p, ci, ci, u64p, u64p, u64p, u64p]),
+    "or_sumcheck_verify_ext2": (ci, [trp, u64p, ci, ci, u64p, u64p]),
+    "or_basefold_table_len": (cl, [ci, ci, ci]), "or_basefold_table_generate": (None, [ci, ci, ci, u64, u64p]),
+    "or_basefold_commit": (None, [u64p, ci, ci, ci, u64p, u64p, u64p]),
+    "or_basefold_proof_len": (cl, [ci, ci, ci]),
+    "or_basefold_open_ext2": (None, [u64p, u64p, u64p, ci, ci, ci, u64p, u64p, ci, trp, ci, u64p]),
+    "or_basefold_verify_ext2": (ci, [u64p, u64p, ci, ci, ci, u64p, ci, u64p, ci, trp]),
+}
+for _n, (_r, _a) in _sigs.items():
+    _f = getattr(L, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+def _p(a):
+    return a.ctypes.data_as(u64p)
+
+
+def arr(x):
+    return np.ascontiguousarray(x, dtype=np.uint64)
+
+
+class Transcript:
+    def __init__(self, label=b""):
+        self.t = Transcript_t()
+        L.or_tr_new(ctypes.byref(self.t), label, len(label))
+
+    def clone(self):
+        c = Transcript.__new__(Transcript)
+        c.t = Transcript_t()
+        ctypes.pointer(c.t)[0] = self.t
+        return c
+
+    def append_u64(self, label, v):
+        L.or_tr_append_u64(ctypes.byref(self.t), label, len(label), int(v))
+
+    def append_scalars(self, label, v):
+        a = arr(v)
+        L.or_tr_append_scalars(ctypes.byref(self.t), label, len(label), _p(a), a.size)
+
+    def append_ext2(self, label, v2):
+        a = arr(v2)
+        L.or_tr_append_ext2(ctypes.byref(self.t), label, len(label), _p(a))
+
+    def challenge_scalar(self, label=b""):
+        return int(L.or_tr_challenge_scalar(ctypes.byref(self.t), label, len(label)))
+
+    def challenge_ext2(self, label=b""):
+        o = np.zeros(2, dtype=np.uint64)
+        L.or_tr_challenge_ext2(ctypes.byref(self.t), label, len(label), _p(o))
+        return o
+
+    def state(self):
+        return np.array(self.t.state, dtype=np.uint64)
+
+
+def poseidon2_permute(state8, initial_linear_layer=0):
+    s = arr(state8).copy()
+    L.or_poseidon2_permute(_p(s), initial_linear_layer)
+    return s
+
+
+def monolith_permute(state12):
+    s = arr(state12).copy()
+    L.or_monolith_permute(_p(s))
+    return s
+
+
+def eq_table_ext2(point2, nv):
+    out = np.zeros(2 << nv, dtype=np.uint64)
+    L.or_eq_table_ext2(_p(arr(point2)), nv, _p(out))
+    return out
+
+
+def eq_table(r, nv):
+    out = np.zeros(1 << nv, dtype=np.uint64)
+    L.or_eq_table(_p(arr(r)), nv, _p(out))
+    return out
+
+
+def partial_eval_ext2(in2, nv, r2, m):
+    out = np.zeros(2 << (nv - m), dtype=np.uint64)
+    L.or_partial_eval_ext2(_p(arr(in2)), nv, _p(arr(r2)), m, _p(out))
+    return out
+
+
+def sumcheck_prove_ext2(t, polys_ext2, np_, nv):
+    """polys_ext2: [np][2^nv][2] words (copied). Returns dict like the library's."""
+    p = arr(polys_ext2).copy()
+    msgs = np.zeros((nv, np_ + 1, 2), dtype=np.uint64)
+    chs = np.zeros((nv, 2), dtype=np.uint64)
+    fin = np.zeros((np_, 2), dtype=np.uint64)
+    fe = np.zeros(2, dtype=np.uint64)
+    L.or_sumcheck_prove_ext2(ctypes.byref(t.t), _p(p), np_, nv, _p(msgs), _p(chs), _p(fin), _p(fe))
+    return {"round_msgs": msgs, "challenges": chs, "final_evals": fin, "final_eval": fe}
+
+
+def sumcheck_verify_ext2(t, claim2, np_, nv, round_msgs, final_eval):
+    return bool(L.or_sumcheck_verify_ext2(ctypes.byref(t.t), _p(arr(claim2)), np_, nv, _p(arr(round_msgs)),
+                                          _p(arr(final_eval))))
+
+
+def basefold_table(nv, lr, nrounds, seed):
+    n = L.or_basefold_table_len(nv, lr, nrounds)
+    e = np.zeros(2 * n, dtype=np.uint64)
+    L.or_basefold_table_generate(nv, lr, nrounds, seed, _p(e))
+    return e
+
+
+def basefold_commit(evals, nv, lr, hash_kind=MONOLITH, with_tree=True):
+    cw = np.zeros(1 << (nv + lr), dtype=np.uint64)
+    bh = np.zeros(1 << nv, dtype=np.uint64)
+    tree = np.zeros(((1 << (nv + lr)) - 1) * 4, dtype=np.uint64) if with_tree else None
+    L.or_basefold_commit(_p(arr(evals)), nv, lr, hash_kind, _p(cw), _p(bh), _p(tree) if with_tree else None)
+    return cw, bh, tree
+
+
+def merkle_tree(cw, num_leaves, hash_kind=MONOLITH, ext2=False):
+    tree = np.zeros((2 * num_leaves - 1) * 4, dtype=np.uint64)
+    f = L.or_merkle_tree_ext2 if ext2 else L.or_merkle_tree_gl
+    f(_p(arr(cw)), num_leaves, hash_kind, _p(tree))
+    return tree
+
+
+def basefold_proof_len(nv, lr, nq):
+    return int(L.or_basefold_proof_len(nv, lr, nq))
+
+
+def basefold_open_ext2(cw, bh, root, nv, lr, hash_kind, point2, table, table_nv, t, nq):
+    proof = np.zeros(basefold_proof_len(nv, lr, nq), dtype=np.uint64)
+    L.or_basefold_open_ext2(_p(arr(cw)), _p(arr(bh)), _p(arr(root)), nv, lr, hash_kind, _p(arr(point2)),
+                            _p(arr(table)), table_nv, ctypes.byref(t.t), nq, _p(proof))
+    return proof
+
+
+def basefold_verify_ext2(root, point2, nv, lr, hash_kind, proof, nq, table, table_nv, t):
+    return bool(L.or_basefold_verify_ext2(_p(arr(root)), _p(arr(point2)), nv, lr, hash_kind, _p(arr(proof)), nq,
+                                          _p(arr(table)), table_nv, ctypes.byref(t.t)))
