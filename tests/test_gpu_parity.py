@@ -302,6 +302,7 @@ def test_sumcheck_reference_ffi_loop_and_state():
     fin = cur.to_numpy().reshape(np_, n, 2)[:, 0, :]
     assert np.array_equal(fin, want["final_evals"])
     # stateful interface
+    d = up(ext)  # the loop above ping-ponged through the first upload
     st = M.GpuSumcheckStateExt2(d, np_, nv, base=False)
     t2 = M.Transcript(b"s")
     t2.append_u64(b"num_var", nv)
