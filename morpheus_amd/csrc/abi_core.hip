@@ -298,3 +298,68 @@ MAM_API int mam_profile_collect(double *ms, long *count, double *work) {
     prof_clear();
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Per-thread scratch arenas (common.h)
+// ---------------------------------------------------------------------------
+namespace {
+struct Arena {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+struct ThreadArenas {
+    Arena dev[4];
+    Arena host;
+    ~ThreadArenas() {
+        for (auto &a : dev)
+            if (a.p) (void)hipFree(a.p);
+        if (host.p) (void)hipHostFree(host.p);
+    }
+};
+thread_local ThreadArenas t_arenas;
+}  // namespace
+
+u64 *mam_dev_arena(size_t words, int slot) {
+    if (slot < 0 || slot >= 4) return nullptr;
+    Arena &a = t_arenas.dev[slot];
+    size_t bytes = (words ? words : 1) * sizeof(u64);
+    if (a.bytes < bytes) {
+        if (a.p) {
+            (void)hipStreamSynchronize(mam_stream());
+            (void)hipFree(a.p);
+            a.p = nullptr;
+            a.bytes = 0;
+        }
+        if (hipMalloc(&a.p, bytes) != hipSuccess) return nullptr;
+        a.bytes = bytes;
+    }
+    return (u64 *)a.p;
+}
+
+u64 *mam_host_arena(size_t words) {
+    Arena &a = t_arenas.host;
+    size_t bytes = (words ? words : 1) * sizeof(u64);
+    if (a.bytes < bytes) {
+        if (a.p) {
+            (void)hipStreamSynchronize(mam_stream());
+            (void)hipHostFree(a.p);
+            a.p = nullptr;
+            a.bytes = 0;
+        }
+        if (hipHostMalloc(&a.p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+        a.bytes = bytes;
+    }
+    return (u64 *)a.p;
+}
+
+// Free the calling thread's scratch arenas (they are re-created on demand).
+MAM_API int mam_release_scratch() {
+    (void)hipStreamSynchronize(mam_stream());
+    for (auto &a : t_arenas.dev) {
+        if (a.p) (void)hipFree(a.p);
+        a = Arena();
+    }
+    if (t_arenas.host.p) (void)hipHostFree(t_arenas.host.p);
+    t_arenas.host = Arena();
+    return 0;
+}

@@ -49,3 +49,10 @@ static inline int mam_sync_status() {
     do {                                     \
         if ((x) != hipSuccess) return -1;    \
     } while (0)
+
+// Per-thread reusable device and pinned-host scratch (abi_core.hip). `slot` selects one of
+// a few independent arenas; a request larger than the arena's size reallocates it (the
+// old contents are not kept). Returns nullptr on allocation failure. The memory stays
+// owned by the library until the thread's arenas are released with mam_release_scratch.
+u64 *mam_dev_arena(size_t words, int slot);
+u64 *mam_host_arena(size_t words);

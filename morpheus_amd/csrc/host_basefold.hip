@@ -183,11 +183,26 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     observe_hash(t, C.root);
     for (int i = 0; i < nv; i++) observe_ext2(t, point2 + 2 * i);
 
-    std::vector<void *> allocs;
-    auto alloc = [&](size_t words) -> u64 * {
-        u64 *p = nullptr;
-        if (hipMallocAsync((void **)&p, (words ? words : 1) * sizeof(u64), s) != hipSuccess) return nullptr;
-        allocs.push_back(p);
+    // all device scratch of the opening is carved from the thread's arena (slot 0)
+    const long qlen = query_len(nv, lr);
+    size_t need = 0;
+    auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
+    {
+        need += words(2 * nv) + words(2 * n) + words(n) + 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) +
+                words(8) + words(nq) + words((size_t)nq * qlen);
+        size_t flen = cw_len / 2;
+        for (int r = 0; r < nv; r++) {
+            need += words(2 * flen);
+            if (r < nv - 1) need += words((2 * (flen / 2) - 1) * 4);
+            flen /= 2;
+        }
+    }
+    u64 *arena = mam_dev_arena(need, 0);
+    size_t used = 0;
+    auto alloc = [&](size_t w) -> u64 * {
+        if (!arena) return nullptr;
+        u64 *p = arena + used;
+        used += words(w);
         return p;
     };
     int rc = 0;
@@ -207,8 +222,7 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
             flen /= 2;
         }
     }
-    for (void *p : allocs)
-        if (!p) rc = -1;
+    if (!arena) rc = -1;
     u64 *h = pinned_scratch();
     if (!h) rc = -1;
     RedScratch rs = {red, (u32 *)cnt, red + RED_BLOCKS * 6};
@@ -279,7 +293,6 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     if (!rc && nq > 0) {
         std::vector<u64> idx(nq);
         for (int q = 0; q < nq; q++) idx[q] = mam::tr_challenge_scalar(t, nullptr, 0) % (u64)(cw_len / 2);
-        const long qlen = query_len(nv, lr);
         u64 *d_idx = alloc(nq);
         u64 *d_out = alloc((size_t)nq * qlen);
         if (!d_idx || !d_out) rc = -1;
@@ -304,7 +317,6 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
             rc = -1;
         if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = -1;
     }
-    for (void *p : allocs) free_async(p, s);
     return rc;
 }
 

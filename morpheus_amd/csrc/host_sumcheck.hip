@@ -35,14 +35,33 @@ struct ScState {
     u64 *finals = nullptr;     // device, np Ext2
     u64 *host = nullptr;       // pinned readback, 2 * (MAM_MAX_POLYS + 1) words
     hipStream_t st = nullptr;
+    bool arena = false;        // buffers carved from the thread's scratch arena
 };
 
-int state_alloc(ScState &S) {
+static size_t align_words(size_t w) { return (w + 31) & ~(size_t)31; }
+
+int state_alloc(ScState &S, bool arena) {
     hipStream_t st = S.st;
     size_t half = S.n / 2;
     // buf[0] holds rounds 1, 3, 5, ... (n/2 Ext2 max); buf[1] rounds 2, 4, ... (n/4 Ext2 max)
     S.stride[0] = 2 * half;
     S.stride[1] = half > 1 ? half : 2;
+    if (arena) {
+        const size_t w0 = align_words(S.np * S.stride[0]), w1 = align_words(S.np * S.stride[1]);
+        const size_t wp = align_words(MAM_RED_MAX_BLOCKS * 2 * (MAM_MAX_POLYS + 1)), wc = 32,
+                     wm = align_words(2 * (MAM_MAX_POLYS + 1)), wf = align_words(2 * MAM_MAX_POLYS);
+        u64 *base = mam_dev_arena(w0 + w1 + wp + wc + wm + wf, 1);
+        S.host = mam_host_arena(4 * (MAM_MAX_POLYS + 1));
+        if (!base || !S.host) return -1;
+        S.arena = true;
+        S.buf[0] = base;
+        S.buf[1] = base + w0;
+        S.s.partials = base + w0 + w1;
+        S.s.counter = (u32 *)(base + w0 + w1 + wp);
+        S.s.msg = base + w0 + w1 + wp + wc;
+        S.finals = S.s.msg + wm;
+        return hipMemsetAsync(S.s.counter, 0, 64, st) == hipSuccess ? 0 : -1;
+    }
     if (hipMallocAsync((void **)&S.buf[0], S.np * S.stride[0] * sizeof(u64), st) != hipSuccess) return -1;
     if (hipMallocAsync((void **)&S.buf[1], S.np * S.stride[1] * sizeof(u64), st) != hipSuccess) return -1;
     if (hipMallocAsync((void **)&S.s.partials, MAM_RED_MAX_BLOCKS * 2 * (MAM_MAX_POLYS + 1) * sizeof(u64), st) !=
@@ -59,6 +78,10 @@ int state_alloc(ScState &S) {
 
 void state_free(ScState &S) {
     hipStream_t st = S.st;
+    if (S.arena) {
+        S = ScState();
+        return;
+    }
     if (S.buf[0]) hipFreeAsync(S.buf[0], st);
     if (S.buf[1]) hipFreeAsync(S.buf[1], st);
     if (S.s.partials) hipFreeAsync(S.s.partials, st);
@@ -69,7 +92,7 @@ void state_free(ScState &S) {
     S = ScState();
 }
 
-int state_init(ScState &S, const u64 *d_polys, int np, int nv, bool base, hipStream_t st) {
+int state_init(ScState &S, const u64 *d_polys, int np, int nv, bool base, hipStream_t st, bool arena) {
     if (np < 1 || np > MAM_MAX_POLYS || nv < 1 || nv > 34 || !d_polys) return -2;
     S.np = np;
     S.nv = nv;
@@ -80,7 +103,7 @@ int state_init(ScState &S, const u64 *d_polys, int np, int nv, bool base, hipStr
     S.cur = -1;
     S.cur_size = S.n;
     S.st = st;
-    return state_alloc(S);
+    return state_alloc(S, arena);
 }
 
 const u64 *cur_ptr(const ScState &S, size_t &stride, bool &base) {
@@ -163,7 +186,7 @@ extern "C" __attribute__((visibility("default"))) int mam_sumcheck_prove(
     mam_transcript *t, const u64 *d_polys, int num_polys, int num_vars, int input_is_base, u64 *round_msgs,
     u64 *challenges, u64 *final_evals, u64 *final_eval) {
     ScState S;
-    int rc = state_init(S, d_polys, num_polys, num_vars, input_is_base != 0, mam_stream());
+    int rc = state_init(S, d_polys, num_polys, num_vars, input_is_base != 0, mam_stream(), true);
     if (rc) {
         state_free(S);
         return rc;
@@ -206,7 +229,7 @@ extern "C" __attribute__((visibility("default"))) int mam_sc_state_new(const u64
                                                                       int input_is_base, void **handle) {
     if (!handle) return -2;
     ScState *S = new ScState();
-    int rc = state_init(*S, d_polys, num_polys, num_vars, input_is_base != 0, mam_stream());
+    int rc = state_init(*S, d_polys, num_polys, num_vars, input_is_base != 0, mam_stream(), false);
     if (rc) {
         state_free(*S);
         delete S;

@@ -11,19 +11,24 @@
 // then keeps compressing inside LDS while every wave of the workgroup is still full
 // (256 -> 128 -> 64 nodes), so a 2^22-leaf tree is built by 8 launches with no host
 // synchronization in between, and each digest is written to HBM exactly once.
+#include <stdlib.h>
+
 #include "basefold.h"
 #include "common.h"
 #include "merkle.h"
 #include "monolith.h"
+#include "monolith_wide.h"
 #include "poseidon2.h"
 #include "profile.h"
 
 struct MonoHash {
+    static constexpr bool kWide = true;
     static MAM_HD void gl_leaf(u64 a, u64 b, u64 *o) { mono::hash_gl_leaf(a, b, o); }
     static MAM_HD void ext2_leaf(const u64 *ab, u64 *o) { mono::hash_ext2_leaf(ab, o); }
     static MAM_HD void node(const u64 *l, const u64 *r, u64 *o) { mono::compress(l, r, o); }
 };
 struct P2Hash {
+    static constexpr bool kWide = false;
     static MAM_HD void gl_leaf(u64 a, u64 b, u64 *o) { p2::hash_gl_leaf(a, b, o); }
     static MAM_HD void ext2_leaf(const u64 *ab, u64 *o) { p2::hash_ext2_leaf(ab, o); }
     static MAM_HD void node(const u64 *l, const u64 *r, u64 *o) { p2::compress(l, r, o); }
@@ -108,6 +113,79 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_levels(const u64 *__restric
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lane-parallel variant for small levels: 16 lanes per permutation (monolith_wide.h),
+// WIDE_GROUPS permutations per pass; a workgroup writes a level slab and then the
+// in-LDS levels above it down to one node per workgroup.
+// ---------------------------------------------------------------------------
+static constexpr int WIDE_GROUPS = 32;
+
+template <int SRC>
+__global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__restrict__ src, u64 *__restrict__ tree,
+                                                                size_t first_off, size_t n_first, int nlev,
+                                                                FoldSrc fs) {
+    __shared__ u64 sh[WIDE_GROUPS * 4];
+    const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
+    const size_t i = (size_t)blockIdx.x * WIDE_GROUPS + g;
+    const bool active = i < n_first;
+    u64 x = 0;
+    if (active) {
+        if (SRC == SRC_GL) {
+            if (j < 2) x = src[2 * i + j];
+        } else if (SRC == SRC_EXT2) {
+            if (j < 4) x = src[4 * i + j];
+        } else if (SRC == SRC_NODES) {
+            if (j < 8) x = src[8 * i + j];
+        } else if (j < 4) {
+            // lanes 0,1 build folded element 2i, lanes 2,3 element 2i+1; lane parity picks c0/c1
+            const size_t k = 2 * i + (j >> 1);
+            ext2 v0, v1;
+            if (SRC == SRC_FOLD_GL) {
+                v0 = e2_base(gl_canon(src[2 * k]));
+                v1 = e2_base(gl_canon(src[2 * k + 1]));
+            } else {
+                v0 = e2_load(src, 2 * k);
+                v1 = e2_load(src, 2 * k + 1);
+            }
+            const u64 x0 = gl_canon(fs.table[2 * k]), w = gl_canon(fs.table[2 * k + 1]);
+            const ext2 f = mam_fold_value(v0, v1, e2_sub(fs.alpha, e2_base(x0)), w);
+            x = (j & 1) ? f.c1 : f.c0;
+            fs.out[2 * k + (j & 1)] = x;
+        }
+    }
+    x = mono_wide::permute(x, j);
+    if (active && j < 4) {
+        tree[4 * (first_off + i) + j] = x;
+        sh[4 * g + j] = x;
+    }
+    size_t level_off = first_off, level_n = n_first;
+    int block_n = (int)(n_first < WIDE_GROUPS ? n_first : WIDE_GROUPS);
+    for (int l = 1; l < nlev; l++) {
+        __syncthreads();
+        level_off += level_n;
+        level_n >>= 1;
+        block_n >>= 1;
+        u64 y = (g < block_n && j < 8) ? sh[8 * g + j] : 0;
+        __syncthreads();
+        // wave-uniform: a wave (4 groups) runs the permutation if its first group is active
+        if ((g & ~3) < block_n) y = mono_wide::permute(y, j);
+        if (g < block_n && j < 4) {
+            tree[4 * (level_off + (size_t)blockIdx.x * (WIDE_GROUPS >> l) + g) + j] = y;
+            sh[4 * g + j] = y;
+        }
+    }
+}
+
+static size_t narrow_min() {
+    static size_t v = 0;
+    if (!v) {
+        const char *e = getenv("MAM_MERKLE_NARROW_MIN");
+        v = e ? (size_t)strtoull(e, nullptr, 10) : ((size_t)1 << 16);
+        if (v == 0) v = 1;
+    }
+    return v;
+}
+
 static int ilog2_sz(size_t n) {
     int l = 0;
     while ((size_t(1) << l) < n) l++;
@@ -130,7 +208,37 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
     int total_levels = ilog2_sz(num_leaves) + 1;
     int done = 0;
     bool first = true;
+    const bool can_wide = H::kWide;
     while (done < total_levels) {
+        if (can_wide && n < narrow_min()) {
+            int nl = n <= (size_t)WIDE_GROUPS ? ilog2_sz(n) + 1 : ilog2_sz(WIDE_GROUPS) + 1;
+            if (nl > total_levels - done) nl = total_levels - done;
+            unsigned grid = (unsigned)((n + WIDE_GROUPS - 1) / WIDE_GROUPS);
+            const int threads = WIDE_GROUPS * 16;
+            if (first) {
+                switch (leaf_src) {
+                    case SRC_GL: k_tree_wide<SRC_GL><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs); break;
+                    case SRC_EXT2: k_tree_wide<SRC_EXT2><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs); break;
+                    case SRC_FOLD_GL:
+                        k_tree_wide<SRC_FOLD_GL><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                        break;
+                    default:
+                        k_tree_wide<SRC_FOLD_EXT2><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                        break;
+                }
+            } else {
+                const u64 *child = d_tree + 4 * (off - 2 * n);
+                k_tree_wide<SRC_NODES><<<grid, threads, 0, s>>>(child, d_tree, off, n, nl, fs);
+            }
+            if (hipGetLastError() != hipSuccess) return -1;
+            for (int l = 0; l < nl; l++) {
+                off += n;
+                n >>= 1;
+            }
+            done += nl;
+            first = false;
+            continue;
+        }
         int nl = levels_per_launch(n);
         if (nl > total_levels - done) nl = total_levels - done;
         unsigned grid = (unsigned)((n + TREE_BLOCK - 1) / TREE_BLOCK);

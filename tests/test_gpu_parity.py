@@ -107,6 +107,16 @@ def test_merkle_trees(hash_kind, log_leaves):
         assert np.array_equal(t.to_numpy(), want)
 
 
+def test_merkle_tree_large_mixed_kernels():
+    # 2^17 leaves: bottom levels with thread-per-permutation kernels, top with lane groups
+    nl = 1 << 17
+    rng = np.random.default_rng(250)
+    cw = rng.integers(0, 1 << 64, 2 * nl, dtype=np.uint64)
+    d, t = up(cw), DB((2 * nl - 1) * 4)
+    ok(M.lib.monolith_merkle_tree_gl_ffi(d.ptr, t.ptr, nl))
+    assert np.array_equal(t.to_numpy(), O.merkle_tree(cw, nl, 0, False))
+
+
 def test_merkle_specific_codeword():
     cw = np.array(GOLDEN["monolith_specific_codeword"]["codeword"], dtype=np.uint64)
     d, t = up(cw), DB(7 * 4)
