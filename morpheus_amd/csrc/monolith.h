@@ -10,7 +10,7 @@
 //
 // Layout of the computation here (all integer, exact):
 //   concrete: y_r = sum_c ROW0[(c - r) mod 12] * x_c + RC[round][r]  (mod p)
-//             computed on 22/22/20-bit limbs so every partial sum fits 32 bits
+//             computed on 32-bit halves with 64-bit multiply-accumulate
 //   bars:     per byte b: b ^ (rotl1(~b) & rotl2(b) & rotl3(b)), then rotl1, on x_0..x_3
 //   bricks:   x_i += x_{i-1}^2 for i = 11..1 (old values)
 #pragma once
@@ -61,49 +61,75 @@ __host__ __device__ __forceinline__ u64 rc(int round, int r) {
     return t.v[round - 1][r];
 }
 
-// Limb split: x = l0 + 2^22 l1 + 2^44 l2 with l0, l1 < 2^22, l2 < 2^20.
-// For each output row the three limb sums are < 12 * 26 * 2^22 < 2^31.
-// Recombination: y = s0 + 2^22 s1 + 2^44 s2 + rc, a value below 2^76, then reduced.
-MAM_HD u64 reduce_small_hi(u64 lo, u64 hi) {
-    // lo + hi * 2^64 with hi < 2^32: = lo + hi * (2^32 - 1) (mod p)
-    u64 t = hi * GL_EPS;
-    u64 s = lo + t;
-    if (s < t) s += GL_EPS;
-    return gl_canon(s);
+// Values inside the permutation are kept as arbitrary 64-bit representatives of their
+// residues; only the four bar inputs (bars act on the bits of the canonical value) and the
+// outputs are reduced to [0, p).
+
+// a + b for any 64-bit representatives: a wrap-around of 2^64 is worth 2^32 - 1.
+MAM_HD u64 add_nc(u64 a, u64 b) {
+    u64 s = a + b;
+    if (s < a) {
+        s += GL_EPS;
+        if (s < GL_EPS) s += GL_EPS;
+    }
+    return s;
 }
 
-template <int ROUND>
+// x^2 reduced to a 64-bit representative (not necessarily canonical).
+MAM_HD u64 sqr_nc(u64 x) {
+    u64 lo, hi;
+    mul_wide(x, x, lo, hi);
+    u64 hh = hi >> 32, hl = hi & GL_EPS;
+    u64 t0 = lo - hh;
+    if (lo < hh) t0 -= GL_EPS;
+    u64 t1 = hl * GL_EPS;
+    u64 t2 = t0 + t1;
+    if (t2 < t1) t2 += GL_EPS;
+    return t2;
+}
+
+// lo + 2^64 * hi with hi < 2^32, reduced to a 64-bit representative: 2^64 = 2^32 - 1 (mod p)
+MAM_HD u64 fold_hi_nc(u64 lo, u64 hi) {
+    u64 t = (hi << 32) - hi;
+    u64 s = lo + t;
+    if (s < t) s += GL_EPS;
+    return s;
+}
+
+// Canonical residue of lo + 2^64 * hi with hi < 2^32.
+MAM_HD u64 reduce_small_hi(u64 lo, u64 hi) { return gl_canon(fold_hi_nc(lo, hi)); }
+
+// y_r = sum_c ROW0[(c - r) mod 12] * x_c + rc_r on 32-bit halves of any representatives:
+// sum_c m * lo_c and sum_c m * hi_c are each below 12 * 26 * 2^32 < 2^41 (one 32x32+64
+// multiply-add per term), then y = A + 2^32 B + rc is folded back below 2^64.
+// CANON_ROWS leading rows are made canonical (the next layer's bar inputs).
+template <int ROUND, int CANON_ROWS>
 MAM_HD void concrete(u64 *x) {
-    u32 l0[WIDTH], l1[WIDTH], l2[WIDTH];
+    u32 lo[WIDTH], hi[WIDTH];
 #pragma unroll
     for (int c = 0; c < WIDTH; c++) {
-        l0[c] = (u32)x[c] & 0x3FFFFFu;
-        l1[c] = (u32)(x[c] >> 22) & 0x3FFFFFu;
-        l2[c] = (u32)(x[c] >> 44);
+        lo[c] = (u32)x[c];
+        hi[c] = (u32)(x[c] >> 32);
     }
 #pragma unroll
     for (int r = 0; r < WIDTH; r++) {
-        u32 s0 = 0, s1 = 0, s2 = 0;
+        u64 a = 0, b = 0;
 #pragma unroll
         for (int c = 0; c < WIDTH; c++) {
-            const u32 m = row0((c - r + WIDTH) % WIDTH);
-            s0 += m * l0[c];
-            s1 += m * l1[c];
-            s2 += m * l2[c];
+            const u64 m = row0((c - r + WIDTH) % WIDTH);
+            a += (u64)lo[c] * m;
+            b += (u64)hi[c] * m;
         }
-        // y = s0 + s1 * 2^22 + s2 * 2^44 (+ rc); s2 * 2^44 < 2^75
-        u64 a = (u64)s0 + ((u64)s1 << 22);   // < 2^54
-        u64 b_lo = (u64)s2 << 44;             // low 64 bits of s2 * 2^44
-        u64 b_hi = (u64)s2 >> 20;             // high bits (< 2^11)
-        u64 lo = a + b_lo;
-        u64 hi = b_hi + (lo < a);
+        u64 s = a + (b << 32);
+        u64 h = (b >> 32) + (s < a);
         if (ROUND >= 1 && ROUND <= 5) {
-            u64 k = rc(ROUND, r);
-            u64 lo2 = lo + k;
-            hi += (lo2 < k);
-            lo = lo2;
+            const u64 k = rc(ROUND, r);
+            u64 s2 = s + k;
+            h += (s2 < k);
+            s = s2;
         }
-        x[r] = reduce_small_hi(lo, hi);
+        const u64 y = fold_hi_nc(s, h);
+        x[r] = r < CANON_ROWS ? gl_canon(y) : y;
     }
 }
 
@@ -121,48 +147,50 @@ MAM_HD u64 bar(u64 v) {
 
 MAM_HD void bricks(u64 *x) {
 #pragma unroll
-    for (int i = WIDTH - 1; i >= 1; i--) x[i] = gl_add(x[i], gl_sqr(x[i - 1]));
+    for (int i = WIDTH - 1; i >= 1; i--) x[i] = add_nc(x[i], sqr_nc(x[i - 1]));
 }
 
-template <int ROUND>
+// bar inputs x[0..3] are canonical (made so by the previous concrete layer)
+template <int ROUND, int CANON_ROWS>
 MAM_HD void round_fn(u64 *x) {
-    // a bar output may lie in [p, 2^64); only its residue matters from here on
 #pragma unroll
-    for (int i = 0; i < 4; i++) x[i] = gl_canon(bar(x[i]));
+    for (int i = 0; i < 4; i++) x[i] = bar(x[i]);
     bricks(x);
-    concrete<ROUND>(x);
+    concrete<ROUND, CANON_ROWS>(x);
 }
 
-// Full permutation. Accepts any 64-bit words (the first concrete layer is linear,
-// so the result depends only on the residues).
+// Full permutation. Accepts any 64-bit words (the first concrete layer is linear, so the
+// result depends only on the residues). OUT_WORDS leading output words are canonical:
+// 12 for the full permutation, 4 where only a digest is kept.
+template <int OUT_WORDS = WIDTH>
 MAM_HD void permute(u64 *x) {
-    concrete<0>(x);
-    round_fn<1>(x);
-    round_fn<2>(x);
-    round_fn<3>(x);
-    round_fn<4>(x);
-    round_fn<5>(x);
-    round_fn<6>(x);
+    concrete<0, 4>(x);
+    round_fn<1, 4>(x);
+    round_fn<2, 4>(x);
+    round_fn<3, 4>(x);
+    round_fn<4, 4>(x);
+    round_fn<5, 4>(x);
+    round_fn<6, OUT_WORDS>(x);
 }
 
 // Leaf of a base-field codeword: permute([a, b, 0, ...])[0..4]
 MAM_HD void hash_gl_leaf(u64 a, u64 b, u64 *out4) {
     u64 s[WIDTH] = {a, b, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    permute(s);
+    permute<4>(s);
     for (int i = 0; i < 4; i++) out4[i] = s[i];
 }
 
 // Leaf of an Ext2 codeword: permute([a0, a1, b0, b1, 0, ...])[0..4]
 MAM_HD void hash_ext2_leaf(const u64 *ab, u64 *out4) {
     u64 s[WIDTH] = {ab[0], ab[1], ab[2], ab[3], 0, 0, 0, 0, 0, 0, 0, 0};
-    permute(s);
+    permute<4>(s);
     for (int i = 0; i < 4; i++) out4[i] = s[i];
 }
 
 // Two-to-one node: permute([L0..L3, R0..R3, 0, 0, 0, 0])[0..4]
 MAM_HD void compress(const u64 *l4, const u64 *r4, u64 *out4) {
     u64 s[WIDTH] = {l4[0], l4[1], l4[2], l4[3], r4[0], r4[1], r4[2], r4[3], 0, 0, 0, 0};
-    permute(s);
+    permute<4>(s);
     for (int i = 0; i < 4; i++) out4[i] = s[i];
 }
 

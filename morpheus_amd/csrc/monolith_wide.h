@@ -31,44 +31,44 @@ __device__ __forceinline__ u64 rc_lane(int round, int j) {
     return r;
 }
 
-// y_j = sum_k ROW0[k] x_{(j + k) mod 12} + rc  (row j of the circulant M[j][c] = ROW0[(c - j) mod 12])
+// y_j = sum_k ROW0[k] x_{(j + k) mod 12} + rc  (row j of the circulant M[j][c] = ROW0[(c - j) mod 12]),
+// on 32-bit halves with 64-bit multiply-accumulate (as mono::concrete). Lanes j < 4 return
+// the canonical residue (the next bar inputs, or the digest after the last layer).
 template <int ROUND>
 __device__ __forceinline__ u64 concrete(u64 x, int j) {
-    u32 s0 = 0, s1 = 0, s2 = 0;
+    u64 a = 0, b = 0;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
         int src = j + k;
         src = src >= 12 ? src - 12 : src;
         src = src >= 12 ? src - 12 : src;
         const u64 v = shfl64(x, src);
-        const u32 m = mono::row0(k);
-        s0 += m * ((u32)v & 0x3FFFFFu);
-        s1 += m * ((u32)(v >> 22) & 0x3FFFFFu);
-        s2 += m * (u32)(v >> 44);
+        const u64 m = mono::row0(k);
+        a += (u64)(u32)v * m;
+        b += (u64)(u32)(v >> 32) * m;
     }
-    u64 a = (u64)s0 + ((u64)s1 << 22);
-    u64 b_lo = (u64)s2 << 44;
-    u64 hi = (u64)s2 >> 20;
-    u64 lo = a + b_lo;
-    hi += (lo < a);
+    u64 s = a + (b << 32);
+    u64 h = (b >> 32) + (s < a);
     if (ROUND >= 1 && ROUND <= 5) {
-        u64 k = rc_lane(ROUND, j < 12 ? j : 0);
-        u64 lo2 = lo + k;
-        hi += (lo2 < k);
-        lo = lo2;
+        const u64 k = rc_lane(ROUND, j < 12 ? j : 0);
+        u64 s2 = s + k;
+        h += (s2 < k);
+        s = s2;
     }
-    return mono::reduce_small_hi(lo, hi);
+    const u64 y = mono::fold_hi_nc(s, h);
+    return j < 4 ? gl_canon(y) : y;
 }
 
 template <int ROUND>
 __device__ __forceinline__ u64 round_fn(u64 x, int j) {
-    if (j < 4) x = gl_canon(mono::bar(x));
+    if (j < 4) x = mono::bar(x);
     const u64 prev = shfl64(x, j >= 1 ? j - 1 : 0);
-    if (j >= 1) x = gl_add(x, gl_sqr(prev));
+    if (j >= 1) x = mono::add_nc(x, mono::sqr_nc(prev));
     return concrete<ROUND>(x, j);
 }
 
-// x: this lane's state word (any 64-bit representative); returns the permuted word.
+// x: this lane's state word (any 64-bit representative); returns the permuted word,
+// canonical on lanes 0..3 (the digest lanes).
 __device__ __forceinline__ u64 permute(u64 x, int j) {
     x = concrete<0>(x, j);
     x = round_fn<1>(x, j);
