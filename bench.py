@@ -28,6 +28,13 @@ HBM_PEAK_GBS = 8000.0
 # 32-bit VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz (AMD's 157.3 TFLOP/s
 # FP32 vector figure counts an FMA as 2 flops on 2 packed lanes: 157.3 / 4 = 39.3).
 VALU_PEAK_TOPS = 39.3
+# VALU lane-operations of one Monolith permutation in the leaf kernel: SQ_INSTS_VALU x 64
+# over the permutations of one k_tree_levels<MonoHash, SRC_GL> dispatch (rocprofv3 --pmc,
+# profiles/merkle_leaf_kernel_2p22.md).
+VALU_OPS_PER_PERM = 6451.0
+# HBM bytes of that dispatch at 2^22 leaves: FETCH_SIZE x 2 (a 256 MiB copy reads back as
+# 128 MiB on this counter) + WRITE_SIZE (exact for the copy), same profile file.
+LEAF_LAUNCH_TRAFFIC_BYTES = (65.9 + 224.0) * 2 ** 20
 
 
 def parse():
@@ -35,12 +42,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["basefold", "sumcheck"], default="basefold")
+    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai"], default="basefold")
+    ap.add_argument("--batch", type=int, default=1, help="witnesses per Ajtai commit")
     ap.add_argument("--nv", type=int, default=None)
     ap.add_argument("--queries", type=int, default=100)
     ap.add_argument("--cpu-sample-nv", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--valu-ops-per-perm", type=float, default=float(os.environ.get("MAM_VALU_OPS_PER_PERM", "0")))
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 256 MiB device-to-device copy (PMC byte-counter calibration)")
     return ap.parse_args()
 
 
@@ -57,11 +66,11 @@ def gen_inputs(nv, seed=1):
 
 
 def profile_collect(lib):
-    ms = (ctypes.c_double * 4)()
-    cnt = (ctypes.c_long * 4)()
-    work = (ctypes.c_double * 4)()
+    ms = (ctypes.c_double * 6)()
+    cnt = (ctypes.c_long * 6)()
+    work = (ctypes.c_double * 6)()
     assert lib.mam_profile_collect(ms, cnt, work) == 0
-    return [(ms[i], cnt[i], work[i]) for i in range(4)]
+    return [(ms[i], cnt[i], work[i]) for i in range(6)]
 
 
 def cpu_baseline_basefold(nv, lr, nq):
@@ -91,6 +100,19 @@ def cpu_baseline_sumcheck(nv):
     dt = time.perf_counter() - t0
     return {"value": (2 << nv) / dt, "unit": "elems/s", "cores": 1, "kind": "port",
             "sample": f"oracle Ext2 sumcheck of f*g, 2 x 2^{nv} evaluations, single thread, {dt:.2f} s"}
+
+
+def cpu_baseline_ajtai(nv):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    from morpheus_amd.ajtai import key_from_seed
+    z = gen_inputs(nv, seed=11)
+    t0 = time.perf_counter()
+    O.ajtai_i8_commit(key_from_seed(2024), 256, 1 << nv, z, 1)
+    dt = time.perf_counter() - t0
+    return {"value": (1 << nv) / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+            "sample": f"oracle int8 Ajtai (ChaCha8 matrix generation + exact matvec), 256 x 2^{nv}, "
+                      f"single thread, {dt:.2f} s"}
 
 
 def bench_basefold(args, M):
@@ -135,20 +157,21 @@ def bench_basefold(args, M):
     prof = profile_collect(lib)
     lib.mam_profile_enable(0)
     verified = M.verify_ext2(root, pt, nv, lr, hash_kind, proof, nq, table, M.Transcript(b"bench"))
-    # dominant kernel group: Merkle builds (Monolith permutations)
+    # Dominant kernel: the commit tree's first launch, k_tree_levels<MonoHash, SRC_GL> hashing
+    # 2^22 base-field leaves and two in-LDS levels (7 * 2^20 Monolith permutations).
+    lk_ms, lk_n, lk_perm = prof[4]
+    lk_avg_ms = lk_ms / max(lk_n, 1)
+    perms_per_launch = lk_perm / max(lk_n, 1)
+    perm_rate = perms_per_launch / (lk_avg_ms / 1e3) if lk_ms > 0 else 0.0
+    ach = perm_rate * VALU_OPS_PER_PERM / 1e12
+    roof = {"kernel": "k_tree_levels<MonoHash, SRC_GL> (2^22 leaves + 2 in-LDS levels)",
+            "bound": "valu", "unit": "Tops/s", "peak": VALU_PEAK_TOPS, "achieved": ach,
+            "frac": ach / VALU_PEAK_TOPS, "traffic": LEAF_LAUNCH_TRAFFIC_BYTES if nv == 22 else None,
+            "avg_launch_ms": lk_avg_ms, "launches": lk_n, "perms_per_launch": perms_per_launch,
+            "perms_per_s": perm_rate, "valu_ops_per_perm": VALU_OPS_PER_PERM}
     mk_ms, mk_n, mk_perm = prof[0]
-    perm_rate = mk_perm / (mk_ms / 1e3) if mk_ms > 0 else 0.0
-    roof = {"kernel": "k_tree_levels<Monolith> (Merkle builds: leaves + in-LDS levels)",
-            "bound": "valu", "unit": "Tops/s", "peak": VALU_PEAK_TOPS,
-            "achieved": None, "frac": None, "traffic": None,
-            "perms_per_s": perm_rate, "avg_build_ms": mk_ms / max(mk_n, 1), "builds": mk_n,
-            "perms_per_build": mk_perm / max(mk_n, 1)}
-    if args.valu_ops_per_perm > 0:
-        # each thread runs one permutation's instruction stream: lane-ops = perms x instructions
-        ach = perm_rate * args.valu_ops_per_perm / 1e12
-        roof["achieved"] = ach
-        roof["frac"] = ach / VALU_PEAK_TOPS
-        roof["valu_ops_per_perm"] = args.valu_ops_per_perm
+    roof["all_merkle_builds"] = {"perms_per_s": mk_perm / (mk_ms / 1e3) if mk_ms > 0 else 0.0,
+                                 "avg_build_ms": mk_ms / max(mk_n, 1), "builds_per_step": mk_n / args.steps}
     hbm = {}
     for cat, name in ((1, "open_fold_oracle"), (2, "commit_transform")):
         ms, n, by = prof[cat]
@@ -208,6 +231,49 @@ def bench_sumcheck(args, M):
                          "avg_ms": ms / max(n, 1), "launches": n}}
 
 
+def bench_ajtai(args, M):
+    """int8-matrix Ajtai commit: 2^22-element witness(es), 256-row ChaCha8 matrix, INT8 MFMA."""
+    from morpheus_amd.ajtai import AjtaiI8, key_from_seed
+    nv = args.nv or 22
+    n, rows, batch = 1 << nv, 256, args.batch
+    lib = M.lib
+    t0 = time.perf_counter()
+    A = AjtaiI8(key_from_seed(2024), rows, n)
+    setup_s = time.perf_counter() - t0
+    z = np.concatenate([gen_inputs(nv, seed=10 + w) for w in range(batch)])
+    dz, dout = M.DeviceBuffer.from_numpy(z), M.DeviceBuffer(batch * rows)
+    for _ in range(args.warmup):
+        A.commit_device(dz, batch, dout)
+    M.synchronize()
+    lib.mam_profile_enable(1)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        A.commit_device(dz, batch, dout)
+        lat.append(time.perf_counter() - s0)
+    M.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = profile_collect(lib)
+    lib.mam_profile_enable(0)
+    ms, cnt, by = prof[5]
+    avg = ms / max(cnt, 1)
+    gbs = by / max(cnt, 1) / (avg / 1e3) / 1e9 if ms > 0 else 0.0
+    mfma_tops = 2.0 * rows * n * 8 * batch / (avg / 1e3) / 1e12 if ms > 0 else 0.0
+    lat_ms = np.array(lat) * 1e3
+    return {"metric": METRIC, "value": args.steps * batch * n / elapsed, "unit": "elems/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8->int32 (u64 out)",
+            "data": "synthetic",
+            "config": {"workload": "ajtai_int8_commit", "witness_len": n, "rows": rows, "batch": batch,
+                       "limbs": 8, "matrix": "ChaCha8 int8, expanded once", "parallelism": "single-gpu"},
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "setup_s": setup_s,
+            "roofline": {"kernel": "k_aj_gemm (v_mfma_i32_16x16x32_i8)", "bound": "hbm", "achieved": gbs,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                         "avg_launch_ms": avg, "mfma_int8_tops": mfma_tops, "mfma_int8_frac_of_5pops": mfma_tops / 5000.0}}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -219,12 +285,29 @@ def main():
     if M.lib.goldilocks_cuda_init() != 0:
         print("no usable GPU", file=sys.stderr)
         sys.exit(1)
-    out = bench_basefold(args, M) if args.workload == "basefold" else bench_sumcheck(args, M)
+    if args.calibrate_copy:
+        n = 1 << 25
+        a, b = M.DeviceBuffer(n), M.DeviceBuffer(n)
+        M.check(M.lib.gl_memset_zero(a.ptr, n), "zero")
+        for _ in range(2):
+            t0 = time.perf_counter()
+            M.check(M.lib.cuda_memcpy_dtod(b.ptr, a.ptr, n * 8), "copy")
+            dt = time.perf_counter() - t0
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "wall_ms": dt * 1e3}), flush=True)
+        return
+    if args.workload == "basefold":
+        out = bench_basefold(args, M)
+    elif args.workload == "sumcheck":
+        out = bench_sumcheck(args, M)
+    else:
+        out = bench_ajtai(args, M)
     if not args.no_cpu:
         if args.workload == "basefold":
             out["cpu_baseline"] = cpu_baseline_basefold(args.cpu_sample_nv or 18, 1, args.queries)
-        else:
+        elif args.workload == "sumcheck":
             out["cpu_baseline"] = cpu_baseline_sumcheck(args.cpu_sample_nv or 20)
+        else:
+            out["cpu_baseline"] = cpu_baseline_ajtai(args.cpu_sample_nv or 16)
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
         out["cpu_baseline"]["nproc"] = os.cpu_count()
     print(json.dumps(out), flush=True)

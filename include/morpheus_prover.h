@@ -78,6 +78,8 @@ int mam_basefold_commitment_wrap(uint64_t *d_codeword, uint64_t *d_bh_evals, uin
 int mam_basefold_commitment_buffers(void *handle, uint64_t **d_codeword, uint64_t **d_bh_evals, uint64_t **d_tree,
                                     uint64_t *root_out);
 int mam_basefold_commitment_free(void *handle);
+/* Free the calling thread's cached scratch (re-created on demand). */
+int mam_release_scratch(void);
 /* BasefoldCommitment::open_ext2 (:578-863) with extract_ext2_queries (:2310-2402).
  * point2: num_vars Ext2 (host); d_table: device table generated for table_nv >= num_vars;
  * proof: mam_basefold_proof_len() words in the flat layout
@@ -91,6 +93,19 @@ int mam_basefold_verify_ext2(const uint64_t *root4, const uint64_t *point2, int 
                              const uint64_t *proof, int num_queries, const uint64_t *table, int table_nv,
                              mam_transcript *t);
 
+/* ---- int8-matrix Ajtai commitment on INT8 MFMA (benchmark configuration 3) ----
+ * No reference function has these semantics (the reference's Ajtai is the ring-SIS commit
+ * of almost-goldilocks-cuda-rs/src/ajtai.rs:138-386; its INT8 paths are a multifold GEMM,
+ * cuda_almost_goldilocks/ajtai_tc.cuh:166-225). Defined here:
+ *   M_ij = (int8) byte (j mod 64) of ChaCha8(key, counter = j / 64, nonce = (i, 0, 0)),
+ *   out[w * rows + i] = sum_j M_ij z_w[j]  (mod p).
+ * setup expands M once (rows * n bytes of HBM); rows a multiple of 16 up to 256, n a
+ * multiple of 64. commit: batch <= 32 witnesses of n words each (device), out device. */
+int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void **handle);
+int mam_ajtai_i8_commit(void *handle, const uint64_t *d_z, int batch, uint64_t *d_out);
+int mam_ajtai_i8_matrix_row(void *handle, int row, uint64_t col0, uint64_t count, int8_t *h_out);
+int mam_ajtai_i8_free(void *handle);
+
 /* ---- device primitives without a reference symbol of their own ---- */
 /* Monolith permutation of n 12-word states in place (the kernel behind the reference's
  * monolith_merkle_tree_* builders, cuda/monolith.cuh:94-216) */
@@ -102,7 +117,8 @@ int mam_merkle_tree_async(const uint64_t *d_codeword, uint64_t *d_tree, size_t n
 /* Opt-in timing of kernel groups with HIP events on their launch stream (no reference
  * counterpart; used by bench.py). Categories: 0 Merkle builds (work = permutations),
  * 1 opening fold+oracle (bytes), 2 commit transform (bytes), 3 sumcheck fold+message
- * (bytes). collect() fills 4 entries of each array and clears the record. */
+ * (bytes), 4 single leaf-hashing launches over base-field codewords (permutations),
+ * 5 int8 Ajtai GEMM launches (bytes). collect() fills 6 entries of each array. */
 int mam_profile_enable(int on);
 int mam_profile_collect(double *ms, long *count, double *work);
 

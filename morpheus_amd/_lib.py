@@ -136,7 +136,12 @@ _SIGS = {
     "mam_basefold_verify_ext2": (ci, [u64p, u64p, ci, ci, ci, u64p, ci, u64p, ci, trp]),
     "mam_monolith_permute_batch": (ci, [vp, ci]),
     "mam_merkle_tree_async": (ci, [vp, vp, csz, ci, ci]),
+    "mam_ajtai_i8_setup": (ci, [ctypes.c_char_p, ci, u64, ctypes.POINTER(vp)]),
+    "mam_ajtai_i8_commit": (ci, [vp, vp, ci, vp]),
+    "mam_ajtai_i8_matrix_row": (ci, [vp, ci, u64, u64, ctypes.POINTER(ctypes.c_int8)]),
+    "mam_ajtai_i8_free": (ci, [vp]),
     "mam_profile_enable": (ci, [ci]),
+    "mam_release_scratch": (ci, []),
     "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),
 }
 

@@ -279,7 +279,8 @@ MAM_API int mam_profile_enable(int on) {
 
 // Totals per category (PROF_NCAT entries each): milliseconds, scopes, work units.
 // Categories: 0 Merkle builds (work = permutations), 1 opening fold+oracle (bytes),
-// 2 commit transform (bytes), 3 sumcheck fold+message (bytes).
+// 2 commit transform (bytes), 3 sumcheck fold+message (bytes), 4 single leaf-hashing
+// launches of base-field codewords (permutations), 5 int8 Ajtai GEMM launches (bytes).
 MAM_API int mam_profile_collect(double *ms, long *count, double *work) {
     std::lock_guard<std::mutex> g(g_prof_mu);
     for (int c = 0; c < PROF_NCAT; c++) {

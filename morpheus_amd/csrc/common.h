@@ -22,6 +22,10 @@ static constexpr int MAM_BLOCK = 256;
 
 static inline hipStream_t mam_stream() { return hipStreamPerThread; }
 
+// Workgroups of an in-launch (ticketed) grid reduction: 2 per CU keeps HBM streams full
+// while bounding the contention on the single ticket counter.
+static constexpr unsigned MAM_RED_GRID = 512;
+
 static inline unsigned grid_1d(size_t n, int block = MAM_BLOCK) {
     size_t g = (n + block - 1) / block;
     if (g == 0) g = 1;

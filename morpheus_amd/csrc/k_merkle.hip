@@ -243,6 +243,10 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
         if (nl > total_levels - done) nl = total_levels - done;
         unsigned grid = (unsigned)((n + TREE_BLOCK - 1) / TREE_BLOCK);
         if (first) {
+            // the single dominant kernel: base-field leaves plus in-LDS levels
+            double launch_perms = 0;
+            for (int l = 0; l < nl; l++) launch_perms += (double)(n >> l);
+            ProfScope leaf_prof(leaf_src == SRC_GL && H::kWide ? PROF_LEAF_LAUNCH : -1, launch_perms, s);
             switch (leaf_src) {
                 case SRC_GL: k_tree_levels<H, SRC_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs); break;
                 case SRC_EXT2:

@@ -76,9 +76,12 @@ MAM_HD u64 add_nc(u64 a, u64 b) {
 }
 
 // x^2 reduced to a 64-bit representative (not necessarily canonical).
+// With x = l + 2^32 h: x^2 = l^2 + 2^33 l h + 2^64 h^2 (three 32x32 products).
 MAM_HD u64 sqr_nc(u64 x) {
-    u64 lo, hi;
-    mul_wide(x, x, lo, hi);
+    const u64 l = (u32)x, h = x >> 32;
+    const u64 ll = l * l, lh = l * h, hh2 = h * h;
+    u64 lo = ll + (lh << 33);
+    u64 hi = hh2 + (lh >> 31) + (lo < ll);
     u64 hh = hi >> 32, hl = hi & GL_EPS;
     u64 t0 = lo - hh;
     if (lo < hh) t0 -= GL_EPS;
@@ -98,6 +101,34 @@ MAM_HD u64 fold_hi_nc(u64 lo, u64 hi) {
 
 // Canonical residue of lo + 2^64 * hi with hi < 2^32.
 MAM_HD u64 reduce_small_hi(u64 lo, u64 hi) { return gl_canon(fold_hi_nc(lo, hi)); }
+
+// y = a + 2^32 b + k (mod p) for a, b < 2^41 and the 64-bit constant k, as a 64-bit
+// representative, with 32-bit add-with-carry steps:
+//   y = (a0 + k0) + 2^32 (a1 + b0 + k1) + 2^64 b1 = (w1:w0) + 2^64 w2,
+//   then 2^64 w2 = w2 (2^32 - 1) and one wrap-around of the final 64-bit add is 2^32 - 1.
+MAM_HD u32 addc32(u32 x, u32 y, u32 cin, u32 &cout) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_addc(x, y, cin, &cout);
+#else
+    u64 s = (u64)x + y + cin;
+    cout = (u32)(s >> 32);
+    return (u32)s;
+#endif
+}
+
+MAM_HD u64 combine(u64 a, u64 b, u64 k) {
+    u32 c0, c1, c2;
+    const u32 w0 = addc32((u32)a, (u32)k, 0, c0);
+    u32 w1 = addc32((u32)b, (u32)(k >> 32), c0, c1);
+    w1 = addc32(w1, (u32)(a >> 32), 0, c2);
+    const u64 w2 = (u64)((u32)(b >> 32) + c1 + c2);  // < 2^10
+    return fold_hi_nc(((u64)w1 << 32) | w0, w2);
+}
+
+template <int ROUND>
+MAM_HD u64 combine_row(u64 a, u64 b, int r) {
+    return combine(a, b, (ROUND >= 1 && ROUND <= 5) ? rc(ROUND, r) : 0);
+}
 
 // y_r = sum_c ROW0[(c - r) mod 12] * x_c + rc_r on 32-bit halves of any representatives:
 // sum_c m * lo_c and sum_c m * hi_c are each below 12 * 26 * 2^32 < 2^41 (one 32x32+64
@@ -120,16 +151,8 @@ MAM_HD void concrete(u64 *x) {
             a += (u64)lo[c] * m;
             b += (u64)hi[c] * m;
         }
-        u64 s = a + (b << 32);
-        u64 h = (b >> 32) + (s < a);
-        if (ROUND >= 1 && ROUND <= 5) {
-            const u64 k = rc(ROUND, r);
-            u64 s2 = s + k;
-            h += (s2 < k);
-            s = s2;
-        }
-        const u64 y = fold_hi_nc(s, h);
-        x[r] = r < CANON_ROWS ? gl_canon(y) : y;
+        x[r] = combine_row<ROUND>(a, b, r);
+        if (r < CANON_ROWS) x[r] = gl_canon(x[r]);
     }
 }
 

@@ -6,7 +6,15 @@
 
 #include <hip/hip_runtime.h>
 
-enum MamProfCat { PROF_MERKLE = 0, PROF_OPEN_SUMCHECK = 1, PROF_COMMIT_TRANSFORM = 2, PROF_SUMCHECK = 3, PROF_NCAT = 4 };
+enum MamProfCat {
+    PROF_MERKLE = 0,
+    PROF_OPEN_SUMCHECK = 1,
+    PROF_COMMIT_TRANSFORM = 2,
+    PROF_SUMCHECK = 3,
+    PROF_LEAF_LAUNCH = 4,  // one k_tree_levels<Monolith> launch hashing base-field leaves
+    PROF_AJTAI_GEMM = 5,   // one k_aj_gemm launch (work = matrix + limb-plane bytes)
+    PROF_NCAT = 6
+};
 
 bool mam_prof_enabled();
 void mam_prof_push(int cat, double work, hipEvent_t a, hipEvent_t b);
@@ -17,7 +25,7 @@ struct ProfScope {
     hipStream_t s;
     hipEvent_t a = nullptr;
     ProfScope(int c, double w, hipStream_t st) : cat(c), work(w), s(st) {
-        if (mam_prof_enabled() && hipEventCreate(&a) == hipSuccess) {
+        if (c >= 0 && mam_prof_enabled() && hipEventCreate(&a) == hipSuccess) {
             if (hipEventRecord(a, s) != hipSuccess) {
                 (void)hipEventDestroy(a);
                 a = nullptr;

@@ -108,6 +108,13 @@ int or_basefold_verify_ext2(const uint64_t *root4, const uint64_t *point2, int n
                             const uint64_t *proof, int nq, const uint64_t *table, int table_nv,
                             or_transcript *t);
 
+/* ---- ChaCha and the int8-matrix Ajtai commitment (oracle/ajtai.c) ----
+ * ChaCha block: pinned by the RFC 8439 section 2.3.2 ChaCha20 vector.
+ * Commitment: exact integer matrix-vector product (checked against Python big ints). */
+void or_chacha_block(int rounds, const uint32_t *key8, uint32_t counter, const uint32_t *nonce3, uint32_t *out16);
+void or_ajtai_i8_row(const uint8_t *key32, uint32_t row, uint64_t n, int8_t *out);
+void or_ajtai_i8_commit(const uint8_t *key32, int rows, uint64_t n, const uint64_t *z, int batch, uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

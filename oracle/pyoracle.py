@@ -49,6 +49,10 @@ _sigs = {
     "or_basefold_proof_len": (cl, [ci, ci, ci]),
     "or_basefold_open_ext2": (None, [u64p, u64p, u64p, ci, ci, ci, u64p, u64p, ci, trp, ci, u64p]),
     "or_basefold_verify_ext2": (ci, [u64p, u64p, ci, ci, ci, u64p, ci, u64p, ci, trp]),
+    "or_chacha_block": (None, [ci, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
+                               ctypes.POINTER(ctypes.c_uint32)]),
+    "or_ajtai_i8_row": (None, [ctypes.c_char_p, ctypes.c_uint32, u64, ctypes.POINTER(ctypes.c_int8)]),
+    "or_ajtai_i8_commit": (None, [ctypes.c_char_p, ci, u64, u64p, ci, u64p]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(L, _n)
@@ -180,3 +184,24 @@ def basefold_open_ext2(cw, bh, root, nv, lr, hash_kind, point2, table, table_nv,
 def basefold_verify_ext2(root, point2, nv, lr, hash_kind, proof, nq, table, table_nv, t):
     return bool(L.or_basefold_verify_ext2(_p(arr(root)), _p(arr(point2)), nv, lr, hash_kind, _p(arr(proof)), nq,
                                           _p(arr(table)), table_nv, ctypes.byref(t.t)))
+
+
+def chacha_block(rounds, key_words8, counter, nonce3):
+    k = np.ascontiguousarray(key_words8, dtype=np.uint32)
+    nn = np.ascontiguousarray(nonce3, dtype=np.uint32)
+    out = np.zeros(16, dtype=np.uint32)
+    U32 = ctypes.POINTER(ctypes.c_uint32)
+    L.or_chacha_block(rounds, k.ctypes.data_as(U32), counter, nn.ctypes.data_as(U32), out.ctypes.data_as(U32))
+    return out
+
+
+def ajtai_i8_row(key32, row, n):
+    out = np.zeros(n, dtype=np.int8)
+    L.or_ajtai_i8_row(bytes(key32), row, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)))
+    return out
+
+
+def ajtai_i8_commit(key32, rows, n, z, batch):
+    out = np.zeros(batch * rows, dtype=np.uint64)
+    L.or_ajtai_i8_commit(bytes(key32), rows, n, _p(arr(z)), batch, _p(out))
+    return out

@@ -1,0 +1,48 @@
+"""GPU parity of the int8-matrix Ajtai commitment (MFMA) against the oracle's exact matvec."""
+import numpy as np
+import pytest
+
+import pyoracle as O
+
+import morpheus_amd as M
+from morpheus_amd.ajtai import AjtaiI8, key_from_seed
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def gpu_init():
+    assert M.lib.goldilocks_cuda_init() == 0
+
+
+@pytest.mark.parametrize("rows,n,batch", [(256, 4096, 1), (256, 4096, 3), (16, 64, 1), (32, 1 << 15, 2),
+                                          (256, (1 << 14) + 320, 8), (64, 8192, 32)])
+def test_ajtai_i8_matches_oracle(rows, n, batch):
+    print("ajtai", rows, n, batch, flush=True)
+    key = key_from_seed(7 + rows)
+    rng = np.random.default_rng(rows + n + batch)
+    z = rng.integers(0, 1 << 64, batch * n, dtype=np.uint64)  # includes non-canonical words
+    A = AjtaiI8(key, rows, n)
+    got = A.commit(z, batch)
+    want = O.ajtai_i8_commit(key, rows, n, z, batch).reshape(batch, rows)
+    assert np.array_equal(got, want)
+    for r in (0, rows - 1):
+        assert np.array_equal(A.matrix_row(r, 0, min(n, 256)), O.ajtai_i8_row(key, r, n)[:min(n, 256)])
+    A.free()
+
+
+def test_ajtai_i8_extremes_and_linearity():
+    key = key_from_seed(99)
+    rows, n = 256, 4096
+    A = AjtaiI8(key, rows, n)
+    P = M.P
+    for fill in (0, P - 1, (1 << 64) - 1):
+        z = np.full(n, fill, dtype=np.uint64)
+        assert np.array_equal(A.commit(z)[0], O.ajtai_i8_commit(key, rows, n, z, 1))
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, P, n, dtype=np.uint64)
+    b = rng.integers(0, P, n, dtype=np.uint64)
+    s = np.array([(int(x) + int(y)) % P for x, y in zip(a, b)], dtype=np.uint64)
+    ca, cb, cs = A.commit(a)[0], A.commit(b)[0], A.commit(s)[0]
+    assert [int(v) for v in cs] == [(int(x) + int(y)) % P for x, y in zip(ca, cb)]
+    A.free()

@@ -277,3 +277,30 @@ def test_basefold_open_verifies_and_evaluates(nv, lr, nq, hash_kind):
         bad[pos] = (int(bad[pos]) + 1) % P
         assert not O.basefold_verify_ext2(root, pt, nv, lr, hash_kind, bad, nq, table, nv, O.Transcript(b"bf"))
     assert not O.basefold_verify_ext2(root, pt, nv, lr, hash_kind, proof, nq, table, nv, O.Transcript(b"other"))
+
+
+def test_chacha20_rfc8439_vector():
+    g = GOLDEN["chacha20_rfc8439_2_3_2"]
+    kb = bytes(g["key_bytes"])
+    kw = [int.from_bytes(kb[4 * i:4 * i + 4], "little") for i in range(8)]
+    assert [int(x) for x in O.chacha_block(20, kw, g["counter"], g["nonce_words"])] == g["output_words"]
+
+
+def test_ajtai_i8_is_exact_matvec():
+    key = bytes(range(100, 132))
+    rows, n, batch = 16, 256, 3
+    rng = np.random.default_rng(60)
+    z = rng.integers(0, 1 << 64, batch * n, dtype=np.uint64)
+    got = O.ajtai_i8_commit(key, rows, n, z, batch)
+    kw = [int.from_bytes(key[4 * i:4 * i + 4], "little") for i in range(8)]
+    for i in range(rows):
+        # row i from the ChaCha8 stream: byte (j mod 64) of block j / 64 with nonce (i, 0, 0)
+        row = []
+        for b in range(n // 64):
+            w = O.chacha_block(8, kw, b, [i, 0, 0])
+            row += [((int(w[t // 4]) >> (8 * (t % 4))) & 0xFF) for t in range(64)]
+        row = [v - 256 if v >= 128 else v for v in row]
+        assert [int(v) for v in O.ajtai_i8_row(key, i, n)] == row
+        for w_ in range(batch):
+            want = sum(row[j] * (int(z[w_ * n + j]) % P) for j in range(n)) % P
+            assert int(got[w_ * rows + i]) == want
