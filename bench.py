@@ -35,6 +35,9 @@ VALU_OPS_PER_PERM = 6451.0
 # HBM bytes of that dispatch at 2^22 leaves: FETCH_SIZE x 2 (a 256 MiB copy reads back as
 # 128 MiB on this counter) + WRITE_SIZE (exact for the copy), same profile file.
 LEAF_LAUNCH_TRAFFIC_BYTES = (65.9 + 224.0) * 2 ** 20
+# HBM bytes of one int8 Ajtai GEMM launch (witness 2^22, 256 rows, batch 1): FETCH_SIZE
+# 540765 KiB x 2.0 (copy calibration) + WRITE_SIZE 8192 KiB, from the rocprofv3 --pmc runs.
+AJTAI_GEMM_TRAFFIC_BYTES = (540765.0 * 2.0 + 8192.0) * 1024
 
 
 def parse():
@@ -269,8 +272,10 @@ def bench_ajtai(args, M):
                        "limbs": 8, "matrix": "ChaCha8 int8, expanded once", "parallelism": "single-gpu"},
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
             "setup_s": setup_s,
-            "roofline": {"kernel": "k_aj_gemm (v_mfma_i32_16x16x32_i8)", "bound": "hbm", "achieved": gbs,
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+            "roofline": {"kernel": "k_aj_gemm (v_mfma_i32_16x16x64_i8, LDS-staged, fused limb split)",
+                         "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS,
+                         "traffic": AJTAI_GEMM_TRAFFIC_BYTES if (nv == 22 and batch == 1) else None,
                          "avg_launch_ms": avg, "mfma_int8_tops": mfma_tops, "mfma_int8_frac_of_5pops": mfma_tops / 5000.0}}
 
 

@@ -14,16 +14,21 @@
 //   c_i   = sum_l 2^(8l) (sum_j M_ij d'_jl + 128 sum_j M_ij)   (mod p)   = M z (mod p)
 //
 // MI355X layout and kernels:
-// * M is expanded once (setup) into HBM pre-tiled in MFMA A-fragment order: for every
-//   32-column k-step and 16-row tile, the 64 lanes' 8-byte fragments are 512 contiguous
-//   bytes, so each wave streams M with fully coalesced 8-byte loads. Row sums are
-//   accumulated during expansion.
-// * The witnesses become 8*B limb planes of n bytes (B-fragments: 8 contiguous bytes).
-// * The GEMM runs v_mfma_i32_16x16x32_i8 over K-chunks of 2^14 columns (|chunk sum| <=
-//   2^14 * 2^14 = 2^28, exact in int32); one workgroup per chunk, one wave per row tile,
-//   one accumulator tile per 16 limb columns; int32 chunk partials are then summed in int64
-//   and recombined mod p. At B = 1 the kernel streams the 1 GiB matrix once (HBM-bound);
-//   the A fragment is reused across all column tiles, so larger B moves it toward MFMA.
+// * M is expanded once (setup) into HBM pre-tiled in MFMA A-fragment order for
+//   v_mfma_i32_16x16x64_i8: for every 64-column k-step and 16-row tile, the 64 lanes'
+//   16-byte fragments are 1 KiB contiguous (lane l = r + 16 g holds row r, columns
+//   16 g .. 16 g + 15 of the step), so every operand load is a coalesced 16-byte load.
+//   Row sums are accumulated during expansion.
+// * The witnesses are split into 8 limb planes while a workgroup stages them into LDS, in
+//   the same fragment order (lane l = c + 16 g holds plane c of a 16-plane tile, columns
+//   16 g .. 16 g + 15). A and B use the same (lane group, byte) -> column assignment, so the
+//   MFMA's own k order is irrelevant.
+// * The GEMM: one workgroup per K-chunk covers all rows and all planes; it stages DEPTH
+//   k-steps of A and of the limb planes in LDS (double-buffered: the next group's global
+//   loads overlap this group's MFMAs), so every HBM byte is read once; each wave owns an
+//   RTW x CTW block of 16x16 accumulator tiles. |chunk sum| <= 2^14 * chunk <= 2^28, exact in
+//   int32; chunk partials are summed in int64 and recombined mod p by the finish kernel.
+//   At B = 1 the kernel streams the 1 GiB matrix once (HBM-bound).
 #include <string.h>
 
 #include "chacha.h"
@@ -32,24 +37,22 @@
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-static constexpr int AJ_TILE_BYTES = 512;        // 16 rows x 32 columns
-static constexpr size_t AJ_KCHUNK = (size_t)1 << 14;  // columns per workgroup
+static constexpr int AJ_TILE_BYTES = 1024;  // 16 rows (or planes) x 64 columns
 static constexpr int AJ_MAX_BATCH = 32;
 
-// byte offset of M[i][j] in the tiled layout with RT row tiles
-__host__ __device__ __forceinline__ size_t aj_offset(size_t i, size_t j, int RT) {
-    const size_t kt = j >> 5, kc = j & 31, rt = i >> 4, r = i & 15;
-    return (kt * RT + rt) * AJ_TILE_BYTES + (((kc >> 3) << 4) + r) * 8 + (kc & 7);
+// byte offset of entry (i, j) of a tiled operand with `tiles` 16-row tiles per k-step
+__host__ __device__ __forceinline__ size_t aj_offset(size_t i, size_t j, size_t tiles) {
+    const size_t kt = j >> 6, g = (j >> 4) & 3, e = j & 15, t = i >> 4, r = i & 15;
+    return ((kt * tiles + t) * 64 + r + 16 * g) * 16 + e;
 }
 
-// One thread per (row, ChaCha block of 64 columns).
-__global__ void k_aj_expand(const u32 *__restrict__ key_words, int rows, size_t n, u64 *__restrict__ mt,
+// One thread per (ChaCha block of 64 columns, row); rows fastest so stores coalesce.
+__global__ void k_aj_expand(const u32 *__restrict__ key_words, int rows, size_t n, uint4 *__restrict__ mt,
                             unsigned long long *__restrict__ rowsum) {
-    const size_t blocks_per_row = n / 64;
     const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    if (t >= (size_t)rows * blocks_per_row) return;
-    const u32 i = (u32)(t / blocks_per_row);
-    const u32 b = (u32)(t % blocks_per_row);
+    if (t >= (size_t)rows * (n / 64)) return;
+    const u32 i = (u32)(t % rows);
+    const u32 b = (u32)(t / rows);
     u32 key[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) key[k] = key_words[k];
@@ -59,87 +62,185 @@ __global__ void k_aj_expand(const u32 *__restrict__ key_words, int rows, size_t 
     const int RT = rows / 16;
     long long sum = 0;
 #pragma unroll
-    for (int g = 0; g < 8; g++) {
-        const u64 v = ((u64)w[2 * g + 1] << 32) | w[2 * g];  // columns 64b + 8g .. + 7
-        mt[aj_offset(i, (size_t)b * 64 + 8 * g, RT) / 8] = v;
+    for (int g = 0; g < 4; g++) {
+        mt[aj_offset(i, (size_t)b * 64 + 16 * g, RT) / 16] = make_uint4(w[4 * g], w[4 * g + 1], w[4 * g + 2], w[4 * g + 3]);
 #pragma unroll
-        for (int k = 0; k < 8; k++) sum += (signed char)(v >> (8 * k));
+        for (int k = 0; k < 4; k++) {
+            const u32 v = w[4 * g + k];
+            sum += (signed char)v + (signed char)(v >> 8) + (signed char)(v >> 16) + (signed char)(v >> 24);
+        }
     }
     atomicAdd(&rowsum[i], (unsigned long long)sum);
 }
 
-// planes[(w * 8 + l) * n + j] = byte l of canonical z_w[j], XOR 0x80 (= byte - 128 as int8).
-// One thread per (witness, 8 consecutive columns): reads 64 bytes, writes one word per plane.
-__global__ void k_aj_limbs(const u64 *__restrict__ z, int batch, size_t n, u64 *__restrict__ planes) {
-    const size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-    const size_t groups = n / 8;
-    if (t >= (size_t)batch * groups) return;
-    const size_t w = t / groups, g = t % groups;
-    u64 v[8];
+// limb[l][q] = bytes l of w[0..3] packed little-endian (a 4 x 8 byte transpose, 16 v_perm).
+// v_perm_b32(hi, lo, sel): result byte k = byte sel_k of the 8-byte value hi:lo.
+__device__ __forceinline__ void aj_transpose4(const u64 w[4], u32 limb[8][4], int q) {
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = gl_canon(z[w * n + 8 * g + k]);
-#pragma unroll
-    for (int l = 0; l < 8; l++) {
-        u64 out = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) out |= ((v[k] >> (8 * l)) & 0xFF) << (8 * k);
-        planes[((w * 8 + l) * n) / 8 + g] = out ^ 0x8080808080808080ULL;
+    for (int h = 0; h < 2; h++) {
+        const u32 x0 = (u32)(w[0] >> (32 * h)), x1 = (u32)(w[1] >> (32 * h));
+        const u32 x2 = (u32)(w[2] >> (32 * h)), x3 = (u32)(w[3] >> (32 * h));
+        const u32 a = __builtin_amdgcn_perm(x1, x0, 0x05010400u);  // x0.b0 x1.b0 x0.b1 x1.b1
+        const u32 b = __builtin_amdgcn_perm(x1, x0, 0x07030602u);  // x0.b2 x1.b2 x0.b3 x1.b3
+        const u32 c = __builtin_amdgcn_perm(x3, x2, 0x05010400u);
+        const u32 d = __builtin_amdgcn_perm(x3, x2, 0x07030602u);
+        limb[4 * h + 0][q] = __builtin_amdgcn_perm(c, a, 0x05040100u);
+        limb[4 * h + 1][q] = __builtin_amdgcn_perm(c, a, 0x07060302u);
+        limb[4 * h + 2][q] = __builtin_amdgcn_perm(d, b, 0x05040100u);
+        limb[4 * h + 3][q] = __builtin_amdgcn_perm(d, b, 0x07060302u);
     }
 }
 
-// grid: one workgroup per K-chunk; block: 64 * RT threads (wave w = row tile w).
-// partial[(chunk * rows + row) * (CT * 16) + col] = sum over the chunk (int32).
-template <int CT>
-__global__ __launch_bounds__(1024) void k_aj_gemm(const u64 *__restrict__ mt, const u64 *__restrict__ planes,
-                                                  int RT, size_t n, int nplanes, int *__restrict__ partial) {
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const size_t steps = n >> 5;
-    const size_t kt0 = (size_t)blockIdx.x * (AJ_KCHUNK >> 5);
-    size_t kt1 = kt0 + (AJ_KCHUNK >> 5);
+// grid: one workgroup per K-chunk of `kchunk` columns; block: 512 threads; wave
+// (wr, wc) < (WR, CT / CTW) owns row tiles wr*RTW .. +RTW-1 (< RT) and plane tiles
+// wc*CTW .. +CTW-1. partial[(chunk * rows + row) * (CTT * 16) + col] = chunk sum (int32).
+static constexpr int AJF_THREADS = 512;
+
+template <int RTW, int CTW, int CT, int DEPTH>
+__global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__ mt, const u64 *__restrict__ z,
+                                                               int RT, int batch, int WR, size_t n, size_t kchunk,
+                                                               int *__restrict__ partial) {
+    constexpr int TILES = 16 + CT;
+    constexpr int A_ITEMS = DEPTH * 16 * 64 / AJF_THREADS;  // v4i per thread (full RT)
+    __shared__ v4i lds[2][DEPTH][TILES][64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = wave % WR, wc = wave / WR;
+    const bool computes = wave < WR * (CT / CTW);
+    const int lc = lane & 15, q = lane >> 4;
+    const int CTT = (8 * batch + 15) / 16;
+    const size_t steps = n >> 6;
+    const size_t kt0 = (size_t)blockIdx.x * (kchunk >> 6);
+    size_t kt1 = kt0 + (kchunk >> 6);
     if (kt1 > steps) kt1 = steps;
-    v4i acc[CT];
+
+    // staging registers: A fragments, and one (witness, 16-column group) of words per thread
+    v4i ra[A_ITEMS];
+    u64 rz[16];
+    const int a_per_step = RT * 64;
+    const int b_items = DEPTH * batch * 4;
+    auto stage_load = [&](size_t kt) {
 #pragma unroll
-    for (int c = 0; c < CT; c++) acc[c] = v4i{0, 0, 0, 0};
-    const int q = lane >> 4;  // k sub-block of this lane's fragment
-    const int lc = lane & 15;
-    for (size_t kt = kt0; kt < kt1; kt++) {
-        const u64 a = mt[(kt * RT + w) * (AJ_TILE_BYTES / 8) + lane];
-#pragma unroll
-        for (int c = 0; c < CT; c++) {
-            const int col = c * 16 + lc;
-            const u64 b = col < nplanes ? planes[((size_t)col * n + kt * 32 + q * 8) / 8] : 0;
-            acc[c] = __builtin_amdgcn_mfma_i32_16x16x32_i8((long)a, (long)b, acc[c], 0, 0, 0);
+        for (int m = 0; m < A_ITEMS; m++) {
+            const int e = threadIdx.x + m * AJF_THREADS;
+            const int d = e / (16 * 64), r = e % (16 * 64);
+            ra[m] = v4i{0, 0, 0, 0};
+            if (r < a_per_step && kt + d < kt1) ra[m] = mt[(kt + d) * a_per_step + r];
         }
+        const int e = threadIdx.x;
+        if (e < b_items) {
+            const int d = e / (batch * 4), r = e % (batch * 4);
+            if (kt + d < kt1) {
+                const u64 *src = z + (size_t)(r >> 2) * n + (kt + d) * 64 + (r & 3) * 16;
+#pragma unroll
+                for (int k = 0; k < 16; k++) rz[k] = src[k];
+            }
+        }
+    };
+    auto stage_store = [&](int buf, size_t kt) {
+#pragma unroll
+        for (int m = 0; m < A_ITEMS; m++) {
+            const int e = threadIdx.x + m * AJF_THREADS;
+            const int d = e / (16 * 64), r = e % (16 * 64);
+            if (r < a_per_step) lds[buf][d][r >> 6][r & 63] = ra[m];
+        }
+        const int e = threadIdx.x;
+        if (e < b_items) {
+            const int d = e / (batch * 4), r = e % (batch * 4);
+            if (kt + d < kt1) {
+                const int w = r >> 2, g = r & 3;
+#pragma unroll
+                for (int k = 0; k < 16; k++) rz[k] = gl_canon(rz[k]);
+                u32 limb[8][4];  // limb[l][q] = byte l of words 4q .. 4q + 3
+#pragma unroll
+                for (int qq = 0; qq < 4; qq++) aj_transpose4(&rz[4 * qq], limb, qq);
+#pragma unroll
+                for (int l = 0; l < 8; l++) {
+                    v4i o;
+#pragma unroll
+                    for (int qq = 0; qq < 4; qq++) o[qq] = (int)(limb[l][qq] ^ 0x80808080u);
+                    const int p = 8 * w + l;
+                    lds[buf][d][16 + (p >> 4)][(p & 15) + 16 * g] = o;
+                }
+            }
+        }
+    };
+
+    v4i acc[RTW][CTW];
+#pragma unroll
+    for (int i = 0; i < RTW; i++)
+#pragma unroll
+        for (int j = 0; j < CTW; j++) acc[i][j] = v4i{0, 0, 0, 0};
+
+    stage_load(kt0);
+    stage_store(0, kt0);
+    __syncthreads();
+    int cur = 0;
+    for (size_t kt = kt0; kt < kt1; kt += DEPTH) {
+        const bool more = kt + DEPTH < kt1;
+        if (more) stage_load(kt + DEPTH);
+        if (computes) {
+#pragma unroll
+            for (int d = 0; d < DEPTH; d++) {
+                v4i a[RTW];
+#pragma unroll
+                for (int i = 0; i < RTW; i++) a[i] = lds[cur][d][min(wr * RTW + i, 15)][lane];
+#pragma unroll
+                for (int j = 0; j < CTW; j++) {
+                    const v4i b = lds[cur][d][16 + wc * CTW + j][lane];
+#pragma unroll
+                    for (int i = 0; i < RTW; i++)
+                        acc[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b, acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+        if (more) stage_store(cur ^ 1, kt + DEPTH);
+        __syncthreads();
+        cur ^= 1;
     }
-    const int rows = RT * 16;
+    if (!computes) return;
+    const int rows = RT * 16, cols = CTT * 16;
 #pragma unroll
-    for (int c = 0; c < CT; c++) {
+    for (int i = 0; i < RTW; i++) {
+        if (wr * RTW + i >= RT) continue;
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-            const int row = w * 16 + q * 4 + v;
-            const int col = c * 16 + lc;
-            partial[((size_t)blockIdx.x * rows + row) * (CT * 16) + col] = acc[c][v];
+        for (int j = 0; j < CTW; j++) {
+            const int col = (wc * CTW + j) * 16 + lc;
+            if (col >= cols) continue;
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = (wr * RTW + i) * 16 + q * 4 + v;
+                partial[((size_t)blockIdx.x * rows + row) * cols + col] = acc[i][j][v];
+            }
         }
     }
 }
 
-// out[w * rows + i] = sum_l 2^(8l) (S'_{i,(w,l)} + 128 R_i) mod p; one thread per (w, i).
-__global__ void k_aj_finish(const int *__restrict__ partial, const unsigned long long *__restrict__ rowsum, int rows,
-                            int cols, int chunks, int batch, u64 *__restrict__ out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= batch * rows) return;
-    const int w = t / rows, i = t % rows;
-    const long long R = (long long)rowsum[i];
-    u64 acc = 0;
-    for (int l = 0; l < 8; l++) {
-        const int col = w * 8 + l;
-        long long s = 0;
-        for (int c = 0; c < chunks; c++) s += partial[((size_t)c * rows + i) * cols + col];
-        s += 128 * R;  // |s| < 2^38 < p
-        const u64 r = s >= 0 ? (u64)s : GL_P - (u64)(-s);
-        acc = gl_add(acc, gl_mul(r, (u64)1 << (8 * l)));
+// out[w * rows + i] = sum_l 2^(8l) (S'_{i,(w,l)} + 128 R_i) mod p. One workgroup of 256
+// threads per (i, w): thread (slice s, limb l) sums chunks s, s + 32, ...; LDS reduce.
+__global__ __launch_bounds__(256) void k_aj_finish(const int *__restrict__ partial,
+                                                   const unsigned long long *__restrict__ rowsum, int rows, int cols,
+                                                   int chunks, u64 *__restrict__ out) {
+    __shared__ long long red[256];
+    const int i = blockIdx.x % rows, w = blockIdx.x / rows;
+    const int l = threadIdx.x & 7, sl = threadIdx.x >> 3;
+    long long s = 0;
+    for (int c = sl; c < chunks; c += 32) s += partial[((size_t)c * rows + i) * cols + w * 8 + l];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int h = 128; h >= 8; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
     }
-    out[(size_t)w * rows + i] = acc;
+    if (threadIdx.x == 0) {
+        const long long R = (long long)rowsum[i];
+        u64 acc = 0;
+        for (int k = 0; k < 8; k++) {
+            const long long v = red[k] + 128 * R;  // |v| < 2^40 < p
+            const u64 r = v >= 0 ? (u64)v : GL_P - (u64)(-v);
+            acc = gl_add(acc, gl_mul(r, (u64)1 << (8 * k)));
+        }
+        out[(size_t)w * rows + i] = acc;
+    }
 }
 
 namespace {
@@ -148,26 +249,49 @@ struct AjtaiI8 {
     size_t n = 0;
     u64 *mt = nullptr;                     // rows * n bytes, tiled
     unsigned long long *rowsum = nullptr;  // rows
-    u64 *planes = nullptr;                 // planes_batch * 8 * n bytes, grown on demand
-    int planes_batch = 0;
-    int *partial = nullptr;                // chunks * rows * 16 * ceil(batch / 2)
+    int *partial = nullptr;                // grown on demand (chunks * rows * CTT * 16 ints)
+    size_t partial_bytes = 0;
 };
+
+// Grow a per-handle device buffer; the stream is drained before an old buffer is freed.
+int aj_reserve(void **buf, size_t *have, size_t need, hipStream_t s) {
+    if (need <= *have) return 0;
+    if (*buf) {
+        (void)hipStreamSynchronize(s);
+        (void)hipFree(*buf);
+        *buf = nullptr;
+        *have = 0;
+    }
+    if (hipMalloc(buf, need) != hipSuccess) return -1;
+    *have = need;
+    return 0;
+}
+
+template <int RTW, int CTW, int CT, int DEPTH>
+void aj_launch(const AjtaiI8 *A, const u64 *d_z, int batch, size_t kchunk, unsigned grid, hipStream_t s) {
+    static_assert(DEPTH * 32 * 4 <= AJF_THREADS, "one witness group per staging thread");
+    static_assert((16 / RTW) * (CT / CTW) <= AJF_THREADS / 64, "wave blocks fit the workgroup");
+    const int RT = A->rows / 16;
+    const int WR = (RT + RTW - 1) / RTW;
+    k_aj_gemm<RTW, CTW, CT, DEPTH><<<grid, AJF_THREADS, 0, s>>>((const v4i *)A->mt, d_z, RT, batch, WR, A->n,
+                                                                     kchunk, A->partial);
+}
 }  // namespace
 
-// Expand the matrix for (key, rows, n): rows a multiple of 16 (<= 256), n a multiple of 64.
+// Expand the matrix for (key, rows, n): rows a multiple of 16 (<= 256), n a multiple of 64
+// (<= 2^30, which keeps every recombined limb sum below p).
 MAM_API int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void **handle) {
-    if (!handle || !key32 || rows <= 0 || rows % 16 || rows > 256 || n == 0 || n % 64) return -2;
+    if (!handle || !key32 || rows <= 0 || rows % 16 || rows > 256 || n == 0 || n % 64 || n > ((uint64_t)1 << 30))
+        return -2;
     *handle = nullptr;
     AjtaiI8 *A = new AjtaiI8();
     A->rows = rows;
     A->n = n;
-    const size_t chunks = (n + AJ_KCHUNK - 1) / AJ_KCHUNK;
     hipStream_t s = mam_stream();
     u32 *d_key = nullptr;
     int rc = 0;
     if (hipMalloc((void **)&A->mt, (size_t)rows * n) != hipSuccess ||
         hipMalloc((void **)&A->rowsum, rows * sizeof(u64)) != hipSuccess ||
-        hipMalloc((void **)&A->partial, chunks * rows * 16 * (AJ_MAX_BATCH / 2) * sizeof(int)) != hipSuccess ||
         hipMalloc((void **)&d_key, 32) != hipSuccess)
         rc = -1;
     u32 kw[8];
@@ -178,7 +302,7 @@ MAM_API int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void 
     if (!rc && hipMemsetAsync(A->rowsum, 0, rows * sizeof(u64), s) != hipSuccess) rc = -1;
     if (!rc) {
         const size_t threads = (size_t)rows * (n / 64);
-        k_aj_expand<<<grid_1d(threads), MAM_BLOCK, 0, s>>>(d_key, rows, n, A->mt, A->rowsum);
+        k_aj_expand<<<grid_1d(threads), MAM_BLOCK, 0, s>>>(d_key, rows, n, (uint4 *)A->mt, A->rowsum);
         if (hipGetLastError() != hipSuccess) rc = -1;
     }
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = -1;
@@ -186,7 +310,6 @@ MAM_API int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void 
     if (rc) {
         if (A->mt) (void)hipFree(A->mt);
         if (A->rowsum) (void)hipFree(A->rowsum);
-        if (A->partial) (void)hipFree(A->partial);
         delete A;
         return rc;
     }
@@ -200,36 +323,27 @@ MAM_API int mam_ajtai_i8_commit(void *handle, const u64 *d_z, int batch, u64 *d_
     if (!A || !d_z || !d_out || batch < 1 || batch > AJ_MAX_BATCH) return -2;
     hipStream_t s = mam_stream();
     const size_t n = A->n;
-    const int RT = A->rows / 16;
-    const size_t chunks = (n + AJ_KCHUNK - 1) / AJ_KCHUNK;
     const int nplanes = 8 * batch;
-    int ct = (nplanes + 15) / 16;
-    int CT = ct <= 1 ? 1 : ct <= 2 ? 2 : ct <= 4 ? 4 : ct <= 8 ? 8 : 16;
-    if (batch > A->planes_batch) {
-        if (A->planes) {
-            (void)hipStreamSynchronize(s);
-            (void)hipFree(A->planes);
-            A->planes = nullptr;
-            A->planes_batch = 0;
+    const int CTT = (nplanes + 15) / 16;
+    const int CT = CTT <= 1 ? 1 : CTT <= 2 ? 2 : CTT <= 4 ? 4 : CTT <= 8 ? 8 : 16;
+    // 2^13 columns per workgroup (512 workgroups at n = 2^22) while the partials stay small,
+    // 2^14 for the widest batches; both keep chunk sums below 2^28.
+    const size_t kchunk = CT <= 4 ? ((size_t)1 << 13) : ((size_t)1 << 14);
+    const size_t chunks = (n + kchunk - 1) / kchunk;
+    if (aj_reserve((void **)&A->partial, &A->partial_bytes, chunks * A->rows * CTT * 16 * sizeof(int), s))
+        return -1;
+    {
+        ProfScope prof(PROF_AJTAI_GEMM, (double)A->rows * n + (double)nplanes * n, s);
+        switch (CT) {
+            case 1: aj_launch<2, 1, 1, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
+            case 2: aj_launch<2, 2, 2, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
+            case 4: aj_launch<4, 2, 4, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
+            case 8: aj_launch<4, 4, 8, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
+            default: aj_launch<4, 8, 16, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
         }
-        if (hipMalloc((void **)&A->planes, (size_t)batch * 8 * n) != hipSuccess) return -1;
-        A->planes_batch = batch;
-    }
-    k_aj_limbs<<<grid_1d((size_t)batch * (n / 8)), MAM_BLOCK, 0, s>>>(d_z, batch, n, A->planes);
-    if (hipGetLastError() != hipSuccess) return -1;
-    const unsigned grid = (unsigned)chunks;
-    const int threads = 64 * RT;
-    ProfScope prof(PROF_AJTAI_GEMM, (double)A->rows * n + (double)nplanes * n, s);
-    switch (CT) {
-        case 1: k_aj_gemm<1><<<grid, threads, 0, s>>>(A->mt, A->planes, RT, n, nplanes, A->partial); break;
-        case 2: k_aj_gemm<2><<<grid, threads, 0, s>>>(A->mt, A->planes, RT, n, nplanes, A->partial); break;
-        case 4: k_aj_gemm<4><<<grid, threads, 0, s>>>(A->mt, A->planes, RT, n, nplanes, A->partial); break;
-        case 8: k_aj_gemm<8><<<grid, threads, 0, s>>>(A->mt, A->planes, RT, n, nplanes, A->partial); break;
-        default: k_aj_gemm<16><<<grid, threads, 0, s>>>(A->mt, A->planes, RT, n, nplanes, A->partial); break;
     }
     if (hipGetLastError() != hipSuccess) return -1;
-    k_aj_finish<<<grid_1d((size_t)batch * A->rows), MAM_BLOCK, 0, s>>>(A->partial, A->rowsum, A->rows, CT * 16,
-                                                                        (int)chunks, batch, d_out);
+    k_aj_finish<<<batch * A->rows, 256, 0, s>>>(A->partial, A->rowsum, A->rows, CTT * 16, (int)chunks, d_out);
     return mam_sync_status();
 }
 
@@ -258,7 +372,6 @@ MAM_API int mam_ajtai_i8_free(void *handle) {
     (void)hipStreamSynchronize(mam_stream());
     (void)hipFree(A->mt);
     (void)hipFree(A->rowsum);
-    (void)hipFree(A->planes);
     (void)hipFree(A->partial);
     delete A;
     return 0;

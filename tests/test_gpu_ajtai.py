@@ -16,7 +16,8 @@ def gpu_init():
 
 
 @pytest.mark.parametrize("rows,n,batch", [(256, 4096, 1), (256, 4096, 3), (16, 64, 1), (32, 1 << 15, 2),
-                                          (256, (1 << 14) + 320, 8), (64, 8192, 32)])
+                                          (256, (1 << 14) + 320, 8), (64, 8192, 32), (48, 4096 + 64, 5),
+                                          (256, 8192, 12), (80, (1 << 13) + 192, 17)])
 def test_ajtai_i8_matches_oracle(rows, n, batch):
     print("ajtai", rows, n, batch, flush=True)
     key = key_from_seed(7 + rows)
