@@ -311,10 +311,13 @@ struct Arena {
 struct ThreadArenas {
     Arena dev[4];
     Arena host;
+    u64 *mailbox = nullptr;  // fine-grained pinned host words, MAM_MAILBOX_WORDS + 1 (flag last)
+    u64 seq = 0;
     ~ThreadArenas() {
         for (auto &a : dev)
             if (a.p) (void)hipFree(a.p);
         if (host.p) (void)hipHostFree(host.p);
+        if (mailbox) (void)hipHostFree(mailbox);
     }
 };
 thread_local ThreadArenas t_arenas;
@@ -362,5 +365,55 @@ MAM_API int mam_release_scratch() {
     }
     if (t_arenas.host.p) (void)hipHostFree(t_arenas.host.p);
     t_arenas.host = Arena();
+    if (t_arenas.mailbox) (void)hipHostFree(t_arenas.mailbox);
+    t_arenas.mailbox = nullptr;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Host mailbox (common.h): a one-thread-per-word kernel copies device words into
+// fine-grained pinned host memory and then sets a sequence flag with a system-scope
+// release store; the host spins on the flag instead of a copy + stream synchronization.
+// ---------------------------------------------------------------------------
+__global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq) {
+    const int t = threadIdx.x;
+    if (t < na)
+        box[t] = a[t];
+    else if (t < na + nb)
+        box[t] = b[t - na];
+    __threadfence_system();
+    __syncthreads();
+    if (t == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s) {
+    if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS) return -2;
+    if (!t_arenas.mailbox) {
+        if (hipHostMalloc((void **)&t_arenas.mailbox, (MAM_MAILBOX_WORDS + 1) * sizeof(u64),
+                          hipHostMallocCoherent) != hipSuccess) {
+            t_arenas.mailbox = nullptr;
+            return -1;
+        }
+        t_arenas.mailbox[MAM_MAILBOX_WORDS] = 0;
+        t_arenas.seq = 0;
+    }
+    u64 *box = t_arenas.mailbox;
+    const u64 seq = ++t_arenas.seq;
+    k_publish<<<1, 64, 0, s>>>(a, na, b, nb, box, seq);
+    if (hipGetLastError() != hipSuccess) return -1;
+    // Spin on the flag; every few thousand polls ask the stream whether it failed or drained.
+    for (unsigned it = 1;; it++) {
+        if (__atomic_load_n(&box[MAM_MAILBOX_WORDS], __ATOMIC_ACQUIRE) == seq) break;
+        if ((it & 4095) == 0) {
+            const hipError_t e = hipStreamQuery(s);
+            if (e == hipSuccess) {
+                if (__atomic_load_n(&box[MAM_MAILBOX_WORDS], __ATOMIC_ACQUIRE) == seq) break;
+                return -1;
+            }
+            if (e != hipErrorNotReady) return -1;
+        }
+        __builtin_ia32_pause();
+    }
+    for (int i = 0; i < na + nb; i++) out[i] = box[i];
     return 0;
 }

@@ -42,6 +42,14 @@ static inline unsigned grid_stride(size_t n, int block = MAM_BLOCK, unsigned cap
     return (unsigned)g;
 }
 
+// Grid for the ticketed reduction kernels: a single workgroup (which skips the ticket and
+// its device-scope fences) while the work is at most 16 items per thread, else one item
+// per thread up to MAM_RED_GRID workgroups.
+static inline unsigned grid_reduce(size_t n, int block = MAM_BLOCK) {
+    if (n <= (size_t)block * 16) return 1;
+    return grid_stride(n, block, MAM_RED_GRID);
+}
+
 static inline int mam_launch_status() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 static inline int mam_sync_status() {
@@ -60,3 +68,10 @@ static inline int mam_sync_status() {
 // owned by the library until the thread's arenas are released with mam_release_scratch.
 u64 *mam_dev_arena(size_t words, int slot);
 u64 *mam_host_arena(size_t words);
+
+// Copies na device words from a, then nb from b, to the host array `out` through the
+// calling thread's pinned mailbox, ordered after all work already queued on stream s.
+// Costs one tiny kernel and a host spin on a flag instead of a copy and a stream
+// synchronization. Returns 0, -1 (stream error) or -2 (na + nb > MAM_MAILBOX_WORDS).
+static constexpr int MAM_MAILBOX_WORDS = 63;
+int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s);

@@ -127,6 +127,7 @@ u64 *pinned_scratch() {
 }
 
 int read_back(u64 *host, const u64 *dev, size_t words, hipStream_t s) {
+    if (words <= (size_t)MAM_MAILBOX_WORDS) return mam_publish_wait(dev, (int)words, nullptr, 0, host, s);
     if (hipMemcpyAsync(host, dev, words * sizeof(u64), hipMemcpyDeviceToHost, s) != hipSuccess) return -1;
     return hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
 }
@@ -266,17 +267,15 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
         rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, C.hash, s);
         if (rc) break;
         const size_t leaves = flen / 2;
-        if (hipMemcpyAsync(h, rs.result, 6 * sizeof(u64), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipMemcpyAsync(h + 8, ftree[round] + mam_merkle_root_offset(leaves), 4 * sizeof(u64),
-                           hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess) {
+        // oracle (3 Ext2) and the new root in one mailbox round trip
+        if (mam_publish_wait(rs.result, 6, ftree[round] + mam_merkle_root_offset(leaves), 4, h, s)) {
             rc = -1;
             break;
         }
         u64 *o = o_oracles + 6L * (round + 1);
         memcpy(o, h, 6 * sizeof(u64));
         for (int k = 0; k < 3; k++) observe_ext2(t, o + 2 * k);
-        memcpy(o_roots + 4L * round, h + 8, 4 * sizeof(u64));
+        memcpy(o_roots + 4L * round, h + 6, 4 * sizeof(u64));
         observe_hash(t, o_roots + 4L * round);
         cw_in = fcw[round];
         cw_base = false;

@@ -138,9 +138,7 @@ int state_round_message(ScState &S) {
         rc = mam_sc_launch_msg(S.np, in, stride, S.cur_size / 2, base, S.s, S.st);
         if (rc) return rc;
     }
-    size_t words = 2 * (S.np + 1);
-    if (hipMemcpyAsync(S.host, S.s.msg, words * sizeof(u64), hipMemcpyDeviceToHost, S.st) != hipSuccess) return -1;
-    return hipStreamSynchronize(S.st) == hipSuccess ? 0 : -1;
+    return mam_publish_wait(S.s.msg, 2 * (S.np + 1), nullptr, 0, S.host, S.st);
 }
 
 int state_fold(ScState &S, ext2 ch) {
@@ -163,9 +161,7 @@ int state_final(ScState &S) {
         S.cur_size = 1;
         S.finished = true;
     }
-    if (hipMemcpyAsync(S.host, S.finals, 2 * S.np * sizeof(u64), hipMemcpyDeviceToHost, S.st) != hipSuccess)
-        return -1;
-    return hipStreamSynchronize(S.st) == hipSuccess ? 0 : -1;
+    return mam_publish_wait(S.finals, 2 * S.np, nullptr, 0, S.host, S.st);
 }
 
 const uint8_t L_NUM_VAR[] = "num_var";

@@ -4,7 +4,7 @@
 // the same transcript for any representation the caller passes.
 #include <string.h>
 
-#include "poseidon2.h"
+#include "poseidon2_host.h"
 #include "transcript.h"
 
 #define MAM_API extern "C" __attribute__((visibility("default")))
@@ -15,7 +15,7 @@ static void duplex(mam_transcript *t) {
     for (int64_t i = 0; i < t->in_count; i++) t->state[i] = t->in_buf[i];
     for (int64_t i = t->in_count; i < 4; i++) t->state[i] = 0;
     t->in_count = 0;
-    p2::permute(t->state);
+    p2h::permute(t->state);
     for (int i = 0; i < 4; i++) t->out_buf[3 - i] = t->state[i];
     t->out_count = 4;
 }
@@ -91,4 +91,4 @@ MAM_API void mam_transcript_challenge_ext2(mam_transcript *t, const uint8_t *lab
     mam::tr_challenge_ext2(t, label, len, out2);
 }
 // Host Poseidon2-w8 permutation (for checking the device one).
-MAM_API void mam_poseidon2_permute_host(uint64_t *state8) { p2::permute(state8); }
+MAM_API void mam_poseidon2_permute_host(uint64_t *state8) { p2h::permute(state8); }

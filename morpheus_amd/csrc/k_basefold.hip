@@ -733,14 +733,14 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(const u64 *__res
 }
 
 int mam_open_oracle0(const u64 *eq, const u64 *bh, size_t pairs, const RedScratch &rs, hipStream_t s) {
-    unsigned g = grid_stride(pairs, MAM_BLOCK, MAM_RED_GRID);
+    unsigned g = grid_reduce(pairs);
     k_open_oracle0<<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.counter, rs.result);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,
                          size_t half_out, const RedScratch &rs, hipStream_t s) {
-    unsigned g = grid_stride(half_out, MAM_BLOCK, MAM_RED_GRID);
+    unsigned g = grid_reduce(half_out);
     ProfScope prof(PROF_OPEN_SUMCHECK, (double)half_out * (64.0 + (bh_base ? 32.0 : 64.0) + 128.0), s);
     if (bh_base)
         k_open_fold_oracle<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, ch, eq_out, bh_out, half_out, rs.partials, rs.counter,

@@ -228,7 +228,7 @@ __global__ void k_sc_fold_final(const u64 *__restrict__ in, size_t in_stride, in
 
 template <int NP>
 static int launch_msg(const u64 *in, size_t stride, size_t half, bool base, const ScScratch &s, hipStream_t st) {
-    unsigned g = grid_stride(half, MAM_BLOCK, MAM_RED_GRID);
+    unsigned g = grid_reduce(half);
     if (base)
         k_sc_msg<NP, true><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.counter, s.msg);
     else
@@ -239,7 +239,7 @@ static int launch_msg(const u64 *in, size_t stride, size_t half, bool base, cons
 template <int NP>
 static int launch_fold_msg(const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t half_out, bool base_in,
                            ext2 ch, const ScScratch &s, hipStream_t st) {
-    unsigned g = grid_stride(half_out, MAM_BLOCK, MAM_RED_GRID);
+    unsigned g = grid_reduce(half_out);
     // per output pair and polynomial: read 4 inputs (base or Ext2), write 2 Ext2
     ProfScope prof(PROF_SUMCHECK, (double)half_out * NP * ((base_in ? 32.0 : 64.0) + 32.0), st);
     if (base_in)

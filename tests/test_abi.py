@@ -87,6 +87,12 @@ def test_host_permutations_match_oracle():
         b = s8.copy()
         M.lib.mam_poseidon2_permute_host(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
         assert np.array_equal(b, O.poseidon2_permute(s8, 0))
+    P = (1 << 64) - (1 << 32) + 1
+    for edge in ([0] * 8, [P - 1] * 8, [P] * 8, [(1 << 64) - 1] * 8, [P - 1, 0, P, 1, (1 << 64) - 1, 2, P + 5, 7]):
+        s8 = np.array(edge, dtype=np.uint64)
+        b = s8.copy()
+        M.lib.mam_poseidon2_permute_host(b.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)))
+        assert np.array_equal(b, O.poseidon2_permute(s8, 0))
 
 
 @pytest.mark.parametrize("nv,lr,nr,seed", [(4, 1, 4, 42), (6, 3, 6, 7), (10, 1, 10, 42)])
