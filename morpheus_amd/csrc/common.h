@@ -42,11 +42,11 @@ static inline unsigned grid_stride(size_t n, int block = MAM_BLOCK, unsigned cap
     return (unsigned)g;
 }
 
-// Grid for the ticketed reduction kernels: a single workgroup (which skips the ticket and
-// its device-scope fences) while the work is at most 16 items per thread, else one item
-// per thread up to MAM_RED_GRID workgroups.
+// Grid for the ticketed reduction kernels: one item per thread up to MAM_RED_GRID
+// workgroups; work that fits one workgroup runs as a single workgroup, which skips the
+// ticket and its device-scope fences. (A single workgroup looping over more items is
+// latency-bound: 16 items per thread cost more than the ticket.)
 static inline unsigned grid_reduce(size_t n, int block = MAM_BLOCK) {
-    if (n <= (size_t)block * 16) return 1;
     return grid_stride(n, block, MAM_RED_GRID);
 }
 
