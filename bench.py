@@ -69,11 +69,11 @@ def gen_inputs(nv, seed=1):
 
 
 def profile_collect(lib):
-    ms = (ctypes.c_double * 6)()
-    cnt = (ctypes.c_long * 6)()
-    work = (ctypes.c_double * 6)()
+    ms = (ctypes.c_double * 7)()
+    cnt = (ctypes.c_long * 7)()
+    work = (ctypes.c_double * 7)()
     assert lib.mam_profile_collect(ms, cnt, work) == 0
-    return [(ms[i], cnt[i], work[i]) for i in range(6)]
+    return [(ms[i], cnt[i], work[i]) for i in range(7)]
 
 
 def cpu_baseline_basefold(nv, lr, nq):

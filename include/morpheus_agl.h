@@ -1,0 +1,100 @@
+/*
+ * morpheus_agl.h -- the almost-Goldilocks (q = 2^64 - 2^32 - 31) hot-path subset of
+ * libmorpheus_amd.so, under the names and signatures of the reference's
+ * almost-goldilocks-cuda-rs/src/ffi.rs (static library `almost_goldilocks_cuda_wrapper`),
+ * so that crate's Rust wrappers link against this library unchanged.
+ *
+ * Conventions as in the reference: device pointers to u64 field elements (canonical on
+ * output), Ext2 = F_q[X]/(X^2 - 3) interleaved [c0, c1]; return 0 on success, -1 on a
+ * runtime error or rejected argument, -2 unsupported batch size B, -3 unsupported chunk
+ * size (wrapper.cu:1132-1205); calls complete before returning unless named `_stream`.
+ * The memory/device entry points (cuda_malloc, cuda_memcpy_*, ...) are shared with the
+ * Goldilocks crate and declared in morpheus_gl.h.
+ */
+#ifndef MORPHEUS_AGL_H
+#define MORPHEUS_AGL_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int almost_goldilocks_cuda_init(void); /* ffi.rs:13 */
+
+/* base field batch ops, ffi.rs:103-112 */
+int agl_batch_add_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+int agl_batch_sub_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+int agl_batch_mul_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+int agl_batch_neg_ffi(const uint64_t *a, uint64_t *r, int n);
+int agl_batch_square_ffi(const uint64_t *a, uint64_t *r, int n);
+int agl_batch_double_ffi(const uint64_t *a, uint64_t *r, int n);
+int agl_batch_inverse_ffi(const uint64_t *a, uint64_t *r, int n);
+int agl_batch_exp_ffi(const uint64_t *a, uint64_t exp, uint64_t *r, int n);
+int agl_batch_mul_scalar_ffi(uint64_t scalar, const uint64_t *a, uint64_t *r, int n);
+int agl_batch_div_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+
+/* Ext2 batch ops, ffi.rs:122-138 */
+int aext2_batch_add_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+int aext2_batch_sub_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+int aext2_batch_mul_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+int aext2_batch_inverse_ffi(const uint64_t *a, uint64_t *r, int n);
+int aext2_batch_neg_ffi(const uint64_t *a, uint64_t *r, int n);
+int aext2_batch_square_ffi(const uint64_t *a, uint64_t *r, int n);
+int aext2_batch_frobenius_ffi(const uint64_t *a, uint64_t *r, int n);
+int aext2_batch_conjugate_ffi(const uint64_t *a, uint64_t *r, int n);
+int aext2_batch_exp_ffi(const uint64_t *a, uint64_t exp, uint64_t *r, int n);
+int aext2_batch_mul_scalar_ffi(uint64_t scalar, const uint64_t *a, uint64_t *r, int n);
+int aext2_scale_accumulate_ffi(uint64_t scalar_c0, uint64_t scalar_c1, const uint64_t *src, uint64_t *acc, int n);
+int agl_to_aext2_batch_ffi(const uint64_t *input, uint64_t *output, int n);
+int aext2_to_agl_batch_ffi(const uint64_t *input, uint64_t *output, int n);
+
+/* eq tables, ffi.rs:161-196: the table lands in d_buf_a for even log_n, d_buf_b for odd
+ * (where the reference's ping-pong ends) and *d_result points at it. Batched: point l at
+ * d_r_all + l * log_n Ext2, table l at buffer + l * leaf_stride Ext2 elements. */
+int agl_eq_dp_all_ffi(const uint64_t *d_r, uint64_t *d_buf_a, uint64_t *d_buf_b, int log_n, uint64_t **d_result);
+int aext2_eq_dp_all_ffi(const uint64_t *d_r, uint64_t *d_buf_a, uint64_t *d_buf_b, int log_n, uint64_t **d_result);
+int aext2_eq_dp_all_stream_ffi(const uint64_t *d_r, uint64_t *d_buf_a, uint64_t *d_buf_b, int log_n,
+                               uint64_t **d_result, void *stream);
+int aext2_eq_dp_all_batched_ffi(const uint64_t *d_r_all, uint64_t *d_buf_a_all, uint64_t *d_buf_b_all, int log_n,
+                                int num_leaves, size_t leaf_stride, uint64_t **d_result, void *stream);
+
+/* partial evaluation, ffi.rs:198-212 (first m variables, LSB first) */
+int agl_partial_eval_ffi(uint64_t *d_data, const uint64_t *d_r, int log_n, int m);
+int agl_partial_eval_ext2_from_base_ffi(const uint64_t *d_input, uint64_t *d_output, const uint64_t *d_r, int log_n,
+                                        int m);
+
+/* sumcheck with per-workgroup partials, ffi.rs:228-263 */
+int agl_sumcheck_round_message_ffi(const uint64_t *d_polys, uint64_t *d_partial, int d, size_t original_size,
+                                   size_t half, int num_blocks);
+int agl_sumcheck_fold_ffi(const uint64_t *d_input, uint64_t *d_output, uint64_t challenge, int d,
+                          size_t original_size, size_t half);
+int aext2_sumcheck_round_message_ffi(const uint64_t *d_polys, uint64_t *d_partial, int d, size_t original_size,
+                                     size_t half, int num_blocks);
+int aext2_sumcheck_fold_ffi(const uint64_t *d_input, uint64_t *d_output, uint64_t challenge_c0,
+                            uint64_t challenge_c1, int d, size_t original_size, size_t half);
+/* batched same-size degree-2 sumcheck, ffi.rs:265-272 and :320-328 (GpuBatchedSamePointState,
+ * sumcheck_prover.rs:468-1027): partial[(block_x * num_leaves + leaf) * 3 + c] Ext2 */
+int aext2_sumcheck_batched_round_message_ffi(const uint64_t *d_polys, uint64_t *d_partial, size_t original_size,
+                                             size_t half, int num_leaves, int num_blocks_x);
+int aext2_sumcheck_batched_fold_ffi(const uint64_t *d_input, uint64_t *d_output, uint64_t challenge_c0,
+                                    uint64_t challenge_c1, size_t original_size, size_t half, int num_leaves);
+
+/* ring Ajtai over F_q[X]/(X^64 + 1), KAPPA = 42 (ajtai.rs:138-386); d_key: 8 u32 on the
+ * device; outputs [B][42][64] canonical coefficients. */
+int ajtai_commit_dense_batched_ffi(const uint32_t *d_key, const uint64_t *d_z, uint64_t N, int B, int chunk,
+                                   uint64_t *d_out);                                   /* ffi.rs:426-433 */
+int ajtai_commit_dense_batched_at_ffi(const uint32_t *d_key, const uint64_t *d_z, uint64_t N, int B, int chunk,
+                                      uint64_t col_offset, uint64_t *d_out);           /* ffi.rs:500-508 */
+int ajtai_commit_wide_ffi(const uint32_t *d_key, const uint64_t *d_z_wide, uint64_t N, uint64_t col_offset,
+                          int chunk, uint64_t *d_out);                                 /* ffi.rs:595-602 */
+int ajtai_commit_sparse_ffi(const uint32_t *d_key, const uint64_t *d_positions, uint64_t K, int chunk,
+                            uint64_t *d_out);                                          /* ffi.rs:649-655 */
+/* No reference counterpart: host copy of M_{row, col} (64 coefficients) for checks. */
+int mam_ajtai_ring_matrix_entry(const uint8_t *key32, uint32_t row, uint64_t col, uint64_t *out64);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MORPHEUS_AGL_H */

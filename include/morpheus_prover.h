@@ -118,7 +118,8 @@ int mam_merkle_tree_async(const uint64_t *d_codeword, uint64_t *d_tree, size_t n
  * counterpart; used by bench.py). Categories: 0 Merkle builds (work = permutations),
  * 1 opening fold+oracle (bytes), 2 commit transform (bytes), 3 sumcheck fold+message
  * (bytes), 4 single leaf-hashing launches over base-field codewords (permutations),
- * 5 int8 Ajtai GEMM launches (bytes). collect() fills 6 entries of each array. */
+ * 5 int8 Ajtai GEMM launches (bytes), 6 dense ring-Ajtai launches (ring columns x witnesses).
+ * collect() fills 7 entries of each array. */
 int mam_profile_enable(int on);
 int mam_profile_collect(double *ms, long *count, double *work);
 

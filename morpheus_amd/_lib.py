@@ -140,6 +140,37 @@ _SIGS = {
     "mam_ajtai_i8_commit": (ci, [vp, vp, ci, vp]),
     "mam_ajtai_i8_matrix_row": (ci, [vp, ci, u64, u64, ctypes.POINTER(ctypes.c_int8)]),
     "mam_ajtai_i8_free": (ci, [vp]),
+    # almost-Goldilocks crate (morpheus_agl.h)
+    "almost_goldilocks_cuda_init": (ci, []),
+    **{f"agl_batch_{op}_ffi": (ci, [vp, vp, vp, ci]) for op in ("add", "sub", "mul", "div")},
+    **{f"agl_batch_{op}_ffi": (ci, [vp, vp, ci]) for op in ("neg", "square", "double", "inverse")},
+    "agl_batch_exp_ffi": (ci, [vp, u64, vp, ci]),
+    "agl_batch_mul_scalar_ffi": (ci, [u64, vp, vp, ci]),
+    **{f"aext2_batch_{op}_ffi": (ci, [vp, vp, vp, ci]) for op in ("add", "sub", "mul")},
+    **{f"aext2_batch_{op}_ffi": (ci, [vp, vp, ci])
+       for op in ("inverse", "neg", "square", "frobenius", "conjugate")},
+    "aext2_batch_exp_ffi": (ci, [vp, u64, vp, ci]),
+    "aext2_batch_mul_scalar_ffi": (ci, [u64, vp, vp, ci]),
+    "aext2_scale_accumulate_ffi": (ci, [u64, u64, vp, vp, ci]),
+    "agl_to_aext2_batch_ffi": (ci, [vp, vp, ci]),
+    "aext2_to_agl_batch_ffi": (ci, [vp, vp, ci]),
+    "agl_eq_dp_all_ffi": (ci, [vp, vp, vp, ci, ctypes.POINTER(vp)]),
+    "aext2_eq_dp_all_ffi": (ci, [vp, vp, vp, ci, ctypes.POINTER(vp)]),
+    "aext2_eq_dp_all_stream_ffi": (ci, [vp, vp, vp, ci, ctypes.POINTER(vp), vp]),
+    "aext2_eq_dp_all_batched_ffi": (ci, [vp, vp, vp, ci, ci, csz, ctypes.POINTER(vp), vp]),
+    "agl_partial_eval_ffi": (ci, [vp, vp, ci, ci]),
+    "agl_partial_eval_ext2_from_base_ffi": (ci, [vp, vp, vp, ci, ci]),
+    "agl_sumcheck_round_message_ffi": (ci, [vp, vp, ci, csz, csz, ci]),
+    "agl_sumcheck_fold_ffi": (ci, [vp, vp, u64, ci, csz, csz]),
+    "aext2_sumcheck_round_message_ffi": (ci, [vp, vp, ci, csz, csz, ci]),
+    "aext2_sumcheck_fold_ffi": (ci, [vp, vp, u64, u64, ci, csz, csz]),
+    "aext2_sumcheck_batched_round_message_ffi": (ci, [vp, vp, csz, csz, ci, ci]),
+    "aext2_sumcheck_batched_fold_ffi": (ci, [vp, vp, u64, u64, csz, csz, ci]),
+    "ajtai_commit_dense_batched_ffi": (ci, [vp, vp, u64, ci, ci, vp]),
+    "ajtai_commit_dense_batched_at_ffi": (ci, [vp, vp, u64, ci, ci, u64, vp]),
+    "ajtai_commit_wide_ffi": (ci, [vp, vp, u64, u64, ci, vp]),
+    "ajtai_commit_sparse_ffi": (ci, [vp, vp, u64, ci, vp]),
+    "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
     "mam_profile_enable": (ci, [ci]),
     "mam_release_scratch": (ci, []),
     "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),

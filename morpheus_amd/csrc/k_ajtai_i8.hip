@@ -37,7 +37,6 @@
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-static constexpr int AJ_TILE_BYTES = 1024;  // 16 rows (or planes) x 64 columns
 static constexpr int AJ_MAX_BATCH = 32;
 
 // byte offset of entry (i, j) of a tiled operand with `tiles` 16-row tiles per k-step

@@ -1,0 +1,345 @@
+// Ajtai commitment over R = F_q[X]/(X^64 + 1), q = 2^64 - 2^32 - 31, KAPPA = 42 rows:
+//   c_i = sum_j M_{i, j + col_offset} * z_j  in R,
+// with M_{i,j} generated on the fly from a 32-byte key by ChaCha8 and rejection sampling.
+//
+// Reference semantics: almost-goldilocks-cuda-rs/src/ajtai.rs:138-386 (commit_batched,
+// commit_batched_at, commit, commit_wide(_device)), the FFI wrappers of
+// almost-goldilocks-cuda-rs/cuda/wrapper.cu:1139-1205 and :1741-1760 (B in {1,2,4,8,16}
+// else -2, chunk in {64,128,256,1024,4096} else -3), the matrix PRG
+// cuda_almost_goldilocks/ajtai_chacha8.cuh:115-178 (coefficient 8 b + k of M_{i,j} is the
+// k-th accepted u64 of ChaCha8(key, counter = b (+ 8 per retry), nonce = (i, j_lo, j_hi))),
+// and the CPU reference ajtai_cpu_reference.cuh:58-242 (negacyclic products, sparse form).
+//
+// MI355X layout: a workgroup is one 64-lane wave; lane r owns output coefficient r of the
+// 7 rows of its row group (6 groups cover the 42 rows). Per matrix column j the wave runs
+// 56 ChaCha8 blocks in parallel (lane = row x block) and writes each row's 64 coefficients
+// twice into LDS as L[t] = -M[t], L[64 + t] = M[t]; then (X^l M)[r] = L[64 + r - l], so a set
+// bit l of a binary witness word adds one contiguous LDS row to every lane, with no sign
+// logic. Sums are kept as separate 64-bit sums of the low and high 32-bit halves (one
+// multiply-add each, no carries) and reduced mod q once per workgroup; a second kernel adds
+// the per-chunk partials.
+#include "chacha.h"
+#include "common.h"
+#include "fieldq.h"
+#include "profile.h"
+
+static constexpr int RING_D = 64;
+static constexpr int RING_KAPPA = 42;
+static constexpr int RING_RG = 7;  // rows per group
+static constexpr int RING_GROUPS = RING_KAPPA / RING_RG;
+
+// Coefficients 8 blk .. 8 blk + 7 of M_{row, j}: the accepted 64-bit words (< q, in order) of
+// ChaCha8 blocks with counters blk, blk + 8, blk + 16, ... (16 tries; unfilled slots 0).
+__host__ __device__ __forceinline__ void ring_prg_block(const u32 key[8], u32 row, u64 j, u32 blk, u64 out[8]) {
+    const u32 nonce[3] = {row, (u32)j, (u32)(j >> 32)};
+    u32 w[16];
+    chacha::block<8>(key, blk, nonce, w);
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        out[k] = ((u64)w[2 * k + 1] << 32) | w[2 * k];
+        ok = ok && out[k] < fq::Q;
+    }
+    if (ok) return;
+    // rare path: collect accepted words into a shift register (oldest in out[0])
+    int written = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) out[k] = 0;
+    for (u32 retry = 0; retry < 16 && written < 8; retry++) {
+        chacha::block<8>(key, blk + 8 * retry, nonce, w);
+        for (int k = 0; k < 8 && written < 8; k++) {
+            const u64 s = ((u64)w[2 * k + 1] << 32) | w[2 * k];
+            if (s < fq::Q) {
+#pragma unroll
+                for (int t = 0; t < 7; t++) out[t] = out[t + 1];
+                out[7] = s;
+                written++;
+            }
+        }
+    }
+    // fewer than 8 accepted (never observed): move them to the front, zeros after
+    for (int pad = 8 - written; pad > 0; pad--) {
+        const u64 first = out[0];
+#pragma unroll
+        for (int t = 0; t < 7; t++) out[t] = out[t + 1];
+        out[7] = first;
+    }
+}
+
+namespace {
+
+// Generate M_{i, col} for the 7 rows of group g into L[ii][0..128) (lanes 0..55 work).
+__device__ __forceinline__ void ring_fill(const u32 key[8], int g, u64 col, u64 (*L)[2 * RING_D]) {
+    const int lane = threadIdx.x;
+    if (lane < RING_RG * 8) {
+        const int ii = lane >> 3, blk = lane & 7;
+        u64 c[8];
+        ring_prg_block(key, (u32)(g * RING_RG + ii), col, (u32)blk, c);
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            L[ii][RING_D + 8 * blk + k] = c[k];
+            L[ii][8 * blk + k] = fq::neg(c[k]);
+        }
+    }
+}
+
+__device__ __forceinline__ u64 ring_finish(u64 lo_sum, u64 hi_sum) {
+    // value = lo_sum + 2^32 hi_sum  (< 2^87)
+    const u64 sh = hi_sum << 32;
+    const u64 lo = lo_sum + sh;
+    const u64 hi = (hi_sum >> 32) + (lo < sh);
+    return fq::reduce128(lo, hi);
+}
+
+__device__ __forceinline__ u64 uniform_u64(u64 v) {
+    const u32 lo = __builtin_amdgcn_readfirstlane((u32)v);
+    const u32 hi = __builtin_amdgcn_readfirstlane((u32)(v >> 32));
+    return ((u64)hi << 32) | lo;
+}
+
+// Dense binary witnesses: z[b * N + j] bit l = coefficient l of z_{b,j}.
+// grid (chunks, RING_GROUPS, witness groups of BW); partial[((chunk * B + b) * KAPPA + i) * 64 + r].
+// `one` is always 1: as a runtime value it keeps each half-word accumulation a single
+// 32x32+64 multiply-add (v_mad_u64_u32) instead of a move plus a 64-bit add.
+template <int BW>
+__global__ __launch_bounds__(64) void k_ring_dense(const u32 *__restrict__ d_key, const u64 *__restrict__ z, u64 N,
+                                                   int B, u64 col_offset, u64 cols, u32 one,
+                                                   u64 *__restrict__ partial) {
+    __shared__ u64 L[RING_RG][2 * RING_D];
+    const int r = threadIdx.x, g = blockIdx.y;
+    const int b0 = blockIdx.z * BW;
+    u32 key[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) key[k] = d_key[k];
+    u64 lo[RING_RG][BW], hi[RING_RG][BW];
+#pragma unroll
+    for (int ii = 0; ii < RING_RG; ii++)
+#pragma unroll
+        for (int w = 0; w < BW; w++) lo[ii][w] = hi[ii][w] = 0;
+    const u64 j0 = (u64)blockIdx.x * cols;
+    const u64 j1 = j0 + cols < N ? j0 + cols : N;
+    for (u64 j = j0; j < j1; j++) {
+        ring_fill(key, g, j + col_offset, L);
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < BW; w++) {
+            if (b0 + w >= B) break;
+            u64 mask = uniform_u64(z[(u64)(b0 + w) * N + j]);
+            while (mask) {
+                const int l = __builtin_ctzll(mask);
+                mask &= mask - 1;
+#pragma unroll
+                for (int ii = 0; ii < RING_RG; ii++) {
+                    const u64 v = L[ii][RING_D + r - l];
+                    lo[ii][w] = (u64)(u32)v * one + lo[ii][w];
+                    hi[ii][w] = (u64)(u32)(v >> 32) * one + hi[ii][w];
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int w = 0; w < BW; w++) {
+        if (b0 + w >= B) break;
+#pragma unroll
+        for (int ii = 0; ii < RING_RG; ii++)
+            partial[(((u64)blockIdx.x * B + b0 + w) * RING_KAPPA + g * RING_RG + ii) * RING_D + r] =
+                ring_finish(lo[ii][w], hi[ii][w]);
+    }
+}
+
+// Wide witness: z[j * 64 + l] field coefficients (canonicalized on load).
+// grid (chunks, RING_GROUPS); partial[(chunk * KAPPA + i) * 64 + r].
+__global__ __launch_bounds__(64) void k_ring_wide(const u32 *__restrict__ d_key, const u64 *__restrict__ z, u64 N,
+                                                  u64 col_offset, u64 cols, u64 *__restrict__ partial) {
+    __shared__ u64 L[RING_RG][2 * RING_D];
+    __shared__ u64 zs[RING_D];
+    const int r = threadIdx.x, g = blockIdx.y;
+    u32 key[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) key[k] = d_key[k];
+    // 128-bit accumulator per row plus an overflow word: sum of up to 64 * cols products < 2^128 * 2^19
+    u64 a0[RING_RG], a1[RING_RG], a2[RING_RG];
+#pragma unroll
+    for (int ii = 0; ii < RING_RG; ii++) a0[ii] = a1[ii] = a2[ii] = 0;
+    const u64 j0 = (u64)blockIdx.x * cols;
+    const u64 j1 = j0 + cols < N ? j0 + cols : N;
+    for (u64 j = j0; j < j1; j++) {
+        ring_fill(key, g, j + col_offset, L);
+        zs[r] = fq::canon(z[j * RING_D + r]);
+        __syncthreads();
+        for (int l = 0; l < RING_D; l++) {
+            const u64 zl = zs[l];
+#pragma unroll
+            for (int ii = 0; ii < RING_RG; ii++) {
+                u64 plo, phi;
+                mul_wide(L[ii][RING_D + r - l], zl, plo, phi);
+                a0[ii] += plo;
+                const u64 c0 = a0[ii] < plo;
+                const u64 t = phi + c0;  // phi <= 2^64 - 2, no wrap
+                a1[ii] += t;
+                a2[ii] += a1[ii] < t;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int ii = 0; ii < RING_RG; ii++) {
+        // a0 + 2^64 a1 + 2^128 a2;  2^128 = C^2 mod q
+        const u64 top = fq::mul(fq::canon(a2[ii]), fq::mul(fq::C, fq::C));
+        const u64 v = fq::add(fq::reduce128(a0[ii], a1[ii]), top);
+        partial[((u64)blockIdx.x * RING_KAPPA + g * RING_RG + ii) * RING_D + r] = v;
+    }
+}
+
+// Sparse binary witness as positions p (column p >> 6, coefficient p & 63); duplicates add.
+// grid (chunks, RING_GROUPS); partial[(chunk * KAPPA + i) * 64 + r].
+__global__ __launch_bounds__(64) void k_ring_sparse(const u32 *__restrict__ d_key, const u64 *__restrict__ pos, u64 K,
+                                                    u64 per, u32 one, u64 *__restrict__ partial) {
+    __shared__ u64 L[RING_RG][2 * RING_D];
+    const int r = threadIdx.x, g = blockIdx.y;
+    u32 key[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) key[k] = d_key[k];
+    u64 lo[RING_RG], hi[RING_RG];
+#pragma unroll
+    for (int ii = 0; ii < RING_RG; ii++) lo[ii] = hi[ii] = 0;
+    const u64 k0 = (u64)blockIdx.x * per;
+    const u64 k1 = k0 + per < K ? k0 + per : K;
+    u64 cur = ~0ull;
+    for (u64 k = k0; k < k1; k++) {
+        const u64 p = uniform_u64(pos[k]);
+        const u64 j = p >> 6;
+        const int l = (int)(p & 63);
+        if (j != cur) {  // consecutive positions in one column share its matrix entries
+            __syncthreads();
+            ring_fill(key, g, j, L);
+            __syncthreads();
+            cur = j;
+        }
+#pragma unroll
+        for (int ii = 0; ii < RING_RG; ii++) {
+            const u64 v = L[ii][RING_D + r - l];
+            lo[ii] = (u64)(u32)v * one + lo[ii];
+            hi[ii] = (u64)(u32)(v >> 32) * one + hi[ii];
+        }
+    }
+#pragma unroll
+    for (int ii = 0; ii < RING_RG; ii++)
+        partial[((u64)blockIdx.x * RING_KAPPA + g * RING_RG + ii) * RING_D + r] = ring_finish(lo[ii], hi[ii]);
+}
+
+// out[t] = sum over chunks of partial[chunk * total + t] (mod q)
+__global__ void k_ring_reduce(const u64 *__restrict__ partial, u64 chunks, u64 total, u64 *__restrict__ out) {
+    const u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x;
+    if (t >= total) return;
+    u64 lo = 0, hi = 0;
+    for (u64 c = 0; c < chunks; c++) fq::acc_add(lo, hi, partial[c * total + t]);
+    out[t] = fq::reduce128(lo, hi);
+}
+
+bool ring_chunk_ok(int chunk) {
+    return chunk == 64 || chunk == 128 || chunk == 256 || chunk == 1024 || chunk == 4096;
+}
+
+// Columns per workgroup: at most the caller's chunk, fewer while the grid would leave the
+// 256 CUs short of about 16 workgroups each.
+u64 ring_cols(u64 n, int chunk, u64 groups) {
+    const u64 target = 4096;
+    u64 cols = (n * groups + target - 1) / target;
+    if (cols < 1) cols = 1;
+    if (cols > (u64)chunk) cols = (u64)chunk;
+    return cols;
+}
+
+int ring_reduce_to(const u64 *partial, u64 chunks, u64 total, u64 *d_out, hipStream_t s) {
+    k_ring_reduce<<<grid_1d(total), MAM_BLOCK, 0, s>>>(partial, chunks, total, d_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace
+
+// ffi.rs:500-508 / wrapper.cu:1150-1170
+MAM_API int ajtai_commit_dense_batched_at_ffi(const u32 *d_key, const u64 *d_z, u64 N, int B, int chunk,
+                                              u64 col_offset, u64 *d_out) {
+    if (!ring_chunk_ok(chunk)) return -3;
+    if (B != 1 && B != 2 && B != 4 && B != 8 && B != 16) return -2;
+    if (N == 0 || !d_key || !d_z || !d_out) return -1;
+    hipStream_t s = mam_stream();
+    const int BW = B >= 4 ? 4 : B;
+    const u64 wgroups = (u64)(B + BW - 1) / BW;
+    const u64 cols = ring_cols(N, chunk, RING_GROUPS * wgroups);
+    const u64 chunks = (N + cols - 1) / cols;
+    const u64 total = (u64)B * RING_KAPPA * RING_D;
+    if (chunks > 0x7FFFFFFFull) return -1;
+    u64 *partial = nullptr;
+    if (hipMallocAsync((void **)&partial, chunks * total * sizeof(u64), s) != hipSuccess) return -1;
+    const dim3 grid((unsigned)chunks, RING_GROUPS, (unsigned)wgroups);
+    {
+        ProfScope prof(PROF_AJTAI_RING, (double)N * B, s);
+        if (BW == 1) k_ring_dense<1><<<grid, 64, 0, s>>>(d_key, d_z, N, B, col_offset, cols, 1u, partial);
+        else if (BW == 2) k_ring_dense<2><<<grid, 64, 0, s>>>(d_key, d_z, N, B, col_offset, cols, 1u, partial);
+        else k_ring_dense<4><<<grid, 64, 0, s>>>(d_key, d_z, N, B, col_offset, cols, 1u, partial);
+    }
+    int rc = hipGetLastError() == hipSuccess ? 0 : -1;
+    if (!rc) rc = ring_reduce_to(partial, chunks, total, d_out, s);
+    (void)hipFreeAsync(partial, s);
+    return rc ? rc : mam_sync_status();
+}
+
+// ffi.rs:426-433
+MAM_API int ajtai_commit_dense_batched_ffi(const u32 *d_key, const u64 *d_z, u64 N, int B, int chunk, u64 *d_out) {
+    return ajtai_commit_dense_batched_at_ffi(d_key, d_z, N, B, chunk, 0, d_out);
+}
+
+// ffi.rs:595-602 / wrapper.cu:1188-1205: z_wide is [N][64] field coefficients
+MAM_API int ajtai_commit_wide_ffi(const u32 *d_key, const u64 *d_z_wide, u64 N, u64 col_offset, int chunk,
+                                  u64 *d_out) {
+    if (!ring_chunk_ok(chunk)) return -3;
+    if (N == 0 || !d_key || !d_z_wide || !d_out) return -1;
+    hipStream_t s = mam_stream();
+    const u64 cols = ring_cols(N, chunk, RING_GROUPS);
+    const u64 chunks = (N + cols - 1) / cols;
+    const u64 total = (u64)RING_KAPPA * RING_D;
+    if (chunks > 0x7FFFFFFFull) return -1;
+    u64 *partial = nullptr;
+    if (hipMallocAsync((void **)&partial, chunks * total * sizeof(u64), s) != hipSuccess) return -1;
+    k_ring_wide<<<dim3((unsigned)chunks, RING_GROUPS), 64, 0, s>>>(d_key, d_z_wide, N, col_offset, cols, partial);
+    int rc = hipGetLastError() == hipSuccess ? 0 : -1;
+    if (!rc) rc = ring_reduce_to(partial, chunks, total, d_out, s);
+    (void)hipFreeAsync(partial, s);
+    return rc ? rc : mam_sync_status();
+}
+
+// ffi.rs:649-655 / wrapper.cu:1741-1760: K positions p = 64 j + l (one binary witness)
+MAM_API int ajtai_commit_sparse_ffi(const u32 *d_key, const u64 *d_positions, u64 K, int chunk, u64 *d_out) {
+    if (!ring_chunk_ok(chunk)) return -3;
+    if (!d_key || !d_out || (K > 0 && !d_positions)) return -1;
+    hipStream_t s = mam_stream();
+    const u64 total = (u64)RING_KAPPA * RING_D;
+    if (K == 0) {
+        k_ring_reduce<<<grid_1d(total), MAM_BLOCK, 0, s>>>(nullptr, 0, total, d_out);
+        return mam_sync_status();
+    }
+    const u64 per = ring_cols(K, chunk, RING_GROUPS);
+    const u64 chunks = (K + per - 1) / per;
+    if (chunks > 0x7FFFFFFFull) return -1;
+    u64 *partial = nullptr;
+    if (hipMallocAsync((void **)&partial, chunks * total * sizeof(u64), s) != hipSuccess) return -1;
+    k_ring_sparse<<<dim3((unsigned)chunks, RING_GROUPS), 64, 0, s>>>(d_key, d_positions, K, per, 1u, partial);
+    int rc = hipGetLastError() == hipSuccess ? 0 : -1;
+    if (!rc) rc = ring_reduce_to(partial, chunks, total, d_out, s);
+    (void)hipFreeAsync(partial, s);
+    return rc ? rc : mam_sync_status();
+}
+
+// Host copy of matrix entry M_{row, col} (64 coefficients), for checks.
+MAM_API int mam_ajtai_ring_matrix_entry(const uint8_t *key32, uint32_t row, uint64_t col, uint64_t *out64) {
+    if (!key32 || !out64) return -2;
+    u32 key[8];
+    for (int k = 0; k < 8; k++)
+        key[k] = (u32)key32[4 * k] | ((u32)key32[4 * k + 1] << 8) | ((u32)key32[4 * k + 2] << 16) |
+                 ((u32)key32[4 * k + 3] << 24);
+    for (u32 b = 0; b < 8; b++) ring_prg_block(key, row, col, b, out64 + 8 * b);
+    return 0;
+}

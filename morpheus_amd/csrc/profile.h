@@ -13,7 +13,8 @@ enum MamProfCat {
     PROF_SUMCHECK = 3,
     PROF_LEAF_LAUNCH = 4,  // one k_tree_levels<Monolith> launch hashing base-field leaves
     PROF_AJTAI_GEMM = 5,   // one k_aj_gemm launch (work = matrix + limb-plane bytes)
-    PROF_NCAT = 6
+    PROF_AJTAI_RING = 6,   // one dense ring-Ajtai launch (work = ring columns x witnesses)
+    PROF_NCAT = 7
 };
 
 bool mam_prof_enabled();

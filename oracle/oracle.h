@@ -115,6 +115,31 @@ void or_chacha_block(int rounds, const uint32_t *key8, uint32_t counter, const u
 void or_ajtai_i8_row(const uint8_t *key32, uint32_t row, uint64_t n, int8_t *out);
 void or_ajtai_i8_commit(const uint8_t *key32, int rows, uint64_t n, const uint64_t *z, int batch, uint64_t *out);
 
+/* ---- almost-Goldilocks q = 2^64 - 2^32 - 31 (oracle/q.c) ----
+ * field/Ext2: pinned by the reference's golden vectors (almost_field_test.cu:143-180,
+ * almost_extension_test.cu:107-126) and algebraic identities checked against Python ints.
+ * eq / partial eval / sumcheck: closed forms (sum of eq = 1, MLE evaluation, g(0)+g(1)). */
+uint64_t or_q_add(uint64_t a, uint64_t b);
+uint64_t or_q_sub(uint64_t a, uint64_t b);
+uint64_t or_q_mul(uint64_t a, uint64_t b);
+uint64_t or_q_inv(uint64_t a);
+void or_q2_mul(const uint64_t *a, const uint64_t *b, uint64_t *out);
+void or_q2_inv(const uint64_t *a, uint64_t *out);
+void or_q_eq_table(int ext, const uint64_t *r, int nv, uint64_t *out);
+void or_q_partial_eval(const uint64_t *in, int in_ext, int nv, const uint64_t *r, int r_ext, int m, uint64_t *out);
+void or_q_sumcheck_msg(int ext, const uint64_t *polys, int d, uint64_t original_size, uint64_t half, uint64_t *out);
+void or_q_sumcheck_fold(int ext, const uint64_t *in, uint64_t *out, const uint64_t *ch2, int d,
+                        uint64_t original_size, uint64_t half);
+/* ring Ajtai over F_q[X]/(X^64+1), KAPPA = 42 (ajtai_cpu_reference.cuh:58-242).
+ * Pinned by the ring identities (negacyclic X^64 = -1, linearity, binary = sparse = wide
+ * on the same witness) and the RFC-pinned ChaCha block; no reference vector exists. */
+void or_ring_matrix_entry(const uint32_t *key8, uint32_t row, uint64_t j, uint64_t *out64);
+void or_ring_commit_binary(const uint32_t *key8, const uint64_t *z, uint64_t N, int B, uint64_t col_offset,
+                           uint64_t *out);
+void or_ring_commit_wide(const uint32_t *key8, const uint64_t *z_wide, uint64_t N, uint64_t col_offset,
+                         uint64_t *out);
+void or_ring_commit_sparse(const uint32_t *key8, const uint64_t *pos, uint64_t K, uint64_t *out);
+
 #ifdef __cplusplus
 }
 #endif

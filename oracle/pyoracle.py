@@ -53,6 +53,16 @@ _sigs = {
                                ctypes.POINTER(ctypes.c_uint32)]),
     "or_ajtai_i8_row": (None, [ctypes.c_char_p, ctypes.c_uint32, u64, ctypes.POINTER(ctypes.c_int8)]),
     "or_ajtai_i8_commit": (None, [ctypes.c_char_p, ci, u64, u64p, ci, u64p]),
+    "or_q_add": (u64, [u64, u64]), "or_q_sub": (u64, [u64, u64]), "or_q_mul": (u64, [u64, u64]),
+    "or_q_inv": (u64, [u64]), "or_q2_mul": (None, [u64p, u64p, u64p]), "or_q2_inv": (None, [u64p, u64p]),
+    "or_q_eq_table": (None, [ci, u64p, ci, u64p]),
+    "or_q_partial_eval": (None, [u64p, ci, ci, u64p, ci, ci, u64p]),
+    "or_q_sumcheck_msg": (None, [ci, u64p, ci, u64, u64, u64p]),
+    "or_q_sumcheck_fold": (None, [ci, u64p, u64p, u64p, ci, u64, u64]),
+    "or_ring_matrix_entry": (None, [ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, u64, u64p]),
+    "or_ring_commit_binary": (None, [ctypes.POINTER(ctypes.c_uint32), u64p, u64, ci, u64, u64p]),
+    "or_ring_commit_wide": (None, [ctypes.POINTER(ctypes.c_uint32), u64p, u64, u64, u64p]),
+    "or_ring_commit_sparse": (None, [ctypes.POINTER(ctypes.c_uint32), u64p, u64, u64p]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(L, _n)
@@ -205,3 +215,73 @@ def ajtai_i8_commit(key32, rows, n, z, batch):
     out = np.zeros(batch * rows, dtype=np.uint64)
     L.or_ajtai_i8_commit(bytes(key32), rows, n, _p(arr(z)), batch, _p(out))
     return out
+
+
+# ---- almost-Goldilocks (oracle/q.c) ----
+Q = 0xFFFFFFFEFFFFFFE1
+KAPPA = 42
+
+
+def _u32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def q_eq_table(r, nv, ext=True):
+    r = arr(r)
+    out = np.zeros((2 if ext else 1) << nv, dtype=np.uint64)
+    L.or_q_eq_table(1 if ext else 0, _p(r), nv, _p(out))
+    return out
+
+
+def q_partial_eval(data, in_ext, nv, r, r_ext, m):
+    data, r = arr(data), arr(r)
+    out = np.zeros(2 << (nv - m), dtype=np.uint64)
+    L.or_q_partial_eval(_p(data), in_ext, nv, _p(r), r_ext, m, _p(out))
+    return out
+
+
+def q_sumcheck_msg(ext, polys, d, original_size, half):
+    polys = arr(polys)
+    out = np.zeros(2 * (d + 1), dtype=np.uint64)
+    L.or_q_sumcheck_msg(ext, _p(polys), d, original_size, half, _p(out))
+    return out
+
+
+def q_sumcheck_fold(ext, polys, ch2, d, original_size, half):
+    polys = arr(polys)
+    out = polys.copy()
+    ch = arr(ch2)
+    L.or_q_sumcheck_fold(ext, _p(polys), _p(out), _p(ch), d, original_size, half)
+    return out
+
+
+def ring_key_words(key32):
+    return np.frombuffer(bytes(key32), dtype="<u4").astype(np.uint32).copy()
+
+
+def ring_matrix_entry(key32, row, col):
+    kw = ring_key_words(key32)
+    out = np.zeros(64, dtype=np.uint64)
+    L.or_ring_matrix_entry(_u32p(kw), row, col, _p(out))
+    return out
+
+
+def ring_commit_binary(key32, z, N, B, col_offset=0):
+    kw, z = ring_key_words(key32), arr(z)
+    out = np.zeros(B * KAPPA * 64, dtype=np.uint64)
+    L.or_ring_commit_binary(_u32p(kw), _p(z), N, B, col_offset, _p(out))
+    return out.reshape(B, KAPPA, 64)
+
+
+def ring_commit_wide(key32, z_wide, N, col_offset=0):
+    kw, z = ring_key_words(key32), arr(z_wide)
+    out = np.zeros(KAPPA * 64, dtype=np.uint64)
+    L.or_ring_commit_wide(_u32p(kw), _p(z), N, col_offset, _p(out))
+    return out.reshape(KAPPA, 64)
+
+
+def ring_commit_sparse(key32, positions):
+    kw, pos = ring_key_words(key32), arr(positions)
+    out = np.zeros(KAPPA * 64, dtype=np.uint64)
+    L.or_ring_commit_sparse(_u32p(kw), _p(pos), len(pos), _p(out))
+    return out.reshape(KAPPA, 64)

@@ -1,0 +1,245 @@
+/*
+ * oracle/q.c -- TEST INFRASTRUCTURE ONLY (the oracle).
+ *
+ * The almost-Goldilocks field q = 2^64 - 2^32 - 31 and its extension F_q[X]/(X^2 - 3),
+ * computed with 128-bit remainders (independent of the library's Solinas reduction);
+ * q-twins of the eq table, partial evaluation and reference-style sumcheck round message
+ * and fold; and the ring Ajtai commitment over R = F_q[X]/(X^64 + 1) with KAPPA = 42.
+ *
+ * Followed files:
+ *   field / Ext2: almost-goldilocks-cuda-rs/src/field.rs:11-133, extension.rs:55-86,
+ *     cuda_almost_goldilocks/almost_extension.cuh:31-170
+ *   eq: almost_eq_lagrange.cuh:25-120 (index bit i <-> r_i)
+ *   partial eval: almost_partial_eval.cuh (fold pairs (2y, 2y+1) at r_i, LSB first)
+ *   sumcheck: almost_sumcheck_prover.cuh:19-224 (values at c = 0..d), :233-330 (batched)
+ *   ring Ajtai: ajtai_cpu_reference.cuh:58-242 (negacyclic shift, naive product, PRG,
+ *     full / sparse commit), ajtai_chacha8.cuh:115-178 (rejection sampling order),
+ *     almost-goldilocks-cuda-rs/src/ajtai.rs:138-386 (batched, col_offset, wide layout)
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oracle.h"
+
+#define QP 0xFFFFFFFEFFFFFFE1ULL
+typedef unsigned __int128 u128;
+
+static uint64_t cq(uint64_t a) { return a % QP; }
+uint64_t or_q_add(uint64_t a, uint64_t b) { return (uint64_t)(((u128)cq(a) + cq(b)) % QP); }
+uint64_t or_q_sub(uint64_t a, uint64_t b) { return (uint64_t)(((u128)cq(a) + QP - cq(b)) % QP); }
+uint64_t or_q_mul(uint64_t a, uint64_t b) { return (uint64_t)(((u128)cq(a) * cq(b)) % QP); }
+static uint64_t q_pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    a = cq(a);
+    while (e) {
+        if (e & 1) r = or_q_mul(r, a);
+        a = or_q_mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+uint64_t or_q_inv(uint64_t a) { return q_pow(a, QP - 2); }
+
+void or_q2_mul(const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    /* (a0 + a1 X)(b0 + b1 X), X^2 = 3: schoolbook */
+    uint64_t c0 = or_q_add(or_q_mul(a[0], b[0]), or_q_mul(3, or_q_mul(a[1], b[1])));
+    uint64_t c1 = or_q_add(or_q_mul(a[0], b[1]), or_q_mul(a[1], b[0]));
+    out[0] = c0;
+    out[1] = c1;
+}
+void or_q2_inv(const uint64_t *a, uint64_t *out) {
+    uint64_t n = or_q_sub(or_q_mul(a[0], a[0]), or_q_mul(3, or_q_mul(a[1], a[1])));
+    uint64_t ni = or_q_inv(n);
+    out[0] = or_q_mul(a[0], ni);
+    out[1] = or_q_mul(or_q_sub(0, a[1]), ni);
+}
+
+static void e_add(uint64_t *r, const uint64_t *a, const uint64_t *b) {
+    r[0] = or_q_add(a[0], b[0]);
+    r[1] = or_q_add(a[1], b[1]);
+}
+static void e_sub(uint64_t *r, const uint64_t *a, const uint64_t *b) {
+    r[0] = or_q_sub(a[0], b[0]);
+    r[1] = or_q_sub(a[1], b[1]);
+}
+
+/* eq table, Ext2 (ext = 1, r and out interleaved) or base (ext = 0) */
+void or_q_eq_table(int ext, const uint64_t *r, int nv, uint64_t *out) {
+    uint64_t n = 1ULL << nv;
+    for (uint64_t x = 0; x < n; x++) {
+        uint64_t acc[2] = {1, 0};
+        for (int i = 0; i < nv; i++) {
+            uint64_t ri[2] = {ext ? cq(r[2 * i]) : cq(r[i]), ext ? cq(r[2 * i + 1]) : 0};
+            uint64_t f[2];
+            if ((x >> i) & 1) {
+                f[0] = ri[0];
+                f[1] = ri[1];
+            } else {
+                f[0] = or_q_sub(1, ri[0]);
+                f[1] = or_q_sub(0, ri[1]);
+            }
+            or_q2_mul(acc, f, acc);
+        }
+        if (ext) {
+            out[2 * x] = acc[0];
+            out[2 * x + 1] = acc[1];
+        } else {
+            out[x] = acc[0];
+        }
+    }
+}
+
+/* fold the first m variables of an nv-variable table: in_ext selects base/Ext2 input,
+ * r_ext base/Ext2 points; output Ext2 (interleaved) of 2^(nv-m) elements */
+void or_q_partial_eval(const uint64_t *in, int in_ext, int nv, const uint64_t *r, int r_ext, int m, uint64_t *out) {
+    uint64_t n = 1ULL << nv;
+    uint64_t *cur = (uint64_t *)malloc(2 * n * sizeof(uint64_t));
+    for (uint64_t x = 0; x < n; x++) {
+        cur[2 * x] = in_ext ? cq(in[2 * x]) : cq(in[x]);
+        cur[2 * x + 1] = in_ext ? cq(in[2 * x + 1]) : 0;
+    }
+    for (int i = 0; i < m; i++) {
+        uint64_t ri[2] = {r_ext ? cq(r[2 * i]) : cq(r[i]), r_ext ? cq(r[2 * i + 1]) : 0};
+        n /= 2;
+        for (uint64_t y = 0; y < n; y++) {
+            uint64_t d[2], t[2];
+            e_sub(d, cur + 4 * y + 2, cur + 4 * y);
+            or_q2_mul(ri, d, t);
+            e_add(cur + 2 * y, cur + 4 * y, t);
+        }
+    }
+    memcpy(out, cur, 2 * n * sizeof(uint64_t));
+    free(cur);
+}
+
+/* Sum over y < half of prod_i (a_i + c (b_i - a_i)), c = 0..d, for d polynomials of
+ * original_size elements back to back (base: ext = 0; Ext2 interleaved: ext = 1).
+ * out: (d + 1) Ext2 values (c1 = 0 for base inputs). */
+void or_q_sumcheck_msg(int ext, const uint64_t *polys, int d, uint64_t original_size, uint64_t half, uint64_t *out) {
+    for (int c = 0; c <= d; c++) out[2 * c] = out[2 * c + 1] = 0;
+    uint64_t stride = ext ? 2 * original_size : original_size;
+    for (uint64_t y = 0; y < half; y++) {
+        for (int c = 0; c <= d; c++) {
+            uint64_t prod[2] = {1, 0};
+            for (int i = 0; i < d; i++) {
+                const uint64_t *p = polys + i * stride;
+                uint64_t a[2] = {ext ? cq(p[4 * y]) : cq(p[2 * y]), ext ? cq(p[4 * y + 1]) : 0};
+                uint64_t b[2] = {ext ? cq(p[4 * y + 2]) : cq(p[2 * y + 1]), ext ? cq(p[4 * y + 3]) : 0};
+                uint64_t df[2], v[2], cc[2] = {(uint64_t)c, 0};
+                e_sub(df, b, a);
+                or_q2_mul(cc, df, v);
+                e_add(v, a, v);
+                or_q2_mul(prod, v, prod);
+            }
+            e_add(out + 2 * c, out + 2 * c, prod);
+        }
+    }
+}
+
+/* out[i][y] = a + ch (b - a) for pairs (2y, 2y + 1) of each of d polynomials */
+void or_q_sumcheck_fold(int ext, const uint64_t *in, uint64_t *out, const uint64_t *ch2, int d,
+                        uint64_t original_size, uint64_t half) {
+    uint64_t stride = ext ? 2 * original_size : original_size;
+    for (int i = 0; i < d; i++) {
+        const uint64_t *p = in + i * stride;
+        uint64_t *o = out + i * stride;
+        for (uint64_t y = 0; y < half; y++) {
+            uint64_t a[2] = {ext ? cq(p[4 * y]) : cq(p[2 * y]), ext ? cq(p[4 * y + 1]) : 0};
+            uint64_t b[2] = {ext ? cq(p[4 * y + 2]) : cq(p[2 * y + 1]), ext ? cq(p[4 * y + 3]) : 0};
+            uint64_t df[2], v[2], chv[2] = {cq(ch2[0]), ext ? cq(ch2[1]) : 0};
+            e_sub(df, b, a);
+            or_q2_mul(chv, df, v);
+            e_add(v, a, v);
+            if (ext) {
+                o[2 * y] = v[0];
+                o[2 * y + 1] = v[1];
+            } else {
+                o[y] = v[0];
+            }
+        }
+    }
+}
+
+/* ---- ring Ajtai ---- */
+#define RD 64
+#define RKAPPA 42
+
+/* ajtai_chacha8.cuh:115-178: coefficients 8 blk .. 8 blk + 7 */
+static void prg_block(const uint32_t *key8, uint32_t row, uint64_t j, uint32_t blk, uint64_t *out8) {
+    uint32_t nonce[3] = {row, (uint32_t)j, (uint32_t)(j >> 32)};
+    uint32_t w[16];
+    int written = 0;
+    for (int k = 0; k < 8; k++) out8[k] = 0;
+    for (uint32_t retry = 0; retry < 16 && written < 8; retry++) {
+        or_chacha_block(8, key8, blk + retry * 8u, nonce, w);
+        for (int k = 0; k < 8 && written < 8; k++) {
+            uint64_t s = ((uint64_t)w[2 * k + 1] << 32) | w[2 * k];
+            if (s < QP) out8[written++] = s;
+        }
+    }
+}
+
+void or_ring_matrix_entry(const uint32_t *key8, uint32_t row, uint64_t j, uint64_t *out64) {
+    for (uint32_t b = 0; b < 8; b++) prg_block(key8, row, j, b, out64 + 8 * b);
+}
+
+/* out += a * b in F_q[X]/(X^64 + 1) (naive, ajtai_cpu_reference.cuh:84-101) */
+static void ring_mul_acc(const uint64_t *a, const uint64_t *b, uint64_t *out) {
+    for (int i = 0; i < RD; i++) {
+        if (!a[i]) continue;
+        for (int k = 0; k < RD; k++) {
+            uint64_t v = or_q_mul(a[i], b[k]);
+            int r = i + k;
+            if (r < RD) out[r] = or_q_add(out[r], v);
+            else out[r - RD] = or_q_sub(out[r - RD], v);
+        }
+    }
+}
+
+/* out[b][i][r] for B binary witnesses z[b * N + j] (bit l = coefficient l) */
+void or_ring_commit_binary(const uint32_t *key8, const uint64_t *z, uint64_t N, int B, uint64_t col_offset,
+                           uint64_t *out) {
+    memset(out, 0, (size_t)B * RKAPPA * RD * sizeof(uint64_t));
+    uint64_t m[RD], zr[RD];
+    for (uint64_t j = 0; j < N; j++) {
+        for (int i = 0; i < RKAPPA; i++) {
+            or_ring_matrix_entry(key8, (uint32_t)i, j + col_offset, m);
+            for (int b = 0; b < B; b++) {
+                uint64_t bits = z[(uint64_t)b * N + j];
+                if (!bits) continue;
+                for (int l = 0; l < RD; l++) zr[l] = (bits >> l) & 1;
+                ring_mul_acc(zr, m, out + ((uint64_t)b * RKAPPA + i) * RD);
+            }
+        }
+    }
+}
+
+/* z_wide[j * 64 + l]: field coefficients */
+void or_ring_commit_wide(const uint32_t *key8, const uint64_t *z_wide, uint64_t N, uint64_t col_offset,
+                         uint64_t *out) {
+    memset(out, 0, (size_t)RKAPPA * RD * sizeof(uint64_t));
+    uint64_t m[RD], zr[RD];
+    for (uint64_t j = 0; j < N; j++) {
+        for (int l = 0; l < RD; l++) zr[l] = cq(z_wide[j * RD + l]);
+        for (int i = 0; i < RKAPPA; i++) {
+            or_ring_matrix_entry(key8, (uint32_t)i, j + col_offset, m);
+            ring_mul_acc(zr, m, out + (uint64_t)i * RD);
+        }
+    }
+}
+
+/* positions p = 64 j + l of a binary witness (duplicates add) */
+void or_ring_commit_sparse(const uint32_t *key8, const uint64_t *pos, uint64_t K, uint64_t *out) {
+    memset(out, 0, (size_t)RKAPPA * RD * sizeof(uint64_t));
+    uint64_t m[RD], zr[RD];
+    for (uint64_t k = 0; k < K; k++) {
+        uint64_t j = pos[k] >> 6;
+        int l = (int)(pos[k] & 63);
+        memset(zr, 0, sizeof(zr));
+        zr[l] = 1;
+        for (int i = 0; i < RKAPPA; i++) {
+            or_ring_matrix_entry(key8, (uint32_t)i, j, m);
+            ring_mul_acc(zr, m, out + (uint64_t)i * RD);
+        }
+    }
+}
