@@ -38,6 +38,10 @@ LEAF_LAUNCH_TRAFFIC_BYTES = (65.9 + 224.0) * 2 ** 20
 # HBM bytes of one int8 Ajtai GEMM launch (witness 2^22, 256 rows, batch 1): FETCH_SIZE
 # 540765 KiB x 2.0 (copy calibration) + WRITE_SIZE 8192 KiB, from the rocprofv3 --pmc runs.
 AJTAI_GEMM_TRAFFIC_BYTES = (540765.0 * 2.0 + 8192.0) * 1024
+# Wave-level VALU instructions of k_ring_dense (from its gfx950 ISA): one column fill
+# (ChaCha8 block, rejection check, two LDS rows) and one set bit of one 7-row group.
+RING_PRG_VALU = 440.0
+RING_BIT_VALU = 18.0
 
 
 def parse():
@@ -45,7 +49,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai"], default="basefold")
+    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai"], default="basefold")
     ap.add_argument("--batch", type=int, default=1, help="witnesses per Ajtai commit")
     ap.add_argument("--nv", type=int, default=None)
     ap.add_argument("--queries", type=int, default=100)
@@ -103,6 +107,19 @@ def cpu_baseline_sumcheck(nv):
     dt = time.perf_counter() - t0
     return {"value": (2 << nv) / dt, "unit": "elems/s", "cores": 1, "kind": "port",
             "sample": f"oracle Ext2 sumcheck of f*g, 2 x 2^{nv} evaluations, single thread, {dt:.2f} s"}
+
+
+def cpu_baseline_ring_ajtai(log_cols):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    n = 1 << log_cols
+    z = np.random.default_rng(12).integers(0, 1 << 64, n, dtype=np.uint64)
+    t0 = time.perf_counter()
+    O.ring_commit_binary(RING_KEY, z, n, 1)
+    dt = time.perf_counter() - t0
+    return {"value": 64 * n / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+            "sample": f"oracle ring Ajtai (ChaCha8 matrix + negacyclic products), 42 rows x 2^{log_cols} ring "
+                      f"columns of 64 binary coefficients, single thread, {dt:.2f} s"}
 
 
 def cpu_baseline_ajtai(nv):
@@ -279,6 +296,56 @@ def bench_ajtai(args, M):
                          "avg_launch_ms": avg, "mfma_int8_tops": mfma_tops, "mfma_int8_frac_of_5pops": mfma_tops / 5000.0}}
 
 
+RING_KEY = bytes((7 * i + 3) % 256 for i in range(32))
+
+
+def bench_ring_ajtai(args, M):
+    """Ring Ajtai commit over F_q[X]/(X^64+1), 42 rows: 2^nv binary coefficients per witness
+    (2^(nv-6) ring columns, density 1/2), B witnesses per launch, matrix regenerated by ChaCha8."""
+    from morpheus_amd import ring_ajtai as R
+    nv = args.nv or 22
+    n, b = 1 << (nv - 6), args.batch
+    lib = M.lib
+    z = np.random.default_rng(13).integers(0, 1 << 64, b * n, dtype=np.uint64)
+    seed = R.Seed.from_bytes(RING_KEY)
+    dk, dz, dout = R._key_buffer(seed), M.DeviceBuffer.from_numpy(z), M.DeviceBuffer(b * R.KAPPA * R.RING_DIM)
+    chunk = R.pick_default_chunk(n, b)
+    for _ in range(args.warmup):
+        M.check(lib.ajtai_commit_dense_batched_ffi(dk.ptr, dz.ptr, n, b, chunk, dout.ptr), "ring commit")
+    lib.mam_profile_enable(1)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s0 = time.perf_counter()
+        M.check(lib.ajtai_commit_dense_batched_ffi(dk.ptr, dz.ptr, n, b, chunk, dout.ptr), "ring commit")
+        lat.append(time.perf_counter() - s0)
+    elapsed = time.perf_counter() - t0
+    prof = profile_collect(lib)
+    lib.mam_profile_enable(0)
+    ms, cnt, _ = prof[6]
+    avg = ms / max(cnt, 1)
+    blocks = 42 * n * 8 * ((b + 3) // 4 if b >= 4 else 1)  # ChaCha8 blocks per launch
+    set_bits = int(sum(bin(int(w)).count("1") for w in z))
+    lat_ms = np.array(lat) * 1e3
+    # VALU issue slots per launch: each 64-lane wave issues RING_PRG_VALU instructions per
+    # ChaCha8 column fill (56 blocks) and RING_BIT_VALU per set bit per 7-row group.
+    fills = 6 * n * ((b + 3) // 4 if b >= 4 else 1)
+    slots = (fills * RING_PRG_VALU + set_bits * 6 * RING_BIT_VALU) * 64
+    ach = slots / (avg / 1e3) / 1e12 if avg > 0 else 0.0
+    return {"metric": METRIC, "value": args.steps * b * (1 << nv) / elapsed, "unit": "elems/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64 (mod q)",
+            "data": "synthetic",
+            "config": {"workload": "ring_ajtai_commit", "binary_coeffs": 1 << nv, "ring_columns": n, "kappa": 42,
+                       "ring_dim": 64, "batch": b, "chunk": chunk, "matrix": "ChaCha8, regenerated per launch",
+                       "parallelism": "single-gpu"},
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "roofline": {"kernel": "k_ring_dense (ChaCha8 + LDS negacyclic rows)", "bound": "valu",
+                         "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
+                         "traffic": None, "avg_launch_ms": avg, "chacha_blocks_per_s": blocks / (avg / 1e3),
+                         "set_bits": set_bits}}
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -304,15 +371,19 @@ def main():
         out = bench_basefold(args, M)
     elif args.workload == "sumcheck":
         out = bench_sumcheck(args, M)
-    else:
+    elif args.workload == "ajtai":
         out = bench_ajtai(args, M)
+    else:
+        out = bench_ring_ajtai(args, M)
     if not args.no_cpu:
         if args.workload == "basefold":
             out["cpu_baseline"] = cpu_baseline_basefold(args.cpu_sample_nv or 18, 1, args.queries)
         elif args.workload == "sumcheck":
             out["cpu_baseline"] = cpu_baseline_sumcheck(args.cpu_sample_nv or 20)
-        else:
+        elif args.workload == "ajtai":
             out["cpu_baseline"] = cpu_baseline_ajtai(args.cpu_sample_nv or 16)
+        else:
+            out["cpu_baseline"] = cpu_baseline_ring_ajtai(args.cpu_sample_nv or 13)
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
         out["cpu_baseline"]["nproc"] = os.cpu_count()
     print(json.dumps(out), flush=True)

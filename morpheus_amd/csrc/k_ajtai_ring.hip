@@ -229,31 +229,57 @@ __global__ __launch_bounds__(64) void k_ring_sparse(const u32 *__restrict__ d_ke
         partial[((u64)blockIdx.x * RING_KAPPA + g * RING_RG + ii) * RING_D + r] = ring_finish(lo[ii], hi[ii]);
 }
 
-// out[t] = sum over chunks of partial[chunk * total + t] (mod q)
-__global__ void k_ring_reduce(const u64 *__restrict__ partial, u64 chunks, u64 total, u64 *__restrict__ out) {
+// out[s * total + t] = sum of partial[c * total + t] over chunks c in slice s (mod q):
+// grid (ceil(total / 256), slices); consecutive threads read consecutive words.
+__global__ void k_ring_reduce(const u64 *__restrict__ partial, u64 chunks, u64 total, u64 per,
+                              u64 *__restrict__ out) {
     const u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x;
     if (t >= total) return;
+    const u64 c0 = blockIdx.y * per;
+    const u64 c1 = c0 + per < chunks ? c0 + per : chunks;
     u64 lo = 0, hi = 0;
-    for (u64 c = 0; c < chunks; c++) fq::acc_add(lo, hi, partial[c * total + t]);
-    out[t] = fq::reduce128(lo, hi);
+    for (u64 c = c0; c < c1; c++) fq::acc_add(lo, hi, partial[c * total + t]);
+    out[blockIdx.y * total + t] = fq::reduce128(lo, hi);
 }
 
 bool ring_chunk_ok(int chunk) {
     return chunk == 64 || chunk == 128 || chunk == 256 || chunk == 1024 || chunk == 4096;
 }
 
-// Columns per workgroup: at most the caller's chunk, fewer while the grid would leave the
-// 256 CUs short of about 16 workgroups each.
+// Columns per workgroup: at most the caller's chunk, fewer while the grid would hold fewer
+// than 16384 single-wave workgroups (several rounds of the ~5 resident waves per SIMD on
+// 1024 SIMDs, so the last round's tail is a small share).
 u64 ring_cols(u64 n, int chunk, u64 groups) {
-    const u64 target = 4096;
+    const u64 target = 16384;
     u64 cols = (n * groups + target - 1) / target;
     if (cols < 1) cols = 1;
     if (cols > (u64)chunk) cols = (u64)chunk;
     return cols;
 }
 
-int ring_reduce_to(const u64 *partial, u64 chunks, u64 total, u64 *d_out, hipStream_t s) {
-    k_ring_reduce<<<grid_1d(total), MAM_BLOCK, 0, s>>>(partial, chunks, total, d_out);
+// Sum the chunk partials in two passes (slices of >= 32 chunks, then the slices) so that each
+// thread adds at most a few dozen words; `partial` must hold room for the slice sums after
+// the chunk partials (ring_partial_words).
+u64 ring_slice_len(u64 chunks) {
+    const u64 per = (chunks + 65534) / 65535;  // grid.y stays within 65535
+    return per > 32 ? per : 32;
+}
+u64 ring_partial_words(u64 chunks, u64 total) {
+    const u64 per = ring_slice_len(chunks);
+    return (chunks + (chunks + per - 1) / per) * total;
+}
+
+int ring_reduce_to(u64 *partial, u64 chunks, u64 total, u64 *d_out, hipStream_t s) {
+    const u64 per = ring_slice_len(chunks);
+    const u64 slices = (chunks + per - 1) / per;
+    u64 *mid = partial + chunks * total;
+    const unsigned gx = grid_1d(total);
+    if (slices > 1) {
+        k_ring_reduce<<<dim3(gx, (unsigned)slices), MAM_BLOCK, 0, s>>>(partial, chunks, total, per, mid);
+        k_ring_reduce<<<dim3(gx, 1), MAM_BLOCK, 0, s>>>(mid, slices, total, slices, d_out);
+    } else {
+        k_ring_reduce<<<dim3(gx, 1), MAM_BLOCK, 0, s>>>(partial, chunks, total, chunks, d_out);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -273,7 +299,8 @@ MAM_API int ajtai_commit_dense_batched_at_ffi(const u32 *d_key, const u64 *d_z, 
     const u64 total = (u64)B * RING_KAPPA * RING_D;
     if (chunks > 0x7FFFFFFFull) return -1;
     u64 *partial = nullptr;
-    if (hipMallocAsync((void **)&partial, chunks * total * sizeof(u64), s) != hipSuccess) return -1;
+    if (hipMallocAsync((void **)&partial, ring_partial_words(chunks, total) * sizeof(u64), s) != hipSuccess)
+        return -1;
     const dim3 grid((unsigned)chunks, RING_GROUPS, (unsigned)wgroups);
     {
         ProfScope prof(PROF_AJTAI_RING, (double)N * B, s);
@@ -303,7 +330,8 @@ MAM_API int ajtai_commit_wide_ffi(const u32 *d_key, const u64 *d_z_wide, u64 N, 
     const u64 total = (u64)RING_KAPPA * RING_D;
     if (chunks > 0x7FFFFFFFull) return -1;
     u64 *partial = nullptr;
-    if (hipMallocAsync((void **)&partial, chunks * total * sizeof(u64), s) != hipSuccess) return -1;
+    if (hipMallocAsync((void **)&partial, ring_partial_words(chunks, total) * sizeof(u64), s) != hipSuccess)
+        return -1;
     k_ring_wide<<<dim3((unsigned)chunks, RING_GROUPS), 64, 0, s>>>(d_key, d_z_wide, N, col_offset, cols, partial);
     int rc = hipGetLastError() == hipSuccess ? 0 : -1;
     if (!rc) rc = ring_reduce_to(partial, chunks, total, d_out, s);
@@ -318,14 +346,15 @@ MAM_API int ajtai_commit_sparse_ffi(const u32 *d_key, const u64 *d_positions, u6
     hipStream_t s = mam_stream();
     const u64 total = (u64)RING_KAPPA * RING_D;
     if (K == 0) {
-        k_ring_reduce<<<grid_1d(total), MAM_BLOCK, 0, s>>>(nullptr, 0, total, d_out);
+        k_ring_reduce<<<dim3(grid_1d(total), 1), MAM_BLOCK, 0, s>>>(nullptr, 0, total, 1, d_out);
         return mam_sync_status();
     }
     const u64 per = ring_cols(K, chunk, RING_GROUPS);
     const u64 chunks = (K + per - 1) / per;
     if (chunks > 0x7FFFFFFFull) return -1;
     u64 *partial = nullptr;
-    if (hipMallocAsync((void **)&partial, chunks * total * sizeof(u64), s) != hipSuccess) return -1;
+    if (hipMallocAsync((void **)&partial, ring_partial_words(chunks, total) * sizeof(u64), s) != hipSuccess)
+        return -1;
     k_ring_sparse<<<dim3((unsigned)chunks, RING_GROUPS), 64, 0, s>>>(d_key, d_positions, K, per, 1u, partial);
     int rc = hipGetLastError() == hipSuccess ? 0 : -1;
     if (!rc) rc = ring_reduce_to(partial, chunks, total, d_out, s);
