@@ -49,7 +49,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai"], default="basefold")
+    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch"], default="basefold")
+    ap.add_argument("--instances", type=int, default=64, help="instances per batch (batch workload)")
+    ap.add_argument("--workers", type=int, default=0, help="batch workers (0: autotune)")
     ap.add_argument("--batch", type=int, default=1, help="witnesses per Ajtai commit")
     ap.add_argument("--nv", type=int, default=None)
     ap.add_argument("--queries", type=int, default=100)
@@ -296,6 +298,87 @@ def bench_ajtai(args, M):
                          "avg_launch_ms": avg, "mfma_int8_tops": mfma_tops, "mfma_int8_frac_of_5pops": mfma_tops / 5000.0}}
 
 
+def batch_sizes(count, seed=4):
+    """log2 size ~ U[12, 22] (continuous, seeded); each instance is padded with zeros to the
+    next power of two. Returns (real sizes, num_vars)."""
+    u = np.random.default_rng(seed).uniform(12.0, 22.0, count)
+    n = np.floor(2.0 ** u).astype(np.int64)
+    nv = np.ceil(np.log2(n)).astype(np.int64)
+    return n, nv
+
+
+def merkle_perms(nv, lr):
+    """Monolith permutations of one commit (2^(nv+lr) - 1) plus one opening (sum of flen - 1)."""
+    total = (1 << (nv + lr)) - 1
+    for r in range(nv - 1):
+        total += (1 << (nv + lr - 1 - r)) - 1
+    return total
+
+
+def bench_batch(args, M):
+    """64 sumcheck + Basefold instances of log-uniform size 2^12..2^22, proved concurrently by
+    persistent worker streams (count autotuned) against one-at-a-time proving."""
+    from morpheus_amd.batch import Instance, prove_batch
+    lr, nq = 1, args.queries
+    sizes, nvs = batch_sizes(args.instances)
+    rng = np.random.default_rng(5)
+    insts = []
+    for i, (n, nv) in enumerate(zip(sizes, nvs)):
+        fg = np.zeros(2 << int(nv), dtype=np.uint64)
+        fg[:n] = rng.integers(0, M.P, int(n), dtype=np.uint64)
+        fg[(1 << int(nv)):(1 << int(nv)) + n] = rng.integers(0, M.P, int(n), dtype=np.uint64)
+        insts.append(Instance(fg[:1 << int(nv)], fg[1 << int(nv):], b"c4-%d" % i))
+    tnv = int(nvs.max())
+    table = M.BasefoldTable.generate(tnv, lr, tnv, 42)
+    table.upload()
+    total = int(sizes.sum())
+    perms = sum(merkle_perms(int(v), lr) for v in nvs)
+
+    def run(w, sub=None):
+        batch = insts if sub is None else [insts[i] for i in sub]
+        t0 = time.perf_counter()
+        out = prove_batch(batch, table, lr, 0, nq, workers=w)
+        return time.perf_counter() - t0, out
+
+    # autotune the worker count on a quarter of the instances (every fourth), untimed
+    cands = [args.workers] if args.workers else [1, 2, 4, 8, 12, 16]
+    sub = list(range(0, len(insts), 4))
+    run(cands[-1], sub)  # warm every candidate's threads and buffers
+    tune = {}
+    for w in cands:
+        run(w, sub)
+        tune[w] = min(run(w, sub)[0] for _ in range(2))
+    best = min(tune, key=tune.get)
+    for _ in range(max(args.warmup, 1)):
+        run(best)
+        run(1)
+    seq_t, seq_lat, bat_t, bat_lat = [], [], [], []
+    for _ in range(args.steps):
+        dt, out = run(1)
+        seq_t.append(dt)
+        seq_lat += [o["end_ms"] - o["start_ms"] for o in out]
+        dt, out = run(best)
+        bat_t.append(dt)
+        bat_lat += [o["end_ms"] - o["start_ms"] for o in out]
+    seq_s, bat_s = float(np.mean(seq_t)), float(np.mean(bat_t))
+    ach = perms * VALU_OPS_PER_PERM / bat_s / 1e12
+    return {"metric": METRIC, "value": total / bat_s, "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": bat_s * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64 (Goldilocks, Ext2 challenges)", "data": "synthetic",
+            "config": {"workload": "batched_sumcheck_basefold", "instances": len(insts), "total_elems": total,
+                       "log2_size": "U[12,22] seeded, zero-padded to 2^ceil", "num_vars": [int(v) for v in nvs],
+                       "log_rate": lr, "num_queries": nq, "merkle_hash": "monolith-w12", "workers": best,
+                       "autotune_s": {str(k): v for k, v in tune.items()}, "parallelism": "single-gpu"},
+            "proofs_per_s": len(insts) / bat_s,
+            "sequential": {"value": total / seq_s, "ms_per_batch": seq_s * 1e3, "proofs_per_s": len(insts) / seq_s,
+                           "p50_ms": float(np.percentile(seq_lat, 50)), "p99_ms": float(np.percentile(seq_lat, 99))},
+            "speedup_vs_sequential": seq_s / bat_s,
+            "p50_ms": float(np.percentile(bat_lat, 50)), "p99_ms": float(np.percentile(bat_lat, 99)),
+            "roofline": {"kernel": "all Monolith Merkle work of the batch over its wall time", "bound": "valu",
+                         "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
+                         "traffic": None, "perms": perms, "valu_ops_per_perm": VALU_OPS_PER_PERM}}
+
+
 RING_KEY = bytes((7 * i + 3) % 256 for i in range(32))
 
 
@@ -373,6 +456,8 @@ def main():
         out = bench_sumcheck(args, M)
     elif args.workload == "ajtai":
         out = bench_ajtai(args, M)
+    elif args.workload == "batch":
+        out = bench_batch(args, M)
     else:
         out = bench_ring_ajtai(args, M)
     if not args.no_cpu:
@@ -382,6 +467,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline_sumcheck(args.cpu_sample_nv or 20)
         elif args.workload == "ajtai":
             out["cpu_baseline"] = cpu_baseline_ajtai(args.cpu_sample_nv or 16)
+        elif args.workload == "batch":
+            out["cpu_baseline"] = cpu_baseline_basefold(args.cpu_sample_nv or 18, 1, args.queries)
         else:
             out["cpu_baseline"] = cpu_baseline_ring_ajtai(args.cpu_sample_nv or 13)
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()

@@ -93,6 +93,29 @@ int mam_basefold_verify_ext2(const uint64_t *root4, const uint64_t *point2, int 
                              const uint64_t *proof, int num_queries, const uint64_t *table, int table_nv,
                              mam_transcript *t);
 
+/* ---- batched proving of uneven instances (benchmark configuration 4) ----
+ * No single reference function: each instance runs the reference's product sumcheck
+ * (zk-torch-3/src/sumcheck/gpu_prover.rs:21-89) on T = Transcript::new(label), then
+ * commits f (basefold.rs:382-414) and opens it at the sumcheck challenges on the same T
+ * (basefold.rs:578-863). `workers` persistent host threads, each on its own stream, take
+ * instances largest first; workers = 1 is one-at-a-time proving. Inputs on the device,
+ * outputs on the host; start/end times in ms from the batch start. num_vars >= 2. */
+typedef struct {
+    const uint64_t *d_fg;  /* f then g, 2^num_vars base words each (device) */
+    int num_vars;
+    const uint8_t *label;
+    long label_len;
+    uint64_t *round_msgs;  /* [num_vars][3][2] */
+    uint64_t *challenges;  /* [num_vars][2] */
+    uint64_t *final_evals; /* [2][2] */
+    uint64_t *root;        /* [4] */
+    uint64_t *proof;       /* mam_basefold_proof_len(num_vars, log_rate, num_queries) words */
+    double start_ms, end_ms;
+    int status;
+} mam_batch_instance;
+int mam_prove_batch(mam_batch_instance *inst, int n, const uint64_t *d_table, int table_nv, int log_rate, int hash,
+                    int num_queries, int workers);
+
 /* ---- int8-matrix Ajtai commitment on INT8 MFMA (benchmark configuration 3) ----
  * No reference function has these semantics (the reference's Ajtai is the ring-SIS commit
  * of almost-goldilocks-cuda-rs/src/ajtai.rs:138-386; its INT8 paths are a multifold GEMM,

@@ -33,6 +33,12 @@ class Transcript_t(ctypes.Structure):
 
 trp = ctypes.POINTER(Transcript_t)
 
+
+class BatchInstance_t(ctypes.Structure):
+    _fields_ = [("d_fg", vp), ("num_vars", ci), ("label", ctypes.c_char_p), ("label_len", cl),
+                ("round_msgs", u64p), ("challenges", u64p), ("final_evals", u64p), ("root", u64p),
+                ("proof", u64p), ("start_ms", ctypes.c_double), ("end_ms", ctypes.c_double), ("status", ci)]
+
 _SIGS = {
     # reference-named (morpheus_gl.h)
     "goldilocks_cuda_init": (ci, []),
@@ -171,6 +177,7 @@ _SIGS = {
     "ajtai_commit_wide_ffi": (ci, [vp, vp, u64, u64, ci, vp]),
     "ajtai_commit_sparse_ffi": (ci, [vp, vp, u64, ci, vp]),
     "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
+    "mam_prove_batch": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci]),
     "mam_profile_enable": (ci, [ci]),
     "mam_release_scratch": (ci, []),
     "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),

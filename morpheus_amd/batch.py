@@ -1,0 +1,49 @@
+"""Batched proving of independent sumcheck + Basefold instances of uneven size
+(include/morpheus_prover.h, mam_prove_batch): each instance proves sum f*g with the product
+sumcheck and opens a Basefold commitment of f at the sumcheck challenges, on one transcript."""
+import ctypes
+
+import numpy as np
+
+from ._lib import BatchInstance_t, lib
+from .device import DeviceBuffer, check
+from .prover import proof_len
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+
+
+class Instance:
+    """One instance: f and g (2^num_vars base words each) resident on the device."""
+
+    def __init__(self, f, g, label, d_fg=None):
+        f = np.ascontiguousarray(f, dtype=np.uint64)
+        g = np.ascontiguousarray(g, dtype=np.uint64)
+        assert f.size == g.size and f.size & (f.size - 1) == 0
+        self.num_vars = f.size.bit_length() - 1
+        self.label = bytes(label)
+        self.d_fg = d_fg if d_fg is not None else DeviceBuffer.from_numpy(np.concatenate([f, g]))
+
+
+def prove_batch(instances, table, log_rate=1, hash_kind=0, num_queries=100, workers=1):
+    """Returns one dict per instance (round_msgs, challenges, final_evals, root, proof,
+    start_ms, end_ms) in input order."""
+    d_table = table.upload()
+    n = len(instances)
+    arr = (BatchInstance_t * max(n, 1))()
+    outs = []
+    for i, inst in enumerate(instances):
+        nv = inst.num_vars
+        o = {"round_msgs": np.zeros((nv, 3, 2), dtype=np.uint64), "challenges": np.zeros((nv, 2), dtype=np.uint64),
+             "final_evals": np.zeros((2, 2), dtype=np.uint64), "root": np.zeros(4, dtype=np.uint64),
+             "proof": np.zeros(proof_len(nv, log_rate, num_queries), dtype=np.uint64)}
+        outs.append(o)
+        a = arr[i]
+        a.d_fg, a.num_vars = inst.d_fg.ptr, nv
+        a.label, a.label_len = inst.label, len(inst.label)
+        for k in ("round_msgs", "challenges", "final_evals", "root", "proof"):
+            setattr(a, k, o[k].ctypes.data_as(_u64p))
+    check(lib.mam_prove_batch(arr, n, d_table.ptr, table.num_vars, log_rate, hash_kind, num_queries, workers),
+          "mam_prove_batch")
+    for i, o in enumerate(outs):
+        o["start_ms"], o["end_ms"] = arr[i].start_ms, arr[i].end_ms
+    return outs
