@@ -25,13 +25,40 @@ sys.path.insert(0, ROOT)
 
 METRIC = "proved Goldilocks elems/s (sumcheck+Basefold, 2^22); proofs/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
-# 32-bit VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz (AMD's 157.3 TFLOP/s
-# FP32 vector figure counts an FMA as 2 flops on 2 packed lanes: 157.3 / 4 = 39.3).
-VALU_PEAK_TOPS = 39.3
+# Nominal 32-bit VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz (AMD's
+# 157.3 TFLOP/s FP32 vector figure counts an FMA as 2 flops on 2 packed lanes: 157.3 / 4).
+# The roofline uses the rate measured on the box by the VALU probe (valu_peak()): chains of
+# one instruction form, which issue at one wave instruction per ~4.1 cycles per SIMD. The
+# hash kernels' mix (about a third moves, selects and compares) issues at 3.3-3.7 cycles
+# per instruction (rocprofv3 SQ_INSTS_VALU and GRBM_GUI_ACTIVE), so a hash launch at the
+# VALU issue limit reads as frac slightly above 1.
+VALU_NOMINAL_TOPS = 39.3
+VALU_PEAK_TOPS = VALU_NOMINAL_TOPS  # replaced by the measured rate in main()
+
+
+VALU_PROBE = {}
+
+
+def valu_peak(M):
+    """Measured VALU issue rate (wave instructions x 64 lanes, Tops/s): the fastest of the
+    single-instruction chain probes (v_xad_u32, v_mad_u64_u32, v_lshl_add_u64, v_add_u32),
+    8192 workgroups x 256 threads x 2048 x 8 steps each."""
+    blocks, iters = 8192, 2048
+    names = ("v_xad_u32", "v_mad_u64_u32", "v_lshl_add_u64", "v_add_u32")
+    for kind, name in enumerate(names):
+        ms = ctypes.c_double()
+        M.check(M.lib.mam_valu_probe(kind, blocks, iters, ctypes.byref(ms)), "mam_valu_probe")
+        waves = blocks * 256 // 64
+        VALU_PROBE[name] = waves * iters * 8 * 64 / (ms.value / 1e3) / 1e12
+    return max(VALU_PROBE.values())
 # VALU lane-operations of one Monolith permutation in the leaf kernel: SQ_INSTS_VALU x 64
 # over the permutations of one k_tree_levels<MonoHash, SRC_GL> dispatch (rocprofv3 --pmc,
 # profiles/merkle_leaf_kernel_2p22.md).
 VALU_OPS_PER_PERM = 6451.0
+# VALU instructions one lane executes for one full Monolith permutation with every lane
+# active: SQ_INSTS_VALU / SQ_WAVES of k_mono_permute_n (rocprofv3 --pmc,
+# profiles/monolith_permutation_valu.md). Lane-operations/s = permutations/s x this.
+MONO_PERM_VALU = 6968.0
 # HBM bytes of that dispatch at 2^22 leaves: FETCH_SIZE x 2 (a 256 MiB copy reads back as
 # 128 MiB on this counter) + WRITE_SIZE (exact for the copy), same profile file.
 LEAF_LAUNCH_TRAFFIC_BYTES = (65.9 + 224.0) * 2 ** 20
@@ -49,7 +76,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch"], default="basefold")
+    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch", "hash"],
+                    default="basefold")
     ap.add_argument("--instances", type=int, default=64, help="instances per batch (batch workload)")
     ap.add_argument("--workers", type=int, default=0, help="batch workers (0: autotune)")
     ap.add_argument("--batch", type=int, default=1, help="witnesses per Ajtai commit")
@@ -109,6 +137,20 @@ def cpu_baseline_sumcheck(nv):
     dt = time.perf_counter() - t0
     return {"value": (2 << nv) / dt, "unit": "elems/s", "cores": 1, "kind": "port",
             "sample": f"oracle Ext2 sumcheck of f*g, 2 x 2^{nv} evaluations, single thread, {dt:.2f} s"}
+
+
+def cpu_baseline_hash(nv):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    n = 1 << nv
+    st = np.random.default_rng(7).integers(0, 1 << 63, 12 * n, dtype=np.uint64)
+    t0 = time.perf_counter()
+    for i in range(n):
+        O.L.or_monolith_permute(O._p(st[12 * i:12 * i + 12]))
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "perms/s", "cores": 1, "kind": "port",
+            "sample": f"oracle Monolith-w12 permutation, 2^{nv} states (ctypes call per state), single thread, "
+                      f"{dt:.2f} s"}
 
 
 def cpu_baseline_ring_ajtai(log_cols):
@@ -187,7 +229,8 @@ def bench_basefold(args, M):
     perm_rate = perms_per_launch / (lk_avg_ms / 1e3) if lk_ms > 0 else 0.0
     ach = perm_rate * VALU_OPS_PER_PERM / 1e12
     roof = {"kernel": "k_tree_levels<MonoHash, SRC_GL> (2^22 leaves + 2 in-LDS levels)",
-            "bound": "valu", "unit": "Tops/s", "peak": VALU_PEAK_TOPS, "achieved": ach,
+            "bound": "valu", "unit": "Tops/s", "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
+            "peak_source": "VALU probe (mam_valu_probe)", "probe_tops": dict(VALU_PROBE), "achieved": ach,
             "frac": ach / VALU_PEAK_TOPS, "traffic": LEAF_LAUNCH_TRAFFIC_BYTES if nv == 22 else None,
             "avg_launch_ms": lk_avg_ms, "launches": lk_n, "perms_per_launch": perms_per_launch,
             "perms_per_s": perm_rate, "valu_ops_per_perm": VALU_OPS_PER_PERM}
@@ -298,6 +341,41 @@ def bench_ajtai(args, M):
                          "avg_launch_ms": avg, "mfma_int8_tops": mfma_tops, "mfma_int8_frac_of_5pops": mfma_tops / 5000.0}}
 
 
+def bench_hash(args, M):
+    """Raw permutation throughput, one permutation per lane with every lane active:
+    Monolith-w12 (mam_monolith_permute_batch) and Poseidon2-w8 (poseidon2_hash_batch_ffi)
+    over 2^nv states resident in HBM."""
+    nv = args.nv or 22
+    n = 1 << nv
+    lib = M.lib
+    rng = np.random.default_rng(6)
+    d12 = M.DeviceBuffer.from_numpy(rng.integers(0, M.P, 12 * n, dtype=np.uint64))
+    d8 = M.DeviceBuffer.from_numpy(rng.integers(0, M.P, 8 * n, dtype=np.uint64))
+    o8 = M.DeviceBuffer(8 * n)
+    res = {}
+    for name, call in (("monolith_w12", lambda: lib.mam_monolith_permute_batch(d12.ptr, n)),
+                       ("poseidon2_w8", lambda: lib.poseidon2_hash_batch_ffi(d8.ptr, o8.ptr, n))):
+        for _ in range(args.warmup):
+            M.check(call(), name)
+        M.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            M.check(call(), name)
+        M.synchronize()
+        res[name] = args.steps * n / (time.perf_counter() - t0)
+    ach = res["monolith_w12"] * MONO_PERM_VALU / 1e12
+    return {"metric": METRIC, "value": res["monolith_w12"], "unit": "perms/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": n / res["monolith_w12"] * 1e3, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u64 (Goldilocks)", "data": "synthetic",
+            "config": {"workload": "hash_permutations", "states": n, "hashes": ["monolith_w12", "poseidon2_w8"],
+                       "parallelism": "single-gpu"},
+            "perms_per_s": res,
+            "roofline": {"kernel": "k_mono_permute_n (one Monolith permutation per lane)", "bound": "valu",
+                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
+                         "probe_tops": dict(VALU_PROBE), "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
+                         "traffic": None, "valu_per_perm": MONO_PERM_VALU}}
+
+
 def batch_sizes(count, seed=4):
     """log2 size ~ U[12, 22] (continuous, seeded); each instance is padded with zeros to the
     next power of two. Returns (real sizes, num_vars)."""
@@ -375,7 +453,8 @@ def bench_batch(args, M):
             "speedup_vs_sequential": seq_s / bat_s,
             "p50_ms": float(np.percentile(bat_lat, 50)), "p99_ms": float(np.percentile(bat_lat, 99)),
             "roofline": {"kernel": "all Monolith Merkle work of the batch over its wall time", "bound": "valu",
-                         "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
+                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
+                         "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
                          "traffic": None, "perms": perms, "valu_ops_per_perm": VALU_OPS_PER_PERM}}
 
 
@@ -424,7 +503,8 @@ def bench_ring_ajtai(args, M):
                        "parallelism": "single-gpu"},
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
             "roofline": {"kernel": "k_ring_dense (ChaCha8 + LDS negacyclic rows)", "bound": "valu",
-                         "achieved": ach, "peak": VALU_PEAK_TOPS, "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
+                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
+                         "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
                          "traffic": None, "avg_launch_ms": avg, "chacha_blocks_per_s": blocks / (avg / 1e3),
                          "set_bits": set_bits}}
 
@@ -440,6 +520,8 @@ def main():
     if M.lib.goldilocks_cuda_init() != 0:
         print("no usable GPU", file=sys.stderr)
         sys.exit(1)
+    global VALU_PEAK_TOPS
+    VALU_PEAK_TOPS = valu_peak(M)
     if args.calibrate_copy:
         n = 1 << 25
         a, b = M.DeviceBuffer(n), M.DeviceBuffer(n)
@@ -448,7 +530,8 @@ def main():
             t0 = time.perf_counter()
             M.check(M.lib.cuda_memcpy_dtod(b.ptr, a.ptr, n * 8), "copy")
             dt = time.perf_counter() - t0
-        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "wall_ms": dt * 1e3}), flush=True)
+        print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "wall_ms": dt * 1e3,
+                          "valu_probe_tops": VALU_PROBE}), flush=True)
         return
     if args.workload == "basefold":
         out = bench_basefold(args, M)
@@ -458,6 +541,8 @@ def main():
         out = bench_ajtai(args, M)
     elif args.workload == "batch":
         out = bench_batch(args, M)
+    elif args.workload == "hash":
+        out = bench_hash(args, M)
     else:
         out = bench_ring_ajtai(args, M)
     if not args.no_cpu:
@@ -469,6 +554,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline_ajtai(args.cpu_sample_nv or 16)
         elif args.workload == "batch":
             out["cpu_baseline"] = cpu_baseline_basefold(args.cpu_sample_nv or 18, 1, args.queries)
+        elif args.workload == "hash":
+            out["cpu_baseline"] = cpu_baseline_hash(args.cpu_sample_nv or 18)
         else:
             out["cpu_baseline"] = cpu_baseline_ring_ajtai(args.cpu_sample_nv or 13)
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()

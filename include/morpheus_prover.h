@@ -144,6 +144,10 @@ int mam_merkle_tree_async(const uint64_t *d_codeword, uint64_t *d_tree, size_t n
  * 5 int8 Ajtai GEMM launches (bytes), 6 dense ring-Ajtai launches (ring columns x witnesses).
  * collect() fills 7 entries of each array. */
 int mam_profile_enable(int on);
+/* VALU issue-rate microbenchmark (no reference counterpart; used by bench.py for the
+ * measured VALU peak): kind 0 = v_xad_u32 chains, 1 = v_mad_u64_u32 chains; each thread of
+ * `blocks` x 256 runs iters x 8 single-instruction steps. *ms = the timed launch. */
+int mam_valu_probe(int kind, int blocks, int iters, double *ms);
 int mam_profile_collect(double *ms, long *count, double *work);
 
 #ifdef __cplusplus

@@ -179,6 +179,7 @@ _SIGS = {
     "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
     "mam_prove_batch": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci]),
     "mam_profile_enable": (ci, [ci]),
+    "mam_valu_probe": (ci, [ci, ci, ci, ctypes.POINTER(ctypes.c_double)]),
     "mam_release_scratch": (ci, []),
     "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),
 }
