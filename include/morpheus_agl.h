@@ -65,6 +65,11 @@ int agl_partial_eval_ffi(uint64_t *d_data, const uint64_t *d_r, int log_n, int m
 int agl_partial_eval_ext2_from_base_ffi(const uint64_t *d_input, uint64_t *d_output, const uint64_t *d_r, int log_n,
                                         int m);
 
+/* fused permute + partial evaluation, ffi.rs:214-226 (semantics as the Goldilocks twin) */
+int agl_fused_permute_partial_eval_ffi(const uint64_t *d_evals, uint64_t *d_output, const uint64_t *d_eq_table,
+                                       const uint32_t *d_lo_lut, const uint32_t *d_hi_lut, int n, int m, int half,
+                                       int output_size, int smem_bytes);
+
 /* sumcheck with per-workgroup partials, ffi.rs:228-263 */
 int agl_sumcheck_round_message_ffi(const uint64_t *d_polys, uint64_t *d_partial, int d, size_t original_size,
                                    size_t half, int num_blocks);

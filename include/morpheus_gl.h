@@ -143,6 +143,12 @@ int sumcheck_round_message_ext2_ffi(const uint64_t *d_polys, uint64_t *d_partial
 int sumcheck_fold_ext2_ffi(const uint64_t *d_input, uint64_t *d_output, uint64_t challenge_c0,
                            uint64_t challenge_c1, int d, size_t original_size, size_t half);   /* ffi.rs:619 */
 
+/* fused variable permutation + partial evaluation, ffi.rs:657-668:
+ * output[j] = sum_{b < 2^m} evals[lo_lut[x & (2^half-1)] | hi_lut[x >> half]] * eq[b], x = j 2^m + b */
+int fused_permute_partial_eval_ffi(const uint64_t *d_evals, uint64_t *d_output, const uint64_t *d_eq_table,
+                                   const uint32_t *d_lo_lut, const uint32_t *d_hi_lut, int n, int m, int half,
+                                   int output_size, int smem_bytes);
+
 #ifdef __cplusplus
 }
 #endif

@@ -116,6 +116,31 @@ typedef struct {
 int mam_prove_batch(mam_batch_instance *inst, int n, const uint64_t *d_table, int table_nv, int log_rate, int hash,
                     int num_queries, int workers);
 
+/* ---- matmul proof C = A B (benchmark configuration 5) ----
+ * Composed from the einsum proof of zk-torch-3/src/basicblock/einsum.rs:807-1098 for
+ * "ij,jk->ik": Basefold commitments of A (m x k), B (k x n), C (m x n) (row-major, m = 2^lm,
+ * ...), an optional int8 Ajtai commitment of C, the claim C(r) at a transcript point, one
+ * sumcheck over the inner index of [A(r_i, .), B(., r_j), 1], and Basefold openings of A, B, C
+ * at their points, all on T = Transcript::new(label). The step list is in host_matmul.hip. */
+typedef struct {
+    const uint64_t *d_a, *d_b, *d_c; /* device, canonical words, row-major */
+    int lm, lk, ln;
+    const uint8_t *label;
+    long label_len;
+    uint64_t *roots;       /* [3][4] */
+    uint64_t *ajtai;       /* [ajtai_rows] or NULL */
+    uint64_t *point;       /* [ln + lm][2]: r_j then r_i */
+    uint64_t *claim;       /* [2] */
+    uint64_t *round_msgs;  /* [lk][4][2] */
+    uint64_t *challenges;  /* [lk][2] */
+    uint64_t *final_evals; /* [3][2] */
+    uint64_t *proof_a, *proof_b, *proof_c;
+    double start_ms, end_ms;
+    int status;
+} mam_matmul_instance;
+int mam_matmul_prove(mam_matmul_instance *inst, const uint64_t *d_table, int table_nv, int log_rate, int hash,
+                     int num_queries, void *ajtai_handle, int ajtai_rows);
+
 /* ---- int8-matrix Ajtai commitment on INT8 MFMA (benchmark configuration 3) ----
  * No reference function has these semantics (the reference's Ajtai is the ring-SIS commit
  * of almost-goldilocks-cuda-rs/src/ajtai.rs:138-386; its INT8 paths are a multifold GEMM,
@@ -126,6 +151,9 @@ int mam_prove_batch(mam_batch_instance *inst, int n, const uint64_t *d_table, in
  * multiple of 64. commit: batch <= 32 witnesses of n words each (device), out device. */
 int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void **handle);
 int mam_ajtai_i8_commit(void *handle, const uint64_t *d_z, int batch, uint64_t *d_out);
+/* The same against the first n_used columns (witnesses of n_used words; n_used a multiple of
+ * 64, <= n; batch <= 16 when n_used < n): equal to a commitment with an n_used-column set-up. */
+int mam_ajtai_i8_commit_prefix(void *handle, const uint64_t *d_z, int batch, uint64_t n_used, uint64_t *d_out);
 int mam_ajtai_i8_matrix_row(void *handle, int row, uint64_t col0, uint64_t count, int8_t *h_out);
 int mam_ajtai_i8_free(void *handle);
 

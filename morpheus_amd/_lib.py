@@ -34,6 +34,14 @@ class Transcript_t(ctypes.Structure):
 trp = ctypes.POINTER(Transcript_t)
 
 
+class MatmulInstance_t(ctypes.Structure):
+    _fields_ = [("d_a", vp), ("d_b", vp), ("d_c", vp), ("lm", ci), ("lk", ci), ("ln", ci),
+                ("label", ctypes.c_char_p), ("label_len", cl), ("roots", u64p), ("ajtai", u64p), ("point", u64p),
+                ("claim", u64p), ("round_msgs", u64p), ("challenges", u64p), ("final_evals", u64p),
+                ("proof_a", u64p), ("proof_b", u64p), ("proof_c", u64p), ("start_ms", ctypes.c_double),
+                ("end_ms", ctypes.c_double), ("status", ci)]
+
+
 class BatchInstance_t(ctypes.Structure):
     _fields_ = [("d_fg", vp), ("num_vars", ci), ("label", ctypes.c_char_p), ("label_len", cl),
                 ("round_msgs", u64p), ("challenges", u64p), ("final_evals", u64p), ("root", u64p),
@@ -144,6 +152,7 @@ _SIGS = {
     "mam_merkle_tree_async": (ci, [vp, vp, csz, ci, ci]),
     "mam_ajtai_i8_setup": (ci, [ctypes.c_char_p, ci, u64, ctypes.POINTER(vp)]),
     "mam_ajtai_i8_commit": (ci, [vp, vp, ci, vp]),
+    "mam_ajtai_i8_commit_prefix": (ci, [vp, vp, ci, u64, vp]),
     "mam_ajtai_i8_matrix_row": (ci, [vp, ci, u64, u64, ctypes.POINTER(ctypes.c_int8)]),
     "mam_ajtai_i8_free": (ci, [vp]),
     # almost-Goldilocks crate (morpheus_agl.h)
@@ -178,6 +187,9 @@ _SIGS = {
     "ajtai_commit_sparse_ffi": (ci, [vp, vp, u64, ci, vp]),
     "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
     "mam_prove_batch": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci]),
+    "mam_matmul_prove": (ci, [ctypes.POINTER(MatmulInstance_t), vp, ci, ci, ci, ci, vp, ci]),
+    "fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
+    "agl_fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "mam_profile_enable": (ci, [ci]),
     "mam_valu_probe": (ci, [ci, ci, ci, ctypes.POINTER(ctypes.c_double)]),
     "mam_release_scratch": (ci, []),

@@ -39,6 +39,14 @@ class AjtaiI8:
         self.commit_device(dz, batch, do)
         return do.to_numpy().reshape(batch, self.rows)
 
+    def commit_prefix(self, z, n_used, batch=1):
+        """Commit batch witnesses of n_used words against the first n_used columns."""
+        z = np.ascontiguousarray(z, dtype=np.uint64)
+        assert z.size == batch * n_used
+        dz, do = DeviceBuffer.from_numpy(z), DeviceBuffer(batch * self.rows)
+        check(lib.mam_ajtai_i8_commit_prefix(self.h, dz.ptr, batch, n_used, do.ptr), "mam_ajtai_i8_commit_prefix")
+        return do.to_numpy().reshape(batch, self.rows)
+
     def matrix_row(self, row, col0, count):
         out = np.zeros(count, dtype=np.int8)
         check(lib.mam_ajtai_i8_matrix_row(self.h, row, col0, count, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))),

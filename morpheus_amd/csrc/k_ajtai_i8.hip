@@ -97,8 +97,8 @@ static constexpr int AJF_THREADS = 512;
 
 template <int RTW, int CTW, int CT, int DEPTH>
 __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__ mt, const u64 *__restrict__ z,
-                                                               int RT, int batch, int WR, size_t n, size_t kchunk,
-                                                               int *__restrict__ partial) {
+                                                               int RT, int batch, int ones, int WR, size_t n,
+                                                               size_t kchunk, int *__restrict__ partial) {
     constexpr int TILES = 16 + CT;
     constexpr int A_ITEMS = DEPTH * 16 * 64 / AJF_THREADS;  // v4i per thread (full RT)
     __shared__ v4i lds[2][DEPTH][TILES][64];
@@ -106,7 +106,7 @@ __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__
     const int wr = wave % WR, wc = wave / WR;
     const bool computes = wave < WR * (CT / CTW);
     const int lc = lane & 15, q = lane >> 4;
-    const int CTT = (8 * batch + 15) / 16;
+    const int CTT = (8 * batch + ones + 15) / 16;
     const size_t steps = n >> 6;
     const size_t kt0 = (size_t)blockIdx.x * (kchunk >> 6);
     size_t kt1 = kt0 + (kchunk >> 6);
@@ -116,7 +116,8 @@ __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__
     v4i ra[A_ITEMS];
     u64 rz[16];
     const int a_per_step = RT * 64;
-    const int b_items = DEPTH * batch * 4;
+    const int wb = batch + ones;  // witnesses, plus the all-ones plane when `ones`
+    const int b_items = DEPTH * wb * 4;
     auto stage_load = [&](size_t kt) {
 #pragma unroll
         for (int m = 0; m < A_ITEMS; m++) {
@@ -127,8 +128,8 @@ __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__
         }
         const int e = threadIdx.x;
         if (e < b_items) {
-            const int d = e / (batch * 4), r = e % (batch * 4);
-            if (kt + d < kt1) {
+            const int d = e / (wb * 4), r = e % (wb * 4);
+            if (kt + d < kt1 && (r >> 2) < batch) {
                 const u64 *src = z + (size_t)(r >> 2) * n + (kt + d) * 64 + (r & 3) * 16;
 #pragma unroll
                 for (int k = 0; k < 16; k++) rz[k] = src[k];
@@ -144,8 +145,12 @@ __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__
         }
         const int e = threadIdx.x;
         if (e < b_items) {
-            const int d = e / (batch * 4), r = e % (batch * 4);
-            if (kt + d < kt1) {
+            const int d = e / (wb * 4), r = e % (wb * 4);
+            if (kt + d < kt1 && (r >> 2) == batch) {
+                // the all-ones plane: its GEMM column is the row sum over the committed columns
+                const int p = 8 * batch, g = r & 3;
+                lds[buf][d][16 + (p >> 4)][(p & 15) + 16 * g] = v4i{0x01010101, 0x01010101, 0x01010101, 0x01010101};
+            } else if (kt + d < kt1) {
                 const int w = r >> 2, g = r & 3;
 #pragma unroll
                 for (int k = 0; k < 16; k++) rz[k] = gl_canon(rz[k]);
@@ -216,9 +221,11 @@ __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__
 
 // out[w * rows + i] = sum_l 2^(8l) (S'_{i,(w,l)} + 128 R_i) mod p. One workgroup of 256
 // threads per (i, w): thread (slice s, limb l) sums chunks s, s + 32, ...; LDS reduce.
+// With ones_col >= 0 the row sums come from that partial column (the all-ones plane over the
+// committed columns) instead of the set-up's full-width row sums.
 __global__ __launch_bounds__(256) void k_aj_finish(const int *__restrict__ partial,
                                                    const unsigned long long *__restrict__ rowsum, int rows, int cols,
-                                                   int chunks, u64 *__restrict__ out) {
+                                                   int chunks, int ones_col, u64 *__restrict__ out) {
     __shared__ long long red[256];
     const int i = blockIdx.x % rows, w = blockIdx.x / rows;
     const int l = threadIdx.x & 7, sl = threadIdx.x >> 3;
@@ -231,7 +238,12 @@ __global__ __launch_bounds__(256) void k_aj_finish(const int *__restrict__ parti
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const long long R = (long long)rowsum[i];
+        long long R = 0;
+        if (ones_col >= 0) {
+            for (int c = 0; c < chunks; c++) R += partial[((size_t)c * rows + i) * cols + ones_col];
+        } else {
+            R = (long long)rowsum[i];
+        }
         u64 acc = 0;
         for (int k = 0; k < 8; k++) {
             const long long v = red[k] + 128 * R;  // |v| < 2^40 < p
@@ -267,13 +279,14 @@ int aj_reserve(void **buf, size_t *have, size_t need, hipStream_t s) {
 }
 
 template <int RTW, int CTW, int CT, int DEPTH>
-void aj_launch(const AjtaiI8 *A, const u64 *d_z, int batch, size_t kchunk, unsigned grid, hipStream_t s) {
-    static_assert(DEPTH * 32 * 4 <= AJF_THREADS, "one witness group per staging thread");
+void aj_launch(const AjtaiI8 *A, const u64 *d_z, int batch, int ones, size_t n_used, size_t kchunk, unsigned grid,
+               hipStream_t s) {
+    static_assert(DEPTH * 33 * 4 <= AJF_THREADS, "one witness group per staging thread");
     static_assert((16 / RTW) * (CT / CTW) <= AJF_THREADS / 64, "wave blocks fit the workgroup");
     const int RT = A->rows / 16;
     const int WR = (RT + RTW - 1) / RTW;
-    k_aj_gemm<RTW, CTW, CT, DEPTH><<<grid, AJF_THREADS, 0, s>>>((const v4i *)A->mt, d_z, RT, batch, WR, A->n,
-                                                                     kchunk, A->partial);
+    k_aj_gemm<RTW, CTW, CT, DEPTH><<<grid, AJF_THREADS, 0, s>>>((const v4i *)A->mt, d_z, RT, batch, ones, WR,
+                                                                     n_used, kchunk, A->partial);
 }
 }  // namespace
 
@@ -316,34 +329,50 @@ MAM_API int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void 
     return 0;
 }
 
-// d_z: batch witnesses of n words back to back (device); d_out: batch * rows words (device).
-MAM_API int mam_ajtai_i8_commit(void *handle, const u64 *d_z, int batch, u64 *d_out) {
-    AjtaiI8 *A = (AjtaiI8 *)handle;
+// Commit `batch` witnesses of n_used words each (back to back, device) against the first
+// n_used matrix columns; d_out: batch * rows words (device). n_used = n uses the set-up's row
+// sums; a shorter prefix (a multiple of 64, batch <= 16) adds an all-ones plane to the GEMM,
+// whose column is the prefix row sum (the matrix entries depend only on (key, row, column),
+// so a prefix commitment equals a commitment with a set-up of n_used columns).
+static int aj_commit(AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, u64 *d_out) {
     if (!A || !d_z || !d_out || batch < 1 || batch > AJ_MAX_BATCH) return -2;
+    if (n_used == 0 || n_used % 64 || n_used > A->n) return -2;
+    const int ones = n_used != A->n ? 1 : 0;
+    if (ones && batch > 16) return -2;
     hipStream_t s = mam_stream();
-    const size_t n = A->n;
-    const int nplanes = 8 * batch;
+    const int nplanes = 8 * batch + ones;
     const int CTT = (nplanes + 15) / 16;
     const int CT = CTT <= 1 ? 1 : CTT <= 2 ? 2 : CTT <= 4 ? 4 : CTT <= 8 ? 8 : 16;
     // 2^13 columns per workgroup (512 workgroups at n = 2^22) while the partials stay small,
     // 2^14 for the widest batches; both keep chunk sums below 2^28.
     const size_t kchunk = CT <= 4 ? ((size_t)1 << 13) : ((size_t)1 << 14);
-    const size_t chunks = (n + kchunk - 1) / kchunk;
+    const size_t chunks = (n_used + kchunk - 1) / kchunk;
     if (aj_reserve((void **)&A->partial, &A->partial_bytes, chunks * A->rows * CTT * 16 * sizeof(int), s))
         return -1;
     {
-        ProfScope prof(PROF_AJTAI_GEMM, (double)A->rows * n + (double)nplanes * n, s);
+        ProfScope prof(PROF_AJTAI_GEMM, (double)A->rows * n_used + 8.0 * batch * n_used, s);
         switch (CT) {
-            case 1: aj_launch<2, 1, 1, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
-            case 2: aj_launch<2, 2, 2, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
-            case 4: aj_launch<4, 2, 4, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
-            case 8: aj_launch<4, 4, 8, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
-            default: aj_launch<4, 8, 16, 2>(A, d_z, batch, kchunk, (unsigned)chunks, s); break;
+            case 1: aj_launch<2, 1, 1, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
+            case 2: aj_launch<2, 2, 2, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
+            case 4: aj_launch<4, 2, 4, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
+            case 8: aj_launch<4, 4, 8, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
+            default: aj_launch<4, 8, 16, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
         }
     }
     if (hipGetLastError() != hipSuccess) return -1;
-    k_aj_finish<<<batch * A->rows, 256, 0, s>>>(A->partial, A->rowsum, A->rows, CTT * 16, (int)chunks, d_out);
+    k_aj_finish<<<batch * A->rows, 256, 0, s>>>(A->partial, A->rowsum, A->rows, CTT * 16, (int)chunks,
+                                               ones ? 8 * batch : -1, d_out);
     return mam_sync_status();
+}
+
+// d_z: batch witnesses of n words back to back (device); d_out: batch * rows words (device).
+MAM_API int mam_ajtai_i8_commit(void *handle, const u64 *d_z, int batch, u64 *d_out) {
+    AjtaiI8 *A = (AjtaiI8 *)handle;
+    return A ? aj_commit(A, d_z, batch, A->n, d_out) : -2;
+}
+
+MAM_API int mam_ajtai_i8_commit_prefix(void *handle, const u64 *d_z, int batch, uint64_t n_used, u64 *d_out) {
+    return aj_commit((AjtaiI8 *)handle, d_z, batch, (size_t)n_used, d_out);
 }
 
 // Copy `count` matrix entries of one row starting at column col0 to host (int8).

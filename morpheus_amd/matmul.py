@@ -1,0 +1,72 @@
+"""Matmul proof C = A B (include/morpheus_prover.h, mam_matmul_prove): Basefold commitments
+of A, B, C, an optional int8 Ajtai commitment of C, the claim C(r), one degree-3 sumcheck over
+the inner index, and Basefold openings of A, B and C on one transcript."""
+import ctypes
+
+import numpy as np
+
+from ._lib import MatmulInstance_t, lib
+from .device import DeviceBuffer, check
+from .prover import proof_len
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+P = 0xFFFFFFFF00000001
+
+
+def lift_int(x):
+    """Signed integers -> canonical Goldilocks words."""
+    x = np.asarray(x, dtype=np.int64)
+    return np.where(x < 0, np.uint64(P) - (-x).astype(np.uint64), x.astype(np.uint64)).astype(np.uint64)
+
+
+class MatmulJob:
+    """One statement with A, B, C resident on the device (canonical words, row-major)."""
+
+    def __init__(self, lm, lk, ln, d_a, d_b, d_c, label):
+        self.lm, self.lk, self.ln = lm, lk, ln
+        self.d_a, self.d_b, self.d_c = d_a, d_b, d_c
+        self.label = bytes(label)
+
+    @classmethod
+    def from_int8(cls, a_i8, b_i8, label):
+        """A (m x k) and B (k x n) int8 arrays; C = A B computed exactly on the host."""
+        a = np.asarray(a_i8, dtype=np.int64)
+        b = np.asarray(b_i8, dtype=np.int64)
+        c = a @ b
+        lm, lk = (a.shape[0] - 1).bit_length(), (a.shape[1] - 1).bit_length()
+        ln = (b.shape[1] - 1).bit_length()
+        return cls(lm, lk, ln, DeviceBuffer.from_numpy(lift_int(a).ravel()),
+                   DeviceBuffer.from_numpy(lift_int(b).ravel()), DeviceBuffer.from_numpy(lift_int(c).ravel()), label)
+
+    def outputs(self, log_rate, num_queries, ajtai_rows):
+        lm, lk, ln = self.lm, self.lk, self.ln
+        o = {"roots": np.zeros((3, 4), dtype=np.uint64), "point": np.zeros((ln + lm, 2), dtype=np.uint64),
+             "claim": np.zeros(2, dtype=np.uint64), "round_msgs": np.zeros((lk, 4, 2), dtype=np.uint64),
+             "challenges": np.zeros((lk, 2), dtype=np.uint64), "final_evals": np.zeros((3, 2), dtype=np.uint64),
+             "proof_a": np.zeros(proof_len(lm + lk, log_rate, num_queries), dtype=np.uint64),
+             "proof_b": np.zeros(proof_len(lk + ln, log_rate, num_queries), dtype=np.uint64),
+             "proof_c": np.zeros(proof_len(lm + ln, log_rate, num_queries), dtype=np.uint64)}
+        if ajtai_rows:
+            o["ajtai"] = np.zeros(ajtai_rows, dtype=np.uint64)
+        return o
+
+    def fill(self, s, o):
+        s.d_a, s.d_b, s.d_c = self.d_a.ptr, self.d_b.ptr, self.d_c.ptr
+        s.lm, s.lk, s.ln = self.lm, self.lk, self.ln
+        s.label, s.label_len = self.label, len(self.label)
+        for k in ("roots", "point", "claim", "round_msgs", "challenges", "final_evals", "proof_a", "proof_b",
+                  "proof_c"):
+            setattr(s, k, o[k].ctypes.data_as(_u64p))
+        s.ajtai = o["ajtai"].ctypes.data_as(_u64p) if "ajtai" in o else None
+
+
+def prove_matmul(job, table, log_rate=1, hash_kind=0, num_queries=32, ajtai=None):
+    """Returns the proof fields as numpy arrays. ajtai: an AjtaiI8 set-up with at least |C|
+    columns (or None to skip the Ajtai commitment)."""
+    rows = ajtai.rows if ajtai is not None else 0
+    o = job.outputs(log_rate, num_queries, rows)
+    s = MatmulInstance_t()
+    job.fill(s, o)
+    check(lib.mam_matmul_prove(ctypes.byref(s), table.upload().ptr, table.num_vars, log_rate, hash_kind, num_queries,
+                               ajtai.h if ajtai is not None else None, rows), "mam_matmul_prove")
+    return o

@@ -139,6 +139,10 @@ void or_ring_commit_binary(const uint32_t *key8, const uint64_t *z, uint64_t N, 
 void or_ring_commit_wide(const uint32_t *key8, const uint64_t *z_wide, uint64_t N, uint64_t col_offset,
                          uint64_t *out);
 void or_ring_commit_sparse(const uint32_t *key8, const uint64_t *pos, uint64_t K, uint64_t *out);
+/* fused permute + partial eval over modulus p or q (pinned by: identity LUTs reduce it to
+ * the partial evaluation, which is pinned by direct MLE sums) */
+void or_fused_permute_peval(uint64_t mod, const uint64_t *evals, const uint64_t *eq, const uint32_t *lo,
+                            const uint32_t *hi, int n, int m, int half, long out_size, uint64_t *out);
 
 #ifdef __cplusplus
 }

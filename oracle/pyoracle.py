@@ -63,6 +63,8 @@ _sigs = {
     "or_ring_commit_binary": (None, [ctypes.POINTER(ctypes.c_uint32), u64p, u64, ci, u64, u64p]),
     "or_ring_commit_wide": (None, [ctypes.POINTER(ctypes.c_uint32), u64p, u64, u64, u64p]),
     "or_ring_commit_sparse": (None, [ctypes.POINTER(ctypes.c_uint32), u64p, u64, u64p]),
+    "or_fused_permute_peval": (None, [u64, u64p, u64p, ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32), ci, ci, ci, cl, u64p]),
 }
 for _n, (_r, _a) in _sigs.items():
     _f = getattr(L, _n)
@@ -285,3 +287,22 @@ def ring_commit_sparse(key32, positions):
     out = np.zeros(KAPPA * 64, dtype=np.uint64)
     L.or_ring_commit_sparse(_u32p(kw), _p(pos), len(pos), _p(out))
     return out.reshape(KAPPA, 64)
+
+
+def split_luts(n, new_to_old, half):
+    """LUTs of a variable permutation: bit p of the permuted index is bit new_to_old[p] of the
+    stored index (goldilocks-cuda-rs/src/partial_eval.rs:266-321 builds the same tables)."""
+    lo = np.zeros(1 << half, dtype=np.uint32)
+    hi = np.zeros(1 << (n - half), dtype=np.uint32)
+    for x in range(1 << half):
+        lo[x] = sum(((x >> p) & 1) << new_to_old[p] for p in range(half))
+    for y in range(1 << (n - half)):
+        hi[y] = sum(((y >> p) & 1) << new_to_old[half + p] for p in range(n - half))
+    return lo, hi
+
+
+def fused_permute_peval(mod, evals, eq, lo, hi, n, m, half, out_size):
+    evals, eq = arr(evals), arr(eq)
+    out = np.zeros(2 * out_size, dtype=np.uint64)
+    L.or_fused_permute_peval(mod, _p(evals), _p(eq), _u32p(lo), _u32p(hi), n, m, half, out_size, _p(out))
+    return out

@@ -243,3 +243,23 @@ void or_ring_commit_sparse(const uint32_t *key8, const uint64_t *pos, uint64_t K
         }
     }
 }
+
+/* fused permute + partial evaluation (cuda/fused_permute_peval.cuh:31-120) over the prime
+ * `mod` (p or q): out[j] = sum_b evals[lo[x & (2^half-1)] | hi[x >> half]] * eq[b], x = j 2^m + b */
+void or_fused_permute_peval(uint64_t mod, const uint64_t *evals, const uint64_t *eq, const uint32_t *lo,
+                            const uint32_t *hi, int n, int m, int half, long out_size, uint64_t *out) {
+    uint64_t lo_mask = (1ULL << half) - 1;
+    for (long j = 0; j < out_size; j++) {
+        u128 a0 = 0, a1 = 0;
+        for (uint64_t b = 0; b < (1ULL << m); b++) {
+            uint64_t x = ((uint64_t)j << m) + b;
+            uint64_t old = (uint64_t)(lo[x & lo_mask] | hi[x >> half]);
+            uint64_t v = evals[old] % mod;
+            a0 = (a0 + (u128)v * (eq[2 * b] % mod)) % mod;
+            a1 = (a1 + (u128)v * (eq[2 * b + 1] % mod)) % mod;
+        }
+        out[2 * j] = (uint64_t)a0;
+        out[2 * j + 1] = (uint64_t)a1;
+    }
+    (void)n;
+}

@@ -47,3 +47,17 @@ def test_ajtai_i8_extremes_and_linearity():
     ca, cb, cs = A.commit(a)[0], A.commit(b)[0], A.commit(s)[0]
     assert [int(v) for v in cs] == [(int(x) + int(y)) % P for x, y in zip(ca, cb)]
     A.free()
+
+
+@pytest.mark.parametrize("rows,n,n_used,batch", [(256, 1 << 14, 1 << 12, 1), (64, 8192, 64, 3), (32, 4096, 4032, 16),
+                                                 (256, 1 << 14, 1 << 14, 2)])
+def test_ajtai_i8_prefix_commit(rows, n, n_used, batch):
+    print("ajtai prefix", rows, n, n_used, batch, flush=True)
+    key = key_from_seed(3 + rows)
+    rng = np.random.default_rng(n_used + batch)
+    z = rng.integers(0, 1 << 64, batch * n_used, dtype=np.uint64)
+    A = AjtaiI8(key, rows, n)
+    got = A.commit_prefix(z, n_used, batch)
+    want = O.ajtai_i8_commit(key, rows, n_used, z, batch).reshape(batch, rows)
+    assert np.array_equal(got, want)
+    A.free()

@@ -167,3 +167,22 @@ def test_ring_commit_forms_agree_and_are_linear():
     padded = np.concatenate([np.zeros(3, dtype=np.uint64), z1])
     assert np.array_equal(O.ring_commit_binary(KEY, z1, N, 1, col_offset=3)[0],
                           O.ring_commit_binary(KEY, padded, N + 3, 1)[0])
+
+
+def test_fused_permute_peval_reduces_to_partial_eval():
+    # identity permutation: the fused kernel's oracle equals the pinned partial evaluation
+    rng = np.random.default_rng(6)
+    n, m = 7, 3
+    f = rng.integers(0, 1 << 64, 1 << n, dtype=np.uint64)
+    r = rng.integers(0, 1 << 64, 2 * m, dtype=np.uint64)
+    eq = O.q_eq_table(r, m, ext=True)
+    lo, hi = O.split_luts(n, list(range(n)), n // 2)
+    got = O.fused_permute_peval(Q, f, eq, lo, hi, n, m, n // 2, 1 << (n - m))
+    assert np.array_equal(got, O.q_partial_eval(f, 0, n, r, 1, m))
+    # a permutation moves variables: folding permuted variables = folding a permuted table
+    perm = [4, 0, 6, 1, 3, 5, 2]  # bit p of the new index is bit perm[p] of the stored index
+    lo, hi = O.split_luts(n, perm, 3)
+    permuted = np.array([f[sum(((x >> p) & 1) << perm[p] for p in range(n))] for x in range(1 << n)],
+                        dtype=np.uint64)
+    got = O.fused_permute_peval(Q, f, eq, lo, hi, n, m, 3, 1 << (n - m))
+    assert np.array_equal(got, O.q_partial_eval(permuted, 0, n, r, 1, m))
