@@ -76,11 +76,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch", "hash"],
+    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch", "hash", "serve"],
                     default="basefold")
     ap.add_argument("--instances", type=int, default=64, help="instances per batch (batch workload)")
     ap.add_argument("--workers", type=int, default=0, help="batch workers (0: autotune)")
     ap.add_argument("--batch", type=int, default=1, help="witnesses per Ajtai commit")
+    ap.add_argument("--rate", type=float, default=0.0,
+                    help="serve: mean arrival rate, proofs/s (0: 1.25 x the one-at-a-time capacity)")
+    ap.add_argument("--hbm-budget-gib", type=int, default=96, help="serve: admission budget")
     ap.add_argument("--nv", type=int, default=None)
     ap.add_argument("--queries", type=int, default=100)
     ap.add_argument("--cpu-sample-nv", type=int, default=None)
@@ -458,6 +461,116 @@ def bench_batch(args, M):
                          "traffic": None, "perms": perms, "valu_ops_per_perm": VALU_OPS_PER_PERM}}
 
 
+SERVE_DIMS = (256, 512, 1024, 2048, 4096)
+
+
+def serve_trace(count, seed=8):
+    """m, k, n of each request, uniform over SERVE_DIMS (seeded)."""
+    return np.random.default_rng(seed).choice(SERVE_DIMS, (count, 3))
+
+
+def bench_serve(args, M):
+    """Matmul proving service: requests C = A B with m, k, n uniform over {256, ..., 4096},
+    int8 A, B lifted into Goldilocks, C resident with them in HBM; each proof is the Basefold
+    commits of A, B, C, an int8 Ajtai commit of C (256 rows), the degree-3 sumcheck over k and
+    three openings. Poisson arrivals at a fixed mean rate; continuous batching on W worker
+    streams with HBM-footprint admission, against the same trace proved one at a time."""
+    from morpheus_amd.ajtai import AjtaiI8, key_from_seed
+    from morpheus_amd.matmul import MatmulJob, serve
+    lr, nq = 1, args.queries
+    dims = serve_trace(args.instances)
+    rng = np.random.default_rng(9)
+    jobs, elems, perms = [], 0, 0
+    for i, (m, k, n) in enumerate(dims):
+        a = rng.integers(-128, 128, (m, k), dtype=np.int8)
+        b = rng.integers(-128, 128, (k, n), dtype=np.int8)
+        jobs.append(MatmulJob.from_int8(a, b, b"c5-%d" % i))
+        lm, lk, ln = (int(x).bit_length() - 1 for x in (m, k, n))
+        elems += int(m * k + k * n + m * n)
+        perms += sum(merkle_perms(nv, lr) for nv in (lm + lk, lk + ln, lm + ln))
+    max_nv = int(max(int(m * n).bit_length() - 1 for m, _, n in dims))
+    tnv = int(max(max(int(m * k).bit_length(), int(k * n).bit_length(), int(m * n).bit_length()) - 1
+                  for m, k, n in dims))
+    table = M.BasefoldTable.generate(tnv, lr, tnv, 42)
+    table.upload()
+    aj = AjtaiI8(key_from_seed(2024), 256, 1 << max_nv)
+    budget = args.hbm_budget_gib << 30
+
+    def run(w, arrivals, sub=None):
+        js = jobs if sub is None else [jobs[i] for i in sub]
+        arr = arrivals if sub is None else arrivals[sub]
+        t0 = time.perf_counter()
+        _, admit, done = serve(js, arr, table, lr, 0, nq, aj, w, budget)
+        return time.perf_counter() - t0, admit, done
+
+    zero = np.zeros(len(jobs))
+    # worker count autotuned on every fourth request, all arriving at once (untimed)
+    cands = [args.workers] if args.workers else [2, 4, 8]
+    sub = np.arange(0, len(jobs), 4)
+    run(cands[-1], zero, sub)
+    tune = {}
+    for w in cands:
+        tune[w] = min(run(w, zero, sub)[0] for _ in range(2))
+    best = min(tune, key=tune.get)
+    run(1, zero)
+    # saturation: the whole trace at t = 0
+    sat_seq = min(run(1, zero)[0] for _ in range(max(args.warmup, 1)))
+    sat_srv = min(run(best, zero)[0] for _ in range(max(args.warmup, 1)))
+    rate = args.rate or 1.25 * len(jobs) / sat_seq
+    arrivals = np.cumsum(np.random.default_rng(10).exponential(1e3 / rate, len(jobs)))
+    arrivals -= arrivals[0]
+    res = {}
+    for name, w in (("sequential", 1), ("served", best)):
+        spans, lat = [], []
+        for _ in range(args.steps):
+            dt, admit, done = run(w, arrivals)
+            spans.append(float(done.max()))
+            lat += list(done - arrivals)
+        span = float(np.mean(spans)) / 1e3
+        res[name] = {"value": elems / span, "proofs_per_s": len(jobs) / span, "ms_per_trace": span * 1e3,
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+    srv = res["served"]
+    ach = perms * VALU_OPS_PER_PERM / (srv["ms_per_trace"] / 1e3) / 1e12
+    return {"metric": METRIC, "value": srv["value"], "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": srv["ms_per_trace"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64 (Goldilocks, Ext2 challenges)", "data": "synthetic",
+            "config": {"workload": "matmul_proving_service", "requests": len(jobs),
+                       "dims": "m, k, n uniform over {256, 512, 1024, 2048, 4096} (seeded)",
+                       "entries": "int8 uniform lifted to p", "proved_elems_per_trace": elems,
+                       "arrivals": f"Poisson, mean rate {rate:.1f} proofs/s (1.25 x one-at-a-time capacity)"
+                       if not args.rate else f"Poisson, mean rate {rate:.1f} proofs/s",
+                       "ajtai": "int8, 256 rows, prefix of a 2^%d-column set-up" % max_nv, "log_rate": lr,
+                       "num_queries": nq, "merkle_hash": "monolith-w12", "workers": best,
+                       "hbm_budget_gib": args.hbm_budget_gib, "autotune_s": {str(k): v for k, v in tune.items()},
+                       "parallelism": "single-gpu"},
+            "proofs_per_s": srv["proofs_per_s"], "p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"],
+            "sequential": res["sequential"],
+            "saturated": {"sequential_proofs_per_s": len(jobs) / sat_seq, "served_proofs_per_s": len(jobs) / sat_srv,
+                          "speedup": sat_seq / sat_srv},
+            "roofline": {"kernel": "all Monolith Merkle work of the trace over its span", "bound": "valu",
+                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
+                         "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS, "traffic": None, "perms": perms,
+                         "valu_ops_per_perm": VALU_OPS_PER_PERM}}
+
+
+def cpu_baseline_serve(lg):
+    """The oracle proving one matmul request (m = k = n = 2^lg) end to end."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    from morpheus_amd.ajtai import key_from_seed
+    rng = np.random.default_rng(9)
+    n = 1 << lg
+    a = rng.integers(-128, 128, (n, n))
+    b = rng.integers(-128, 128, (n, n))
+    table = O.basefold_table(2 * lg, 1, 2 * lg, 42)
+    t0 = time.perf_counter()
+    O.matmul_prove(a, b, b"c5-cpu", table, 2 * lg, 1, 100, key_from_seed(2024), 256)
+    dt = time.perf_counter() - t0
+    return {"value": 3 * n * n / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+            "sample": f"oracle matmul proof, m = k = n = {n} (A, B, C Basefold commit + open, int8 Ajtai of C, "
+                      f"sumcheck), single thread, {dt:.2f} s ({1 / dt:.3f} proofs/s at this size)"}
+
+
 RING_KEY = bytes((7 * i + 3) % 256 for i in range(32))
 
 
@@ -543,6 +656,8 @@ def main():
         out = bench_batch(args, M)
     elif args.workload == "hash":
         out = bench_hash(args, M)
+    elif args.workload == "serve":
+        out = bench_serve(args, M)
     else:
         out = bench_ring_ajtai(args, M)
     if not args.no_cpu:
@@ -556,6 +671,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline_basefold(args.cpu_sample_nv or 18, 1, args.queries)
         elif args.workload == "hash":
             out["cpu_baseline"] = cpu_baseline_hash(args.cpu_sample_nv or 18)
+        elif args.workload == "serve":
+            out["cpu_baseline"] = cpu_baseline_serve(args.cpu_sample_nv or 8)
         else:
             out["cpu_baseline"] = cpu_baseline_ring_ajtai(args.cpu_sample_nv or 13)
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()

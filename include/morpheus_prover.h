@@ -121,7 +121,9 @@ int mam_prove_batch(mam_batch_instance *inst, int n, const uint64_t *d_table, in
  * "ij,jk->ik": Basefold commitments of A (m x k), B (k x n), C (m x n) (row-major, m = 2^lm,
  * ...), an optional int8 Ajtai commitment of C, the claim C(r) at a transcript point, one
  * sumcheck over the inner index of [A(r_i, .), B(., r_j), 1], and Basefold openings of A, B, C
- * at their points, all on T = Transcript::new(label). The step list is in host_matmul.hip. */
+ * at their points, all on T = Transcript::new(label). The step list is in host_matmul.hip.
+ * The Ajtai commitment needs |C| = 2^(lm+ln) >= 64 (a multiple of 64 matrix columns);
+ * smaller C with a non-null ajtai returns -2. */
 typedef struct {
     const uint64_t *d_a, *d_b, *d_c; /* device, canonical words, row-major */
     int lm, lk, ln;
@@ -140,6 +142,21 @@ typedef struct {
 } mam_matmul_instance;
 int mam_matmul_prove(mam_matmul_instance *inst, const uint64_t *d_table, int table_nv, int log_rate, int hash,
                      int num_queries, void *ajtai_handle, int ajtai_rows);
+
+/* ---- proving service for matmul proofs (benchmark configuration 5) ----
+ * Requests are released at their arrival offsets (ms from the call), admitted in arrival
+ * order while the sum of admitted requests' device footprints fits hbm_budget (a request
+ * larger than the budget runs alone), and proved by `workers` persistent threads on their
+ * own streams. admit_ms / done_ms are filled in; latency = done_ms - arrival_ms. No
+ * reference counterpart (the reference proves layers one at a time). */
+typedef struct {
+    mam_matmul_instance *inst;
+    double arrival_ms;
+    double admit_ms, done_ms;
+    int status;
+} mam_serve_request;
+int mam_serve_matmul(mam_serve_request *req, int n, const uint64_t *d_table, int table_nv, int log_rate, int hash,
+                     int num_queries, void *ajtai_handle, int ajtai_rows, int workers, uint64_t hbm_budget);
 
 /* ---- int8-matrix Ajtai commitment on INT8 MFMA (benchmark configuration 3) ----
  * No reference function has these semantics (the reference's Ajtai is the ring-SIS commit

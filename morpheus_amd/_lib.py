@@ -42,6 +42,11 @@ class MatmulInstance_t(ctypes.Structure):
                 ("end_ms", ctypes.c_double), ("status", ci)]
 
 
+class ServeRequest_t(ctypes.Structure):
+    _fields_ = [("inst", ctypes.POINTER(MatmulInstance_t)), ("arrival_ms", ctypes.c_double),
+                ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double), ("status", ci)]
+
+
 class BatchInstance_t(ctypes.Structure):
     _fields_ = [("d_fg", vp), ("num_vars", ci), ("label", ctypes.c_char_p), ("label_len", cl),
                 ("round_msgs", u64p), ("challenges", u64p), ("final_evals", u64p), ("root", u64p),
@@ -188,6 +193,7 @@ _SIGS = {
     "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
     "mam_prove_batch": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci]),
     "mam_matmul_prove": (ci, [ctypes.POINTER(MatmulInstance_t), vp, ci, ci, ci, ci, vp, ci]),
+    "mam_serve_matmul": (ci, [ctypes.POINTER(ServeRequest_t), ci, vp, ci, ci, ci, ci, vp, ci, ci, u64]),
     "fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "agl_fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "mam_profile_enable": (ci, [ci]),

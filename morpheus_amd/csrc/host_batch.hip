@@ -19,15 +19,13 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
-#include <condition_variable>
-#include <functional>
-#include <mutex>
 #include <string.h>
 #include <thread>
 #include <vector>
 
 #include "common.h"
 #include "transcript.h"
+#include "workers.h"
 
 extern "C" {
 int mam_sumcheck_prove(mam_transcript *t, const u64 *d_polys, int num_polys, int num_vars, int input_is_base,
@@ -58,49 +56,6 @@ struct mam_batch_instance {
 };
 
 namespace {
-
-// Persistent worker threads: each keeps its stream, arenas, mailbox and commitment buffers
-// between batches (they are created once per thread and grown on demand). The pool is never
-// torn down; its threads wait on a condition variable when idle.
-class WorkerPool {
-   public:
-    // Run f on `n` pool threads and wait for all of them.
-    void run(int n, const std::function<void()> &f) {
-        std::unique_lock<std::mutex> lk(m_);
-        while ((int)count_ < n) {
-            std::thread(&WorkerPool::loop, this, (int)count_).detach();
-            count_++;
-        }
-        job_ = f;
-        active_ = n;
-        remaining_ = n;
-        gen_++;
-        cv_.notify_all();
-        done_.wait(lk, [&] { return remaining_ == 0; });
-    }
-
-   private:
-    void loop(int id) {
-        unsigned seen = 0;
-        for (;;) {
-            std::function<void()> j;
-            {
-                std::unique_lock<std::mutex> lk(m_);
-                cv_.wait(lk, [&] { return gen_ != seen && id < active_; });
-                seen = gen_;
-                j = job_;
-            }
-            j();
-            std::lock_guard<std::mutex> lk(m_);
-            if (--remaining_ == 0) done_.notify_all();
-        }
-    }
-    std::mutex m_;
-    std::condition_variable cv_, done_;
-    std::function<void()> job_;
-    unsigned gen_ = 0, count_ = 0;
-    int active_ = 0, remaining_ = 0;
-};
 
 WorkerPool &pool() {
     static WorkerPool *p = new WorkerPool();

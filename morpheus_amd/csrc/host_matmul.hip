@@ -155,8 +155,8 @@ MAM_API int mam_matmul_prove(mam_matmul_instance *I, const u64 *d_table, int tab
     if (!rc && ajtai_handle && I->ajtai) {
         if (ajtai_rows < 1 || ajtai_rows > 256) return -2;
         rc = mam_ajtai_i8_commit_prefix(ajtai_handle, I->d_c, 1, (uint64_t)1 << nvs[2], d_ajtai);
-        if (!rc) rc = hipMemcpy(I->ajtai, d_ajtai, ajtai_rows * sizeof(u64), hipMemcpyDeviceToHost) == hipSuccess
-                          ? 0 : -1;
+        if (!rc) rc = (hipMemcpyAsync(I->ajtai, d_ajtai, ajtai_rows * sizeof(u64), hipMemcpyDeviceToHost, s) ==
+                           hipSuccess && hipStreamSynchronize(s) == hipSuccess) ? 0 : -1;
         if (!rc) mam::tr_append_scalars(&t, L_AJTAI, sizeof(L_AJTAI) - 1, I->ajtai, ajtai_rows);
     }
     if (rc) return rc;

@@ -260,11 +260,18 @@ struct AjtaiI8 {
     size_t n = 0;
     u64 *mt = nullptr;                     // rows * n bytes, tiled
     unsigned long long *rowsum = nullptr;  // rows
-    int *partial = nullptr;                // grown on demand (chunks * rows * CTT * 16 ints)
-    size_t partial_bytes = 0;
 };
 
-// Grow a per-handle device buffer; the stream is drained before an old buffer is freed.
+// Chunk partials (chunks * rows * CTT * 16 ints), grown on demand. They belong to the calling
+// thread, whose commits all run on that thread's stream, so one set-up serves commits from
+// several threads at once (the proving service's workers share one set-up).
+struct AjScratch {
+    int *partial = nullptr;
+    size_t bytes = 0;
+};
+thread_local AjScratch t_aj_scratch;
+
+// Grow a device buffer; the stream is drained before an old buffer is freed.
 int aj_reserve(void **buf, size_t *have, size_t need, hipStream_t s) {
     if (need <= *have) return 0;
     if (*buf) {
@@ -280,13 +287,13 @@ int aj_reserve(void **buf, size_t *have, size_t need, hipStream_t s) {
 
 template <int RTW, int CTW, int CT, int DEPTH>
 void aj_launch(const AjtaiI8 *A, const u64 *d_z, int batch, int ones, size_t n_used, size_t kchunk, unsigned grid,
-               hipStream_t s) {
+               int *partial, hipStream_t s) {
     static_assert(DEPTH * 33 * 4 <= AJF_THREADS, "one witness group per staging thread");
     static_assert((16 / RTW) * (CT / CTW) <= AJF_THREADS / 64, "wave blocks fit the workgroup");
     const int RT = A->rows / 16;
     const int WR = (RT + RTW - 1) / RTW;
     k_aj_gemm<RTW, CTW, CT, DEPTH><<<grid, AJF_THREADS, 0, s>>>((const v4i *)A->mt, d_z, RT, batch, ones, WR,
-                                                                     n_used, kchunk, A->partial);
+                                                                     n_used, kchunk, partial);
 }
 }  // namespace
 
@@ -347,20 +354,20 @@ static int aj_commit(AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, u64 *
     // 2^14 for the widest batches; both keep chunk sums below 2^28.
     const size_t kchunk = CT <= 4 ? ((size_t)1 << 13) : ((size_t)1 << 14);
     const size_t chunks = (n_used + kchunk - 1) / kchunk;
-    if (aj_reserve((void **)&A->partial, &A->partial_bytes, chunks * A->rows * CTT * 16 * sizeof(int), s))
-        return -1;
+    AjScratch &sc = t_aj_scratch;
+    if (aj_reserve((void **)&sc.partial, &sc.bytes, chunks * A->rows * CTT * 16 * sizeof(int), s)) return -1;
     {
         ProfScope prof(PROF_AJTAI_GEMM, (double)A->rows * n_used + 8.0 * batch * n_used, s);
         switch (CT) {
-            case 1: aj_launch<2, 1, 1, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
-            case 2: aj_launch<2, 2, 2, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
-            case 4: aj_launch<4, 2, 4, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
-            case 8: aj_launch<4, 4, 8, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
-            default: aj_launch<4, 8, 16, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, s); break;
+            case 1: aj_launch<2, 1, 1, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
+            case 2: aj_launch<2, 2, 2, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
+            case 4: aj_launch<4, 2, 4, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
+            case 8: aj_launch<4, 4, 8, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
+            default: aj_launch<4, 8, 16, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
         }
     }
     if (hipGetLastError() != hipSuccess) return -1;
-    k_aj_finish<<<batch * A->rows, 256, 0, s>>>(A->partial, A->rowsum, A->rows, CTT * 16, (int)chunks,
+    k_aj_finish<<<batch * A->rows, 256, 0, s>>>(sc.partial, A->rowsum, A->rows, CTT * 16, (int)chunks,
                                                ones ? 8 * batch : -1, d_out);
     return mam_sync_status();
 }
@@ -400,7 +407,6 @@ MAM_API int mam_ajtai_i8_free(void *handle) {
     (void)hipStreamSynchronize(mam_stream());
     (void)hipFree(A->mt);
     (void)hipFree(A->rowsum);
-    (void)hipFree(A->partial);
     delete A;
     return 0;
 }

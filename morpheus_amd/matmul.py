@@ -5,7 +5,7 @@ import ctypes
 
 import numpy as np
 
-from ._lib import MatmulInstance_t, lib
+from ._lib import MatmulInstance_t, ServeRequest_t, lib
 from .device import DeviceBuffer, check
 from .prover import proof_len
 
@@ -32,7 +32,10 @@ class MatmulJob:
         """A (m x k) and B (k x n) int8 arrays; C = A B computed exactly on the host."""
         a = np.asarray(a_i8, dtype=np.int64)
         b = np.asarray(b_i8, dtype=np.int64)
-        c = a @ b
+        if a.min(initial=0) < -128 or a.max(initial=0) > 127 or b.min(initial=0) < -128 or b.max(initial=0) > 127:
+            raise ValueError("from_int8 takes entries in [-128, 127]")
+        # float64 BLAS is exact here: |c| <= 2^14 k < 2^53 for any k below 2^39
+        c = (a.astype(np.float64) @ b.astype(np.float64)).astype(np.int64)
         lm, lk = (a.shape[0] - 1).bit_length(), (a.shape[1] - 1).bit_length()
         ln = (b.shape[1] - 1).bit_length()
         return cls(lm, lk, ln, DeviceBuffer.from_numpy(lift_int(a).ravel()),
@@ -70,3 +73,26 @@ def prove_matmul(job, table, log_rate=1, hash_kind=0, num_queries=32, ajtai=None
     check(lib.mam_matmul_prove(ctypes.byref(s), table.upload().ptr, table.num_vars, log_rate, hash_kind, num_queries,
                                ajtai.h if ajtai is not None else None, rows), "mam_matmul_prove")
     return o
+
+
+def serve(jobs, arrivals_ms, table, log_rate=1, hash_kind=0, num_queries=32, ajtai=None, workers=4,
+          hbm_budget=64 << 30):
+    """Run the proving service over jobs released at arrivals_ms (offsets from the call).
+    Returns (outputs per job, admit_ms, done_ms arrays)."""
+    rows = ajtai.rows if ajtai is not None else 0
+    n = len(jobs)
+    insts = (MatmulInstance_t * max(n, 1))()
+    reqs = (ServeRequest_t * max(n, 1))()
+    outs = []
+    for i, job in enumerate(jobs):
+        o = job.outputs(log_rate, num_queries, rows)
+        outs.append(o)
+        job.fill(insts[i], o)
+        reqs[i].inst = ctypes.pointer(insts[i])
+        reqs[i].arrival_ms = float(arrivals_ms[i])
+    check(lib.mam_serve_matmul(reqs, n, table.upload().ptr, table.num_vars, log_rate, hash_kind, num_queries,
+                               ajtai.h if ajtai is not None else None, rows, workers, int(hbm_budget)),
+          "mam_serve_matmul")
+    admit = np.array([reqs[i].admit_ms for i in range(n)])
+    done = np.array([reqs[i].done_ms for i in range(n)])
+    return outs, admit, done

@@ -45,9 +45,9 @@ static void key_words(const uint8_t *key32, uint32_t *kw) {
 void or_ajtai_i8_row(const uint8_t *key32, uint32_t row, uint64_t n, int8_t *out) {
     uint32_t kw[8], w[16], nonce[3] = {row, 0, 0};
     key_words(key32, kw);
-    for (uint64_t b = 0; b < n / 64; b++) {
+    for (uint64_t b = 0; 64 * b < n; b++) {
         or_chacha_block(8, kw, (uint32_t)b, nonce, w);
-        for (int t = 0; t < 64; t++) out[64 * b + t] = (int8_t)(w[t / 4] >> (8 * (t % 4)));
+        for (uint64_t t = 0; t < 64 && 64 * b + t < n; t++) out[64 * b + t] = (int8_t)(w[t / 4] >> (8 * (t % 4)));
     }
 }
 

@@ -306,3 +306,60 @@ def fused_permute_peval(mod, evals, eq, lo, hi, n, m, half, out_size):
     out = np.zeros(2 * out_size, dtype=np.uint64)
     L.or_fused_permute_peval(mod, _p(evals), _p(eq), _u32p(lo), _u32p(hi), n, m, half, out_size, _p(out))
     return out
+
+
+def lift_int(x):
+    """Signed integers -> canonical Goldilocks words."""
+    x = np.asarray(x, dtype=np.int64)
+    return np.where(x < 0, np.uint64(P) - (-x).astype(np.uint64), x.astype(np.uint64)).astype(np.uint64)
+
+
+def _ext(words):
+    e = np.zeros((words.size, 2), dtype=np.uint64)
+    e[:, 0] = words
+    return e
+
+
+def matmul_prove(a, b, label, table_entries, table_nv, lr, nq, key=None, rows=0, hash_kind=MONOLITH):
+    """Matmul proof C = A B for integer A (m x k), B (k x n), powers of two, restated step by
+    step (zk-torch-3/src/basicblock/einsum.rs:807-1098 is the statement: partial-evaluate A and
+    B at the output point, then a sumcheck over the inner index with polys [A', B', ones]).
+    Variables are LSB-first over the row-major index: C(j, i) with j the column bits first.
+    Transcript: roots of A, B, C ("matmul-root"), optional int8 Ajtai of C ("matmul-ajtai"),
+    point r = (r_j, r_i) ("matmul-point"), claim C(r) ("matmul-claim"), the degree-3 sumcheck
+    over l, then Basefold openings of A at (r_l, r_i), B at (r_j, r_l), C at r."""
+    a = np.asarray(a, dtype=np.int64)
+    b = np.asarray(b, dtype=np.int64)
+    c = (a.astype(np.float64) @ b.astype(np.float64)).astype(np.int64)  # exact: |c| < 2^53
+    m_, k_ = a.shape
+    n_ = b.shape[1]
+    lm, lk, ln = (m_ - 1).bit_length(), (k_ - 1).bit_length(), (n_ - 1).bit_length()
+    A, B, C = lift_int(a).ravel(), lift_int(b).ravel(), lift_int(c).ravel()
+    nvs = (lm + lk, lk + ln, lm + ln)
+    t = Transcript(label)
+    com = []
+    for X, nv in zip((A, B, C), nvs):
+        cw, bh, tree = basefold_commit(X, nv, lr, hash_kind)
+        com.append((cw, bh, tree[-4:]))
+        t.append_scalars(b"matmul-root", tree[-4:])
+    out = {"roots": np.array([x[2] for x in com])}
+    if key is not None:
+        aj = ajtai_i8_commit(key, rows, 1 << nvs[2], C, 1)
+        t.append_scalars(b"matmul-ajtai", aj)
+        out["ajtai"] = aj
+    point = np.array([t.challenge_ext2(b"matmul-point") for _ in range(ln + lm)], dtype=np.uint64)
+    claim = partial_eval_ext2(_ext(C), nvs[2], point.ravel(), nvs[2]).ravel()[:2]
+    t.append_ext2(b"matmul-claim", claim)
+    r_j, r_i = point[:ln], point[ln:]
+    a_perm = A.reshape(m_, k_).T.ravel()  # row variables first
+    a1 = partial_eval_ext2(_ext(a_perm), nvs[0], r_i.ravel(), lm).reshape(-1, 2)
+    b1 = partial_eval_ext2(_ext(B), nvs[1], r_j.ravel(), ln).reshape(-1, 2)
+    ones = np.zeros((k_, 2), dtype=np.uint64)
+    ones[:, 0] = 1
+    sc = sumcheck_prove_ext2(t, np.stack([a1, b1, ones]), 3, lk)
+    r_l = sc["challenges"]
+    pts = (np.concatenate([r_l, r_i]), np.concatenate([r_j, r_l]), point)
+    for name, (cw, bh, root), nv, pt in zip(("proof_a", "proof_b", "proof_c"), com, nvs, pts):
+        out[name] = basefold_open_ext2(cw, bh, root, nv, lr, hash_kind, pt.ravel(), table_entries, table_nv, t, nq)
+    out.update(point=point, claim=claim, round_msgs=sc["round_msgs"], challenges=r_l, final_evals=sc["final_evals"])
+    return out

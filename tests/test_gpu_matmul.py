@@ -7,7 +7,7 @@ import pyoracle as O
 
 import morpheus_amd as M
 from morpheus_amd.ajtai import AjtaiI8, key_from_seed
-from morpheus_amd.matmul import MatmulJob, lift_int, prove_matmul
+from morpheus_amd.matmul import MatmulJob, prove_matmul, serve
 
 pytestmark = pytest.mark.gpu
 P = O.P
@@ -46,46 +46,8 @@ def test_fused_permute_peval(field, n, m, half, perm_seed):
     assert np.array_equal(d_out.to_numpy(), want)
 
 
-def ext(words):
-    e = np.zeros((words.size, 2), dtype=np.uint64)
-    e[:, 0] = words
-    return e
-
-
 def oracle_matmul(a, b, label, table, lr, nq, key=None, rows=0):
-    c = a @ b
-    m_, k_ = a.shape
-    n_ = b.shape[1]
-    lm, lk, ln = (m_ - 1).bit_length(), (k_ - 1).bit_length(), (n_ - 1).bit_length()
-    A, B, C = lift_int(a).ravel(), lift_int(b).ravel(), lift_int(c).ravel()
-    nvs = (lm + lk, lk + ln, lm + ln)
-    t = O.Transcript(label)
-    com = []
-    for X, nv in zip((A, B, C), nvs):
-        cw, bh, tree = O.basefold_commit(X, nv, lr, 0)
-        com.append((cw, bh, tree[-4:]))
-        t.append_scalars(b"matmul-root", tree[-4:])
-    out = {"roots": np.array([x[2] for x in com])}
-    if key is not None:
-        aj = O.ajtai_i8_commit(key, rows, 1 << nvs[2], C, 1)
-        t.append_scalars(b"matmul-ajtai", aj)
-        out["ajtai"] = aj
-    point = np.array([t.challenge_ext2(b"matmul-point") for _ in range(ln + lm)], dtype=np.uint64)
-    claim = O.partial_eval_ext2(ext(C), nvs[2], point.ravel(), nvs[2]).ravel()[:2]
-    t.append_ext2(b"matmul-claim", claim)
-    r_j, r_i = point[:ln], point[ln:]
-    a_perm = A.reshape(m_, k_).T.ravel()  # row variables first
-    a1 = O.partial_eval_ext2(ext(a_perm), nvs[0], r_i.ravel(), lm).reshape(-1, 2)
-    b1 = O.partial_eval_ext2(ext(B), nvs[1], r_j.ravel(), ln).reshape(-1, 2)
-    ones = np.zeros((k_, 2), dtype=np.uint64)
-    ones[:, 0] = 1
-    sc = O.sumcheck_prove_ext2(t, np.stack([a1, b1, ones]), 3, lk)
-    r_l = sc["challenges"]
-    pts = (np.concatenate([r_l, r_i]), np.concatenate([r_j, r_l]), point)
-    for name, (cw, bh, root), nv, pt in zip(("proof_a", "proof_b", "proof_c"), com, nvs, pts):
-        out[name] = O.basefold_open_ext2(cw, bh, root, nv, lr, 0, pt.ravel(), table.entries, table.num_vars, t, nq)
-    out.update(point=point, claim=claim, round_msgs=sc["round_msgs"], challenges=r_l, final_evals=sc["final_evals"])
-    return out
+    return O.matmul_prove(a, b, label, table.entries, table.num_vars, lr, nq, key, rows)
 
 
 @pytest.mark.parametrize("dims,ajtai", [((3, 4, 3), True), ((4, 2, 5), False), ((2, 6, 4), True)])
@@ -115,3 +77,26 @@ def test_matmul_proof_matches_oracle(dims, ajtai):
     assert np.array_equal(got["proof_c"][:2], got["claim"])
     if aj:
         aj.free()
+
+
+def test_service_results_equal_alone_and_in_the_mix():
+    print("serve", flush=True)
+    rng = np.random.default_rng(21)
+    dims = [(3, 4, 3), (5, 3, 4), (3, 2, 3), (4, 5, 3), (3, 3, 6), (6, 2, 3), (4, 4, 4)]  # |C| >= 64 for Ajtai
+    jobs = []
+    for idx, (lm, lk, ln) in enumerate(dims):
+        a = rng.integers(-128, 128, (1 << lm, 1 << lk))
+        b = rng.integers(-128, 128, (1 << lk, 1 << ln))
+        jobs.append(MatmulJob.from_int8(a, b, b"serve-%d" % idx))
+    tnv = max(max(lm + lk, lk + ln, lm + ln) for lm, lk, ln in dims)
+    table = M.BasefoldTable.generate(tnv, 1, tnv, 5)
+    aj = AjtaiI8(key_from_seed(9), 32, 1 << 10)
+    alone = [prove_matmul(j, table, 1, 0, 7, aj) for j in jobs]
+    arrivals = np.cumsum(rng.exponential(2.0, len(jobs)))
+    for workers, budget in ((3, 64 << 30), (2, 1 << 20)):  # the small budget forces one-at-a-time admission
+        outs, admit, done = serve(jobs, arrivals, table, 1, 0, 7, aj, workers, budget)
+        for i in range(len(jobs)):
+            for k in alone[i]:
+                assert np.array_equal(outs[i][k], alone[i][k]), (workers, i, k)
+        assert np.all(admit >= arrivals - 1e-3) and np.all(done >= admit)
+    aj.free()
