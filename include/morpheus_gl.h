@@ -143,6 +143,31 @@ int sumcheck_round_message_ext2_ffi(const uint64_t *d_polys, uint64_t *d_partial
 int sumcheck_fold_ext2_ffi(const uint64_t *d_input, uint64_t *d_output, uint64_t challenge_c0,
                            uint64_t challenge_c1, int d, size_t original_size, size_t half);   /* ffi.rs:619 */
 
+/* ---- Fiat-Shamir duplex challengers, n independent Poseidon2-w8 sponges (rate 4) in device
+ * memory, one per thread of each batch call (ffi.rs:315-371; cuda/challenger.cuh:43-259).
+ * State: 136 bytes = sponge[8], in[4], out[4] (u64), in_count, out_count (int32). Values
+ * layout: challenger i uses values[i * count ..]; Ext2 = 2 consecutive words, Ext5 = 5.
+ * Launches are asynchronous on the calling thread's stream; the copies synchronize it. ---- */
+int challenger_state_size(void);                                                            /* ffi.rs:315 */
+int challenger_alloc_states(void **d_states, int n);                                        /* ffi.rs:316 */
+int challenger_init_states(void *d_states, int n);                                          /* ffi.rs:317 */
+int challenger_observe_ffi(void *d_states, const uint64_t *d_values, int n);                /* ffi.rs:319 */
+int challenger_observe_slice_ffi(void *d_states, const uint64_t *d_values, int count, int n); /* ffi.rs:325 */
+int challenger_sample_ffi(void *d_states, uint64_t *d_outputs, int n);                      /* ffi.rs:332 */
+int challenger_sample_array_ffi(void *d_states, uint64_t *d_outputs, int count, int n);     /* ffi.rs:338 */
+int challenger_sample_ext2_ffi(void *d_states, uint64_t *d_outputs, int n);                 /* ffi.rs:345 */
+int challenger_sample_ext5_ffi(void *d_states, uint64_t *d_outputs, int n);                 /* ffi.rs:351 */
+int challenger_observe_ext2_ffi(void *d_states, const uint64_t *d_values, int n);           /* ffi.rs:357 */
+int challenger_copy_state_to_host(void *h_state, const void *d_state);                      /* ffi.rs:363 */
+int challenger_copy_state_to_device(void *d_state, const void *h_state);                    /* ffi.rs:368 */
+
+/* acc[i] += (s0 + s1 X) * src[i] over Ext2 X^2 - 7, n Ext2 entries (ffi.rs:633) */
+int ext2_scale_accumulate_ffi(uint64_t scalar_c0, uint64_t scalar_c1, const uint64_t *d_src, uint64_t *d_acc,
+                              int n);
+/* out[x] = in[y] with bit b of x = bit d_perm_map[b] of y; total = 2^n_bits (ffi.rs:645).
+ * d_perm_map is the inverse variable map (new position -> old), as field.rs:205-214 passes it. */
+int bit_permute_gl_ffi(const uint64_t *d_input, uint64_t *d_output, const int *d_perm_map, int n_bits, int total);
+
 /* fused variable permutation + partial evaluation, ffi.rs:657-668:
  * output[j] = sum_{b < 2^m} evals[lo_lut[x & (2^half-1)] | hi_lut[x >> half]] * eq[b], x = j 2^m + b */
 int fused_permute_partial_eval_ffi(const uint64_t *d_evals, uint64_t *d_output, const uint64_t *d_eq_table,

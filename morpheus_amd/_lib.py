@@ -193,6 +193,13 @@ _SIGS = {
     "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
     "mam_prove_batch": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci]),
     "mam_matmul_prove": (ci, [ctypes.POINTER(MatmulInstance_t), vp, ci, ci, ci, ci, vp, ci]),
+    "challenger_state_size": (ci, []), "challenger_alloc_states": (ci, [ctypes.POINTER(vp), ci]),
+    "challenger_init_states": (ci, [vp, ci]), "challenger_observe_ffi": (ci, [vp, vp, ci]),
+    "challenger_observe_slice_ffi": (ci, [vp, vp, ci, ci]), "challenger_sample_ffi": (ci, [vp, vp, ci]),
+    "challenger_sample_array_ffi": (ci, [vp, vp, ci, ci]), "challenger_sample_ext2_ffi": (ci, [vp, vp, ci]),
+    "challenger_sample_ext5_ffi": (ci, [vp, vp, ci]), "challenger_observe_ext2_ffi": (ci, [vp, vp, ci]),
+    "challenger_copy_state_to_host": (ci, [vp, vp]), "challenger_copy_state_to_device": (ci, [vp, vp]),
+    "ext2_scale_accumulate_ffi": (ci, [u64, u64, vp, vp, ci]), "bit_permute_gl_ffi": (ci, [vp, vp, vp, ci, ci]),
     "mam_serve_matmul": (ci, [ctypes.POINTER(ServeRequest_t), ci, vp, ci, ci, ci, ci, vp, ci, ci, u64]),
     "fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "agl_fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),

@@ -156,6 +156,13 @@ MAM_API int aext2_scale_accumulate_ffi(u64 s0, u64 s1, const u64 *src, u64 *acc,
     k_ext_scale_acc<FieldQ><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(qe2{fq::canon(s0), fq::canon(s1)}, src, acc, n);
     return mam_sync_status();
 }
+// Goldilocks twin (goldilocks-cuda-rs/src/ffi.rs:633-639): acc += s * src over Ext2 X^2 - 7,
+// asynchronous like the reference's (wrapper.cu:1807-1815).
+MAM_API int ext2_scale_accumulate_ffi(u64 s0, u64 s1, const u64 *src, u64 *acc, int n) {
+    if (n <= 0) return 0;
+    k_ext_scale_acc<FieldGL><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(ext2{gl_canon(s0), gl_canon(s1)}, src, acc, n);
+    return mam_launch_status();
+}
 MAM_API int agl_to_aext2_batch_ffi(const u64 *in, u64 *out, int n) {
     if (n <= 0) return 0;
     k_base_to_ext<FieldQ><<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(in, out, n);
