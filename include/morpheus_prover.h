@@ -116,6 +116,28 @@ typedef struct {
 int mam_prove_batch(mam_batch_instance *inst, int n, const uint64_t *d_table, int table_nv, int log_rate, int hash,
                     int num_queries, int workers);
 
+/* ---- batch opening of several commitments at several Ext2 points ----
+ * batch_open_ext2 / BasefoldVerifier::batch_verify_ext2 (goldilocks-cuda-rs/src/basefold.rs:
+ * 1202-1525, 1931-2221): commitments of equal num_vars, log_rate and hash; points (host)
+ * [num_points][nv][2]; claims (host) name a commitment and a point. Flat proof (u64):
+ *   outer_oracles[nv][3][2] | combined_root[4] | inner_eval[2] | inner_oracles[nv][3][2] |
+ *   folded_roots[nv-1][4] | final_cw[2^lr][2] |
+ *   nq x { index | values[nv][2][2] (Ext2 at level 0) | paths as in the single opening } |
+ *   nq x num_evals x { values[2] | path[nv+lr-1][4] }
+ * The verifier takes the host table and returns 1 on accept. */
+typedef struct {
+    int poly, point;
+    uint64_t value[2];
+} mam_eval_claim;
+long mam_basefold_batch_proof_len(int num_vars, int log_rate, int num_queries, int num_evals);
+int mam_basefold_batch_open_ext2(void *const *handles, int num_comms, const uint64_t *points, int num_points,
+                                 const mam_eval_claim *evals, int num_evals, const uint64_t *d_table, int table_nv,
+                                 mam_transcript *t, int num_queries, uint64_t *proof);
+int mam_basefold_batch_verify_ext2(const uint64_t *roots, int num_comms, int num_vars, int log_rate, int hash,
+                                   const uint64_t *points, int num_points, const mam_eval_claim *evals,
+                                   int num_evals, const uint64_t *proof, int num_queries, const uint64_t *table,
+                                   int table_nv, mam_transcript *t);
+
 /* ---- matmul proof C = A B (benchmark configuration 5) ----
  * Composed from the einsum proof of zk-torch-3/src/basicblock/einsum.rs:807-1098 for
  * "ij,jk->ik": Basefold commitments of A (m x k), B (k x n), C (m x n) (row-major, m = 2^lm,

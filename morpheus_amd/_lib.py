@@ -42,6 +42,10 @@ class MatmulInstance_t(ctypes.Structure):
                 ("end_ms", ctypes.c_double), ("status", ci)]
 
 
+class EvalClaim_t(ctypes.Structure):
+    _fields_ = [("poly", ci), ("point", ci), ("value", u64 * 2)]
+
+
 class ServeRequest_t(ctypes.Structure):
     _fields_ = [("inst", ctypes.POINTER(MatmulInstance_t)), ("arrival_ms", ctypes.c_double),
                 ("admit_ms", ctypes.c_double), ("done_ms", ctypes.c_double), ("status", ci)]
@@ -200,6 +204,9 @@ _SIGS = {
     "challenger_sample_ext5_ffi": (ci, [vp, vp, ci]), "challenger_observe_ext2_ffi": (ci, [vp, vp, ci]),
     "challenger_copy_state_to_host": (ci, [vp, vp]), "challenger_copy_state_to_device": (ci, [vp, vp]),
     "ext2_scale_accumulate_ffi": (ci, [u64, u64, vp, vp, ci]), "bit_permute_gl_ffi": (ci, [vp, vp, vp, ci, ci]),
+    "mam_basefold_batch_proof_len": (ctypes.c_long, [ci, ci, ci, ci]),
+    "mam_basefold_batch_open_ext2": (ci, [vp, ci, vp, ci, vp, ci, vp, ci, vp, ci, vp]),
+    "mam_basefold_batch_verify_ext2": (ci, [vp, ci, ci, ci, ci, vp, ci, vp, ci, vp, ci, vp, ci, vp]),
     "mam_serve_matmul": (ci, [ctypes.POINTER(ServeRequest_t), ci, vp, ci, ci, ci, ci, vp, ci, ci, u64]),
     "fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "agl_fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),

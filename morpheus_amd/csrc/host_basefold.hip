@@ -24,6 +24,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "basefold.h"
@@ -67,7 +68,8 @@ struct Commitment {
 const int MAX_LEVELS = 40;
 
 struct QuerySrc {
-    const u64 *cw;                     // initial base codeword
+    const u64 *cw;                     // initial codeword (base, or Ext2 when cw_ext2)
+    int cw_ext2;
     const u64 *tree0;                  // initial tree
     const u64 *fcw[MAX_LEVELS];        // folded codewords 0..nv-2 (Ext2)
     const u64 *ftree[MAX_LEVELS];      // their trees
@@ -85,7 +87,12 @@ __global__ void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx, u64 *_
     // values: round 0 from the base codeword, rounds 1..nv-1 from folded codewords
     for (int r = threadIdx.x; r < nv; r += blockDim.x) {
         u64 *v = vals + 4 * r;
-        if (r == 0) {
+        if (r == 0 && q.cw_ext2) {
+            v[0] = q.cw[4 * leaf];
+            v[1] = q.cw[4 * leaf + 1];
+            v[2] = q.cw[4 * leaf + 2];
+            v[3] = q.cw[4 * leaf + 3];
+        } else if (r == 0) {
             v[0] = q.cw[2 * leaf];
             v[1] = 0;
             v[2] = q.cw[2 * leaf + 1];
@@ -172,25 +179,32 @@ void pool_setup() {
     done = true;
 }
 
-int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int table_nv, mam_transcript *t, int nq,
-              u64 *proof) {
-    const int nv = C.nv, lr = C.lr;
-    if (nv < 2 || table_nv < nv) return -2;
-    hipStream_t s = mam_stream();
-    pool_setup();
-    const size_t n = (size_t)1 << nv, cw_len = (size_t)1 << (nv + lr);
-    u64 *pc = proof;
-    // transcript preamble
-    observe_hash(t, C.root);
-    for (int i = 0; i < nv; i++) observe_ext2(t, point2 + 2 * i);
+// The opening proper, from the eq table onwards: the sumcheck of eq * bh with its oracles,
+// the folds of the codeword with their trees, the final codeword and the queries. Writes
+// eval | oracles | folded roots | final cw | queries to `out`. open_ext2 runs it on one
+// commitment (base bh and codeword); the batch opening runs it on the combined codeword
+// (Ext2 bh and codeword), whose proof section has the same layout.
+struct OpenSrc {
+    const u64 *eq;     // Ext2 eq table at the opening point, natural order, 2^nv entries
+    const u64 *bh;     // evaluations in natural order (base or Ext2)
+    bool bh_base;
+    const u64 *cw;     // codeword, Type1 order (base or Ext2), 2^(nv+lr) entries
+    bool cw_base;
+    const u64 *tree0;  // Merkle tree of cw
+    int nv, lr, hash;
+};
 
-    // all device scratch of the opening is carved from the thread's arena (slot 0)
+int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript *t, int nq, u64 *out) {
+    const int nv = S.nv, lr = S.lr;
+    hipStream_t s = mam_stream();
+    const size_t n = (size_t)1 << nv, cw_len = (size_t)1 << (nv + lr);
+    // device scratch of the opening is carved from the thread's arena (slot 0)
     const long qlen = query_len(nv, lr);
     size_t need = 0;
     auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
     {
-        need += words(2 * nv) + words(2 * n) + words(n) + 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) +
-                words(8) + words(nq) + words((size_t)nq * qlen);
+        need += 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) + words(8) + words(nq) +
+                words((size_t)nq * qlen);
         size_t flen = cw_len / 2;
         for (int r = 0; r < nv; r++) {
             need += words(2 * flen);
@@ -207,9 +221,6 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
         return p;
     };
     int rc = 0;
-    u64 *d_point = alloc(2 * nv);
-    u64 *eq = alloc(2 * n);
-    u64 *bh = alloc(n);
     u64 *eqp[2] = {alloc(n), alloc(n / 2)};   // Ext2 ping-pong: n/2 then n/4 elements
     u64 *bhp[2] = {alloc(n), alloc(n / 2)};
     u64 *red = alloc(RED_BLOCKS * 6 + 6);
@@ -228,13 +239,10 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     if (!h) rc = -1;
     RedScratch rs = {red, (u32 *)cnt, red + RED_BLOCKS * 6};
     if (!rc && hipMemsetAsync(cnt, 0, 64, s) != hipSuccess) rc = -1;
-    if (!rc && hipMemcpyAsync(d_point, point2, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) rc = -1;
-    if (!rc) rc = mam_eq_table(d_point, nv, eq, true, s);
-    if (!rc) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
-    if (!rc) rc = mam_open_oracle0(eq, bh, n / 2, rs, s);
+    if (!rc) rc = mam_open_oracle0(S.eq, S.bh, S.bh_base, n / 2, rs, s);
     if (!rc) rc = read_back(h, rs.result, 6, s);
-    u64 *o_eval = pc;
-    u64 *o_oracles = pc + 2;
+    u64 *o_eval = out;
+    u64 *o_oracles = out + 2;
     u64 *o_roots = o_oracles + 6L * nv;
     u64 *o_final = o_roots + 4L * (nv - 1);
     u64 *o_queries = o_final + 2L * (1L << lr);
@@ -247,13 +255,12 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
         o_eval[1] = ev.c1;
         for (int k = 0; k < 3; k++) observe_ext2(t, o_oracles + 2 * k);
     }
-    // round 0: mixed transition
-    const u64 *eq_cur = eq, *bh_cur = bh;
-    bool bh_base = true;
+    const u64 *eq_cur = S.eq, *bh_cur = S.bh;
+    bool bh_base = S.bh_base;
     size_t sc_size = n;
     size_t flen = cw_len / 2;  // folded codeword length of this round
-    const u64 *cw_in = C.codeword;
-    bool cw_base = true;
+    const u64 *cw_in = S.cw;
+    bool cw_base = S.cw_base;
     for (int round = 0; round < nv - 1 && !rc; round++) {
         ext2 ch = sample_ext2(t);
         const int pp = round & 1;
@@ -264,7 +271,7 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
         bh_base = false;
         sc_size /= 2;
         const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, round);
-        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, C.hash, s);
+        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s);
         if (rc) break;
         const size_t leaves = flen / 2;
         // oracle (3 Ext2) and the new root in one mailbox round trip
@@ -285,7 +292,7 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
         // last fold, no tree
         ext2 ch = sample_ext2(t);
         const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, nv - 1);
-        rc = mam_cw_fold(cw_in, lvl, ch, fcw[nv - 1], flen, false, s);
+        rc = mam_cw_fold(cw_in, lvl, ch, fcw[nv - 1], flen, cw_base, s);
         if (!rc) rc = read_back(h, fcw[nv - 1], 2 * flen, s);
         if (!rc) memcpy(o_final, h, 2 * flen * sizeof(u64));
     }
@@ -297,8 +304,9 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
         if (!d_idx || !d_out) rc = -1;
         QuerySrc qs;
         memset(&qs, 0, sizeof(qs));
-        qs.cw = C.codeword;
-        qs.tree0 = C.tree;
+        qs.cw = S.cw;
+        qs.cw_ext2 = S.cw_base ? 0 : 1;
+        qs.tree0 = S.tree0;
         for (int r = 0; r < nv - 1; r++) {
             qs.fcw[r] = fcw[r];
             qs.ftree[r] = ftree[r];
@@ -317,6 +325,29 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
         if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = -1;
     }
     return rc;
+}
+
+int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int table_nv, mam_transcript *t, int nq,
+              u64 *proof) {
+    const int nv = C.nv;
+    if (nv < 2 || table_nv < nv) return -2;
+    hipStream_t s = mam_stream();
+    pool_setup();
+    const size_t n = (size_t)1 << nv;
+    // transcript preamble
+    observe_hash(t, C.root);
+    for (int i = 0; i < nv; i++) observe_ext2(t, point2 + 2 * i);
+    // point, eq table and natural-order bh in the thread's arena slot 3
+    auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
+    u64 *arena = mam_dev_arena(words(2 * nv) + words(2 * n) + words(n), 3);
+    if (!arena) return -1;
+    u64 *d_point = arena, *eq = arena + words(2 * nv), *bh = eq + words(2 * n);
+    if (hipMemcpyAsync(d_point, point2, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+    int rc = mam_eq_table(d_point, nv, eq, true, s);
+    if (!rc) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
+    if (rc) return rc;
+    OpenSrc S{eq, bh, true, C.codeword, true, C.tree, nv, C.lr, C.hash};
+    return open_core(S, d_table, table_nv, t, nq, proof);
 }
 
 // ---------------------------------------------------------------------------
@@ -411,6 +442,384 @@ int verify_ext2(const u64 *root4, const u64 *point2, int nv, int lr, int hash, c
             fidx /= 2;
         }
         pc = path;
+    }
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
+// Batch opening (batch_open_ext2, basefold.rs:1202-1525; batch_verify_ext2 :1931-2221)
+// ---------------------------------------------------------------------------
+// Proof layout: outer_oracles[nv][3][2] | combined_root[4] | [open section of the combined
+// codeword: inner_eval[2] | inner_oracles[nv][3][2] | folded_roots[nv-1][4] | final_cw |
+// queries with Ext2 values at level 0] | nq x num_evals x { values[2] | path[nv+lr-1][4] }.
+struct EvalClaim {
+    int poly, point;
+    u64 value[2];
+};
+
+long batch_proof_len(int nv, int lr, int nq, int ne) {
+    return 6L * nv + 4 + proof_len(nv, lr, nq) + (long)nq * ne * (2 + 4L * (nv + lr - 1));
+}
+
+int batch_ell(int ne) {
+    if (ne <= 1) return 1;
+    int l = 0;
+    while ((1 << l) < ne) l++;
+    return l;
+}
+
+// eq_poly_host (basefold.rs:3008-3025): weights[i] = eq(t, bits of i), t_0 on the most
+// significant index bit.
+std::vector<u64> eq_weights_msb(const std::vector<u64> &tch) {
+    std::vector<u64> w((size_t)1 << tch.size(), 0);
+    w[0] = 1;
+    for (size_t i = 0; i < tch.size(); i++) {
+        const long half = 1L << i;
+        const u64 omr = gl_sub(1, tch[i]);
+        for (long j = half - 1; j >= 0; j--) {
+            const u64 v = w[j];
+            w[2 * j] = gl_mul(v, omr);
+            w[2 * j + 1] = gl_mul(v, tch[i]);
+        }
+    }
+    return w;
+}
+
+// eq_eval_host_ext2 (basefold.rs:3218-3230)
+ext2 eq_eval_ext2(const std::vector<ext2> &r, const u64 *pt2) {
+    ext2 res = e2_one();
+    for (size_t i = 0; i < r.size(); i++) {
+        const ext2 p = ld2(pt2 + 2 * i);
+        res = e2_mul(res, e2_add(e2_mul(r[i], p), e2_mul(e2_sub(e2_one(), r[i]), e2_sub(e2_one(), p))));
+    }
+    return res;
+}
+
+const int LIN_MAX = 16;
+struct LinSrc {
+    const u64 *p[LIN_MAX];
+    u64 s0[LIN_MAX], s1[LIN_MAX];
+    int count;
+};
+
+// out[y] = (accumulate ? out[y] : 0) + sum_k s_k src_k[y] over base-field sources; the output
+// is base (EXT_OUT false, s_k base) or Ext2 (s_k Ext2).
+template <bool EXT_OUT>
+__global__ void k_lincomb(LinSrc L, size_t len, int accumulate, u64 *__restrict__ out) {
+    const size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (y >= len) return;
+    if (EXT_OUT) {
+        ext2 acc = accumulate ? e2_load(out, y) : e2_zero();
+        for (int k = 0; k < L.count; k++) acc = e2_add(acc, e2_scale(gl_canon(L.p[k][y]), ext2{L.s0[k], L.s1[k]}));
+        e2_store(out, y, acc);
+    } else {
+        u64 acc = accumulate ? out[y] : 0;
+        for (int k = 0; k < L.count; k++) acc = gl_add(acc, gl_mul(gl_canon(L.p[k][y]), L.s0[k]));
+        out[y] = acc;
+    }
+}
+
+// Sum of `n` oracles (6 words each) into out (one wave).
+__global__ void k_sum_oracles(const u64 *__restrict__ res, int n, u64 *__restrict__ out) {
+    if (threadIdx.x >= 6) return;
+    u64 acc = 0;
+    for (int i = 0; i < n; i++) acc = gl_add(acc, res[6 * i + threadIdx.x]);
+    out[threadIdx.x] = acc;
+}
+
+// Individual openings: block b = q * ne + e writes the base pair at the query's leaf and the
+// authentication path in that commitment's tree. srcs[2e] = codeword, srcs[2e + 1] = tree.
+__global__ void k_gather_individual(const u64 *const *__restrict__ srcs, const u64 *__restrict__ idx, int ne,
+                                    int depth, u64 *__restrict__ out) {
+    const int b = blockIdx.x;
+    const int q = b / ne, e = b % ne;
+    const u64 leaf = idx[q];
+    const u64 *cw = srcs[2 * e], *tr = srcs[2 * e + 1];
+    u64 *o = out + (size_t)b * (2 + 4 * (size_t)depth);
+    if (threadIdx.x == 0) {
+        o[0] = gl_canon(cw[2 * leaf]);
+        o[1] = gl_canon(cw[2 * leaf + 1]);
+    }
+    const u64 leaves = (u64)1 << depth;
+    for (int l = threadIdx.x; l < depth; l += blockDim.x) {
+        const u64 lvl_off = 2 * leaves - 2 * (leaves >> l);
+        const u64 *d = tr + 4 * (lvl_off + ((leaf >> l) ^ 1));
+        u64 *w = o + 2 + 4 * l;
+        w[0] = d[0];
+        w[1] = d[1];
+        w[2] = d[2];
+        w[3] = d[3];
+    }
+}
+
+// Launch k_lincomb over `srcs` in chunks of LIN_MAX (the first chunk overwrites `out`).
+int lincomb(const std::vector<const u64 *> &srcs, const std::vector<ext2> &sc, bool ext_out, size_t len, u64 *out,
+            hipStream_t s) {
+    size_t k0 = 0;
+    do {
+        LinSrc L;
+        memset(&L, 0, sizeof(L));
+        L.count = (int)std::min((size_t)LIN_MAX, srcs.size() - k0);
+        for (int k = 0; k < L.count; k++) {
+            L.p[k] = srcs[k0 + k];
+            L.s0[k] = sc[k0 + k].c0;
+            L.s1[k] = sc[k0 + k].c1;
+        }
+        if (ext_out) k_lincomb<true><<<grid_1d(len), MAM_BLOCK, 0, s>>>(L, len, k0 > 0, out);
+        else k_lincomb<false><<<grid_1d(len), MAM_BLOCK, 0, s>>>(L, len, k0 > 0, out);
+        if (hipGetLastError() != hipSuccess) return -1;
+        k0 += L.count;
+    } while (k0 < srcs.size());
+    return 0;
+}
+
+int batch_open_ext2(const std::vector<const Commitment *> &C, const u64 *points, int np, const EvalClaim *ev, int ne,
+                    const u64 *d_table, int table_nv, mam_transcript *t, int nq, u64 *proof) {
+    const int nv = C[0]->nv, lr = C[0]->lr, hash = C[0]->hash;
+    if (nv < 2 || table_nv < nv || np < 1 || ne < 1) return -2;
+    for (const Commitment *c : C)
+        if (c->nv != nv || c->lr != lr || c->hash != hash) return -2;
+    for (int e = 0; e < ne; e++)
+        if (ev[e].poly < 0 || ev[e].poly >= (int)C.size() || ev[e].point < 0 || ev[e].point >= np) return -2;
+    hipStream_t s = mam_stream();
+    pool_setup();
+    const size_t n = (size_t)1 << nv, cw_len = (size_t)1 << (nv + lr);
+    // transcript preamble: roots, points, claimed values; batching weights
+    for (const Commitment *c : C) observe_hash(t, c->root);
+    for (int p = 0; p < np; p++)
+        for (int i = 0; i < nv; i++) observe_ext2(t, points + 2L * (p * nv + i));
+    for (int e = 0; e < ne; e++) observe_ext2(t, ev[e].value);
+    std::vector<u64> tch(batch_ell(ne));
+    for (auto &x : tch) x = mam::tr_challenge_scalar(t, nullptr, 0);
+    const std::vector<u64> w = eq_weights_msb(tch);
+
+    // device buffers in the thread's arena slot 2
+    auto words = [](size_t x) { return (x + 31) & ~(size_t)31; };
+    const size_t per_point = words(2 * n) + 2 * words(n) + 2 * (words(n) + words(n / 2));
+    const size_t need = words(2L * np * nv) + np * per_point + words(RED_BLOCKS * 6 + 6) + words(8) +
+                        words(6L * np) + words(6) + words(2 * cw_len) + 2 * words(2 * n) + words((cw_len - 1) * 4) +
+                        words(2 * nv) + words(2 * n) + words(2L * ne) + words(nq) +
+                        words((size_t)nq * ne * (2 + 4 * (nv + lr - 1)));
+    u64 *arena = mam_dev_arena(need, 2);
+    if (!arena) return -1;
+    size_t used = 0;
+    auto alloc = [&](size_t x) {
+        u64 *p = arena + used;
+        used += words(x);
+        return p;
+    };
+    u64 *d_points = alloc(2L * np * nv);
+    std::vector<u64 *> eqs(np), g1(np), gn(np), eqp0(np), eqp1(np), gp0(np), gp1(np);
+    for (int p = 0; p < np; p++) {
+        eqs[p] = alloc(2 * n);
+        g1[p] = alloc(n);
+        gn[p] = alloc(n);
+        eqp0[p] = alloc(n);
+        eqp1[p] = alloc(n / 2);
+        gp0[p] = alloc(n);
+        gp1[p] = alloc(n / 2);
+    }
+    u64 *red = alloc(RED_BLOCKS * 6 + 6), *cnt = alloc(8), *res = alloc(6L * np), *sum6 = alloc(6);
+    u64 *ccw = alloc(2 * cw_len), *cbh1 = alloc(2 * n), *cbhn = alloc(2 * n), *ctree = alloc((cw_len - 1) * 4);
+    u64 *d_r = alloc(2 * nv), *eqr = alloc(2 * n);
+    const u64 **d_srcs = (const u64 **)alloc(2L * ne);
+    u64 *d_idx = alloc(nq), *d_ind = alloc((size_t)nq * ne * (2 + 4 * (nv + lr - 1)));
+    u64 *h = pinned_scratch();
+    if (!h) return -1;
+    int rc = 0;
+    if (hipMemsetAsync(cnt, 0, 64, s) != hipSuccess) return -1;
+    if (hipMemcpyAsync(d_points, points, 2L * np * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
+        return -1;
+    // g_p = sum over the claims at point p of w_e bh_e (natural order), eq_p
+    for (int p = 0; p < np && !rc; p++) {
+        std::vector<const u64 *> srcs;
+        std::vector<ext2> sc;
+        for (int e = 0; e < ne; e++)
+            if (ev[e].point == p) {
+                srcs.push_back(C[ev[e].poly]->bh_evals);
+                sc.push_back(e2_base(w[e]));
+            }
+        rc = lincomb(srcs, sc, false, n, g1[p], s);
+        if (!rc) rc = mam_bitrev(g1[p], gn[p], nv, 1, s);
+        if (!rc) rc = mam_eq_table(d_points + 2L * p * nv, nv, eqs[p], true, s);
+    }
+    if (rc) return rc;
+    // outer sumcheck of sum_p eq_p g_p: per-point oracles, summed on the device
+    u64 *outer = proof;
+    std::vector<ext2> r(nv);
+    auto publish_sum = [&](u64 *dst) {
+        k_sum_oracles<<<1, 64, 0, s>>>(res, np, sum6);
+        if (hipGetLastError() != hipSuccess || read_back(h, sum6, 6, s)) return -1;
+        memcpy(dst, h, 6 * sizeof(u64));
+        for (int k = 0; k < 3; k++) observe_ext2(t, dst + 2 * k);
+        return 0;
+    };
+    for (int p = 0; p < np && !rc; p++) rc = mam_open_oracle0(eqs[p], gn[p], true, n / 2, RedScratch{red, (u32 *)cnt, res + 6L * p}, s);
+    if (!rc) rc = publish_sum(outer);
+    std::vector<const u64 *> ceq(eqs.begin(), eqs.end()), cg(gn.begin(), gn.end());
+    size_t sc_size = n;
+    for (int round = 0; round < nv - 1 && !rc; round++) {
+        r[round] = sample_ext2(t);
+        for (int p = 0; p < np && !rc; p++) {
+            u64 *eo = (round & 1) ? eqp1[p] : eqp0[p], *go = (round & 1) ? gp1[p] : gp0[p];
+            rc = mam_open_fold_oracle(ceq[p], cg[p], round == 0, r[round], eo, go, sc_size / 4,
+                                      RedScratch{red, (u32 *)cnt, res + 6L * p}, s);
+            ceq[p] = eo;
+            cg[p] = go;
+        }
+        sc_size /= 2;
+        if (!rc) rc = publish_sum(outer + 6L * (round + 1));
+    }
+    if (rc) return rc;
+    r[nv - 1] = sample_ext2(t);
+    // combined codeword and bh: sum_e eq(r, point_e) w_e (codeword_e, bh_e)
+    std::vector<ext2> eqpt(np), sc(ne);
+    for (int p = 0; p < np; p++) eqpt[p] = eq_eval_ext2(r, points + 2L * p * nv);
+    std::vector<const u64 *> cws(ne), bhs(ne);
+    for (int e = 0; e < ne; e++) {
+        sc[e] = e2_scale(w[e], eqpt[ev[e].point]);
+        cws[e] = C[ev[e].poly]->codeword;
+        bhs[e] = C[ev[e].poly]->bh_evals;
+    }
+    rc = lincomb(cws, sc, true, cw_len, ccw, s);
+    if (!rc) rc = lincomb(bhs, sc, true, n, cbh1, s);
+    if (!rc) rc = mam_bitrev(cbh1, cbhn, nv, 2, s);
+    if (!rc) rc = mam_merkle_build(ccw, ctree, cw_len / 2, hash, 1, s);
+    if (!rc) rc = read_back(h, ctree + mam_merkle_root_offset(cw_len / 2), 4, s);
+    if (rc) return rc;
+    u64 *croot = proof + 6L * nv;
+    memcpy(croot, h, 4 * sizeof(u64));
+    observe_hash(t, croot);
+    // inner commit phase on the combined codeword at r
+    std::vector<u64> r2(2 * nv);
+    for (int i = 0; i < nv; i++) {
+        r2[2 * i] = r[i].c0;
+        r2[2 * i + 1] = r[i].c1;
+    }
+    if (hipMemcpyAsync(d_r, r2.data(), 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+    rc = mam_eq_table(d_r, nv, eqr, true, s);
+    if (rc) return rc;
+    u64 *inner = croot + 4;
+    OpenSrc S{eqr, cbhn, false, ccw, false, ctree, nv, lr, hash};
+    rc = open_core(S, d_table, table_nv, t, nq, inner);
+    if (rc || nq == 0) return rc;
+    // individual openings at the same leaves
+    const long qlen = query_len(nv, lr);
+    const u64 *queries = inner + proof_len(nv, lr, 0);
+    std::vector<u64> idx(nq);
+    for (int q = 0; q < nq; q++) idx[q] = queries[(size_t)q * qlen];
+    std::vector<const u64 *> srcs(2L * ne);
+    for (int e = 0; e < ne; e++) {
+        srcs[2 * e] = C[ev[e].poly]->codeword;
+        srcs[2 * e + 1] = C[ev[e].poly]->tree;
+    }
+    const int depth = nv + lr - 1;
+    if (hipMemcpyAsync(d_srcs, srcs.data(), srcs.size() * sizeof(u64 *), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(d_idx, idx.data(), nq * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
+        return -1;
+    k_gather_individual<<<nq * ne, 64, 0, s>>>(d_srcs, d_idx, ne, depth, d_ind);
+    if (hipGetLastError() != hipSuccess) return -1;
+    u64 *o_ind = inner + proof_len(nv, lr, nq);
+    if (hipMemcpyAsync(o_ind, d_ind, (size_t)nq * ne * (2 + 4 * depth) * sizeof(u64), hipMemcpyDeviceToHost, s) !=
+        hipSuccess)
+        return -1;
+    return hipStreamSynchronize(s) == hipSuccess ? 0 : -1;
+}
+
+bool check_transition(const u64 *o, const u64 *on, ext2 ch) {
+    const ext2 val = e2_add(ld2(o), e2_add(e2_mul(ld2(o + 2), ch), e2_mul(ld2(o + 4), e2_mul(ch, ch))));
+    const ext2 n0 = ld2(on);
+    return e2_eq(val, e2_add(n0, e2_add(n0, e2_add(ld2(on + 2), ld2(on + 4)))));
+}
+
+int batch_verify_ext2(const u64 *roots, int nc, int nv, int lr, int hash, const u64 *points, int np,
+                      const EvalClaim *ev, int ne, const u64 *proof, int nq, const u64 *table, int table_nv,
+                      mam_transcript *t) {
+    if (nv < 2 || table_nv < nv || nc < 1 || np < 1 || ne < 1) return 0;
+    for (int e = 0; e < ne; e++)
+        if (ev[e].poly < 0 || ev[e].poly >= nc || ev[e].point < 0 || ev[e].point >= np) return 0;
+    const u64 *outer = proof, *croot = proof + 6L * nv, *pc = croot + 4;
+    const ext2 inner_eval = ld2(pc);
+    const u64 *inner = pc + 2, *froots = inner + 6L * nv, *final_cw = froots + 4L * (nv - 1);
+    const u64 *queries = final_cw + 2L * (1L << lr);
+    for (int c = 0; c < nc; c++) observe_hash(t, roots + 4L * c);
+    for (int p = 0; p < np; p++)
+        for (int i = 0; i < nv; i++) observe_ext2(t, points + 2L * (p * nv + i));
+    for (int e = 0; e < ne; e++) observe_ext2(t, ev[e].value);
+    std::vector<u64> tch(batch_ell(ne));
+    for (auto &x : tch) x = mam::tr_challenge_scalar(t, nullptr, 0);
+    const std::vector<u64> w = eq_weights_msb(tch);
+    ext2 claimed = e2_zero();
+    for (int e = 0; e < ne; e++) claimed = e2_add(claimed, e2_scale(w[e], ld2(ev[e].value)));
+    const ext2 o0 = ld2(outer);
+    if (!e2_eq(e2_add(o0, e2_add(o0, e2_add(ld2(outer + 2), ld2(outer + 4)))), claimed)) return 0;
+    for (int k = 0; k < 3; k++) observe_ext2(t, outer + 2 * k);
+    std::vector<ext2> r(nv), ri(nv);
+    for (int round = 0; round < nv - 1; round++) {
+        r[round] = sample_ext2(t);
+        if (!check_transition(outer + 6L * round, outer + 6L * (round + 1), r[round])) return 0;
+        for (int k = 0; k < 3; k++) observe_ext2(t, outer + 6L * (round + 1) + 2 * k);
+    }
+    r[nv - 1] = sample_ext2(t);
+    {
+        const u64 *o = outer + 6L * (nv - 1);
+        const ext2 ch = r[nv - 1];
+        const ext2 fin = e2_add(ld2(o), e2_add(e2_mul(ld2(o + 2), ch), e2_mul(ld2(o + 4), e2_mul(ch, ch))));
+        if (!e2_eq(fin, inner_eval)) return 0;
+    }
+    observe_hash(t, croot);
+    const ext2 i0 = ld2(inner);
+    if (!e2_eq(e2_add(i0, e2_add(i0, e2_add(ld2(inner + 2), ld2(inner + 4)))), inner_eval)) return 0;
+    for (int k = 0; k < 3; k++) observe_ext2(t, inner + 2 * k);
+    for (int round = 0; round < nv - 1; round++) {
+        ri[round] = sample_ext2(t);
+        if (!check_transition(inner + 6L * round, inner + 6L * (round + 1), ri[round])) return 0;
+        for (int k = 0; k < 3; k++) observe_ext2(t, inner + 6L * (round + 1) + 2 * k);
+        observe_hash(t, froots + 4L * round);
+    }
+    ri[nv - 1] = sample_ext2(t);
+    std::vector<ext2> sc(ne);
+    for (int e = 0; e < ne; e++) sc[e] = e2_scale(w[e], eq_eval_ext2(r, points + 2L * ev[e].point * nv));
+    const u64 cw_half = (u64)1 << (nv + lr - 1);
+    const long qlen = query_len(nv, lr);
+    const int depth = nv + lr - 1;
+    const u64 *indiv = queries + (size_t)nq * qlen;
+    for (int q = 0; q < nq; q++) {
+        const u64 leaf = mam::tr_challenge_scalar(t, nullptr, 0) % cw_half;
+        const u64 *qp = queries + (size_t)q * qlen;
+        if (qp[0] != leaf) return 0;
+        const u64 *vals = qp + 1, *path = vals + 4L * nv;
+        u64 lh[4];
+        hash_ext2_leaf(hash, vals, lh);
+        if (!verify_path(hash, lh, path, depth, leaf, croot)) return 0;
+        path += 4L * depth;
+        u64 fidx = leaf;
+        for (int rr = 0; rr < nv; rr++) {
+            const u64 *lvl = table + 2 * table_level_offset(table_nv, lr, rr);
+            const ext2 v0 = ld2(vals + 4L * rr), v1 = ld2(vals + 4L * rr + 2);
+            const ext2 cx = e2_sub(ri[rr], e2_base(gl_canon(lvl[2 * fidx])));
+            const ext2 fr = e2_add(v0, e2_mul(e2_mul(cx, e2_sub(v1, v0)), e2_base(gl_canon(lvl[2 * fidx + 1]))));
+            if (rr < nv - 1) {
+                const u64 *nx = vals + 4L * (rr + 1);
+                if (!e2_eq(fr, (fidx & 1) ? ld2(nx + 2) : ld2(nx))) return 0;
+                hash_ext2_leaf(hash, nx, lh);
+                const int plen = nv + lr - 2 - rr;
+                if (!verify_path(hash, lh, path, plen, fidx / 2, froots + 4L * rr)) return 0;
+                path += 4L * plen;
+            } else if (!e2_eq(fr, ld2(final_cw + 2 * fidx))) {
+                return 0;
+            }
+            fidx /= 2;
+        }
+        ext2 cv0 = e2_zero(), cv1 = e2_zero();
+        for (int e = 0; e < ne; e++) {
+            const u64 *ip = indiv + ((size_t)q * ne + e) * (2 + 4 * depth);
+            hash_gl_leaf(hash, ip[0], ip[1], lh);
+            if (!verify_path(hash, lh, ip + 2, depth, leaf, roots + 4L * ev[e].poly)) return 0;
+            cv0 = e2_add(cv0, e2_scale(gl_canon(ip[0]), sc[e]));
+            cv1 = e2_add(cv1, e2_scale(gl_canon(ip[1]), sc[e]));
+        }
+        if (!e2_eq(cv0, ld2(vals)) || !e2_eq(cv1, ld2(vals + 2))) return 0;
     }
     return 1;
 }
@@ -565,6 +974,34 @@ MAM_API int mam_basefold_open_ext2(void *handle, const u64 *point2, const u64 *d
     Commitment *C = (Commitment *)handle;
     if (!C || !point2 || !d_table || !t || !proof || num_queries < 0) return -2;
     return open_ext2(*C, point2, d_table, table_nv, t, num_queries, proof);
+}
+
+MAM_API long mam_basefold_batch_proof_len(int num_vars, int log_rate, int num_queries, int num_evals) {
+    return batch_proof_len(num_vars, log_rate, num_queries, num_evals);
+}
+
+// batch_open_ext2: handles of commitments with equal num_vars, log_rate and hash; points
+// (host) [num_points][nv][2]; claims (host) with poly / point indices into those arrays.
+MAM_API int mam_basefold_batch_open_ext2(void *const *handles, int num_comms, const u64 *points, int num_points,
+                                         const EvalClaim *evals, int num_evals, const u64 *d_table, int table_nv,
+                                         mam_transcript *t, int num_queries, u64 *proof) {
+    if (!handles || num_comms < 1 || !points || !evals || !d_table || !t || !proof || num_queries < 0) return -2;
+    std::vector<const Commitment *> C(num_comms);
+    for (int c = 0; c < num_comms; c++) {
+        C[c] = (const Commitment *)handles[c];
+        if (!C[c]) return -2;
+    }
+    return batch_open_ext2(C, points, num_points, evals, num_evals, d_table, table_nv, t, num_queries, proof);
+}
+
+// Returns 1 if the batch proof verifies, 0 otherwise (host only; table is the host copy).
+MAM_API int mam_basefold_batch_verify_ext2(const u64 *roots, int num_comms, int num_vars, int log_rate, int hash,
+                                           const u64 *points, int num_points, const EvalClaim *evals, int num_evals,
+                                           const u64 *proof, int num_queries, const u64 *table, int table_nv,
+                                           mam_transcript *t) {
+    if (!roots || !points || !evals || !proof || !table || !t) return 0;
+    return batch_verify_ext2(roots, num_comms, num_vars, log_rate, hash, points, num_points, evals, num_evals, proof,
+                             num_queries, table, table_nv, t);
 }
 
 // Returns 1 if the proof verifies, 0 otherwise (host only; table is the host copy).

@@ -682,15 +682,21 @@ MAM_API int partial_eval_ext2_from_gl_ffi(const u64 *d_input, u64 *d_output, con
 // Opening path kernels (fused): oracle of round 0 and fold + next oracle
 // ---------------------------------------------------------------------------
 
-// Round 0: eq (Ext2) and bh (base), natural order; oracle coefficients reduced in-launch.
+// Round 0: eq (Ext2) and bh (base or Ext2), natural order; oracle coefficients reduced in-launch.
+template <bool BH_BASE>
 __global__ __launch_bounds__(MAM_BLOCK) void k_open_oracle0(const u64 *__restrict__ eq, const u64 *__restrict__ bh,
                                                             size_t pairs, u64 *partials, u32 *counter, u64 *result) {
     __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
     u64 acc[6] = {0, 0, 0, 0, 0, 0};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < pairs; i += (size_t)gridDim.x * blockDim.x) {
         ext2 e0 = e2_load(eq, 2 * i), ehi = e2_load(eq, 2 * i + 1);
-        u64 b0 = gl_canon(bh[2 * i]), bhi = gl_canon(bh[2 * i + 1]);
-        mam_oracle_accumulate(acc, e0, e2_sub(ehi, e0), e2_base(b0), e2_base(gl_sub(bhi, b0)));
+        if (BH_BASE) {
+            u64 b0 = gl_canon(bh[2 * i]), bhi = gl_canon(bh[2 * i + 1]);
+            mam_oracle_accumulate(acc, e0, e2_sub(ehi, e0), e2_base(b0), e2_base(gl_sub(bhi, b0)));
+        } else {
+            ext2 b0 = e2_load(bh, 2 * i), bhi = e2_load(bh, 2 * i + 1);
+            mam_oracle_accumulate(acc, e0, e2_sub(ehi, e0), b0, e2_sub(bhi, b0));
+        }
     }
     block_sum<6>(acc, lds);
     grid_finish<6>(acc, lds, partials, counter, result);
@@ -732,9 +738,13 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(const u64 *__res
     grid_finish<6>(acc, lds, partials, counter, result);
 }
 
-int mam_open_oracle0(const u64 *eq, const u64 *bh, size_t pairs, const RedScratch &rs, hipStream_t s) {
+int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, const RedScratch &rs,
+                     hipStream_t s) {
     unsigned g = grid_reduce(pairs);
-    k_open_oracle0<<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.counter, rs.result);
+    if (bh_base)
+        k_open_oracle0<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.counter, rs.result);
+    else
+        k_open_oracle0<false><<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.counter, rs.result);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

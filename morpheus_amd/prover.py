@@ -6,6 +6,8 @@
   BasefoldTable           goldilocks-cuda-rs/src/basefold.rs:281-360
   BasefoldCommitment      goldilocks-cuda-rs/src/basefold.rs:366-863
   verify_ext2             goldilocks-cuda-rs/src/basefold.rs:1759-1925
+  batch_open_ext2         goldilocks-cuda-rs/src/basefold.rs:1202-1525
+  batch_verify_ext2       goldilocks-cuda-rs/src/basefold.rs:1931-2221
 """
 import ctypes
 
@@ -245,3 +247,49 @@ def verify_ext2(root, point2, num_vars, log_rate, hash_kind, proof, num_queries,
     return bool(lib.mam_basefold_verify_ext2(_ptr(root), _ptr(point2), num_vars, log_rate, hash_kind, _ptr(proof),
                                              num_queries, _ptr(table.entries), table.num_vars,
                                              ctypes.byref(transcript.t)))
+
+
+class EvaluationExt2:
+    """One claim: poly[poly] at points[point] = value (basefold.rs:148-160)."""
+
+    def __init__(self, poly, point, value):
+        self.poly, self.point = int(poly), int(point)
+        self.value = [int(value[0]), int(value[1])]
+
+
+def _claims(evals):
+    from ._lib import EvalClaim_t
+    arr = (EvalClaim_t * len(evals))()
+    for i, e in enumerate(evals):
+        arr[i].poly, arr[i].point = e.poly, e.point
+        arr[i].value[0], arr[i].value[1] = e.value
+    return arr
+
+
+def batch_proof_len(num_vars, log_rate, num_queries, num_evals):
+    return int(lib.mam_basefold_batch_proof_len(num_vars, log_rate, num_queries, num_evals))
+
+
+def batch_open_ext2(comms, points, evals, table, transcript, num_queries):
+    """One proof for every claim; comms share num_vars, log_rate and hash (basefold.rs:1202)."""
+    nv, lr = comms[0].num_vars, comms[0].log_rate
+    pts = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1)
+    assert pts.size % (2 * nv) == 0
+    handles = (ctypes.c_void_p * len(comms))(*[c.h for c in comms])
+    proof = np.zeros(batch_proof_len(nv, lr, num_queries, len(evals)), dtype=np.uint64)
+    d_table = table.upload()
+    check(lib.mam_basefold_batch_open_ext2(handles, len(comms), _ptr(pts), pts.size // (2 * nv), _claims(evals),
+                                           len(evals), d_table.ptr, table.num_vars, ctypes.byref(transcript.t),
+                                           num_queries, _ptr(proof)), "mam_basefold_batch_open_ext2")
+    return proof
+
+
+def batch_verify_ext2(roots, points, evals, proof, table, transcript, num_vars, log_rate, num_queries,
+                      hash_kind=HASH_MONOLITH):
+    roots = np.ascontiguousarray(roots, dtype=np.uint64).reshape(-1)
+    pts = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1)
+    proof = np.ascontiguousarray(proof, dtype=np.uint64)
+    return bool(lib.mam_basefold_batch_verify_ext2(_ptr(roots), roots.size // 4, num_vars, log_rate, hash_kind,
+                                                   _ptr(pts), pts.size // (2 * num_vars), _claims(evals), len(evals),
+                                                   _ptr(proof), num_queries, _ptr(table.entries), table.num_vars,
+                                                   ctypes.byref(transcript.t)))

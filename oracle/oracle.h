@@ -103,6 +103,19 @@ void or_basefold_open_ext2(const uint64_t *codeword, const uint64_t *bh_evals, c
                            int nv, int lr, int hash, const uint64_t *point2,
                            const uint64_t *table, int table_nv, or_transcript *t, int nq,
                            uint64_t *proof);
+/* batch_open_ext2 / batch_verify_ext2 (basefold.rs:1202-1525, 1931-2221): num_comms
+ * commitments (codewords, bh_evals, roots[4]) of the same nv, lr opened at num_points Ext2
+ * points (points[num_points][nv][2]) for num_evals claims (poly, point, value[2]). */
+long or_basefold_batch_proof_len(int nv, int lr, int nq, int num_evals);
+void or_basefold_batch_open_ext2(int num_comms, const uint64_t *const *codewords, const uint64_t *const *bh_list,
+                                 const uint64_t *roots, int nv, int lr, int hash, int num_points,
+                                 const uint64_t *points, int num_evals, const int *eval_poly, const int *eval_point,
+                                 const uint64_t *eval_values, const uint64_t *table, int table_nv, or_transcript *t,
+                                 int nq, uint64_t *proof);
+int or_basefold_batch_verify_ext2(int num_comms, const uint64_t *roots, int nv, int lr, int hash, int num_points,
+                                  const uint64_t *points, int num_evals, const int *eval_poly, const int *eval_point,
+                                  const uint64_t *eval_values, const uint64_t *proof, int nq, const uint64_t *table,
+                                  int table_nv, or_transcript *t);
 /* BasefoldVerifier::verify_ext2 (basefold.rs:1759-1925); returns 1 on accept */
 int or_basefold_verify_ext2(const uint64_t *root4, const uint64_t *point2, int nv, int lr, int hash,
                             const uint64_t *proof, int nq, const uint64_t *table, int table_nv,

@@ -49,6 +49,11 @@ _sigs = {
     "or_basefold_proof_len": (cl, [ci, ci, ci]),
     "or_basefold_open_ext2": (None, [u64p, u64p, u64p, ci, ci, ci, u64p, u64p, ci, trp, ci, u64p]),
     "or_basefold_verify_ext2": (ci, [u64p, u64p, ci, ci, ci, u64p, ci, u64p, ci, trp]),
+    "or_basefold_batch_proof_len": (cl, [ci, ci, ci, ci]),
+    "or_basefold_batch_open_ext2": (None, [ci, ctypes.POINTER(u64p), ctypes.POINTER(u64p), u64p, ci, ci, ci, ci, u64p,
+                                           ci, ctypes.POINTER(ci), ctypes.POINTER(ci), u64p, u64p, ci, trp, ci, u64p]),
+    "or_basefold_batch_verify_ext2": (ci, [ci, u64p, ci, ci, ci, ci, u64p, ci, ctypes.POINTER(ci), ctypes.POINTER(ci),
+                                           u64p, u64p, ci, u64p, ci, trp]),
     "or_chacha_block": (None, [ci, ctypes.POINTER(ctypes.c_uint32), ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32),
                                ctypes.POINTER(ctypes.c_uint32)]),
     "or_ajtai_i8_row": (None, [ctypes.c_char_p, ctypes.c_uint32, u64, ctypes.POINTER(ctypes.c_int8)]),
@@ -196,6 +201,47 @@ def basefold_open_ext2(cw, bh, root, nv, lr, hash_kind, point2, table, table_nv,
 def basefold_verify_ext2(root, point2, nv, lr, hash_kind, proof, nq, table, table_nv, t):
     return bool(L.or_basefold_verify_ext2(_p(arr(root)), _p(arr(point2)), nv, lr, hash_kind, _p(arr(proof)), nq,
                                           _p(arr(table)), table_nv, ctypes.byref(t.t)))
+
+
+def _evals_arrays(evals):
+    poly = np.array([e[0] for e in evals], dtype=np.int32)
+    point = np.array([e[1] for e in evals], dtype=np.int32)
+    vals = np.array([[int(e[2][0]), int(e[2][1])] for e in evals], dtype=np.uint64).reshape(-1)
+    return poly, point, vals
+
+
+def _ip(a):
+    return a.ctypes.data_as(ctypes.POINTER(ci))
+
+
+def basefold_batch_proof_len(nv, lr, nq, num_evals):
+    return int(L.or_basefold_batch_proof_len(nv, lr, nq, num_evals))
+
+
+def basefold_batch_open_ext2(comms, nv, lr, hash_kind, points, evals, table, table_nv, t, nq):
+    """comms: list of (codeword, bh_evals, root4); points: [num_points][nv][2];
+    evals: list of (poly, point, value2)."""
+    cws = [arr(c[0]) for c in comms]
+    bhs = [arr(c[1]) for c in comms]
+    roots = arr(np.concatenate([np.asarray(c[2], dtype=np.uint64).ravel() for c in comms]))
+    pts = arr(np.asarray(points, dtype=np.uint64).ravel())
+    poly, point, vals = _evals_arrays(evals)
+    cwp = (u64p * len(comms))(*[_p(c) for c in cws])
+    bhp = (u64p * len(comms))(*[_p(b) for b in bhs])
+    proof = np.zeros(basefold_batch_proof_len(nv, lr, nq, len(evals)), dtype=np.uint64)
+    L.or_basefold_batch_open_ext2(len(comms), cwp, bhp, _p(roots), nv, lr, hash_kind, len(points), _p(pts), len(evals),
+                                  _ip(poly), _ip(point), _p(vals), _p(arr(table)), table_nv, ctypes.byref(t.t), nq,
+                                  _p(proof))
+    return proof
+
+
+def basefold_batch_verify_ext2(roots, nv, lr, hash_kind, points, evals, proof, nq, table, table_nv, t):
+    roots = arr(np.asarray(roots, dtype=np.uint64).ravel())
+    pts = arr(np.asarray(points, dtype=np.uint64).ravel())
+    poly, point, vals = _evals_arrays(evals)
+    return bool(L.or_basefold_batch_verify_ext2(roots.size // 4, _p(roots), nv, lr, hash_kind, len(points), _p(pts),
+                                                len(evals), _ip(poly), _ip(point), _p(vals), _p(arr(proof)), nq,
+                                                _p(arr(table)), table_nv, ctypes.byref(t.t)))
 
 
 def chacha_block(rounds, key_words8, counter, nonce3):
