@@ -76,11 +76,13 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch", "hash", "serve"],
+    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch", "hash", "serve",
+                                                 "batch_open"],
                     default="basefold")
     ap.add_argument("--instances", type=int, default=64, help="instances per batch (batch workload)")
     ap.add_argument("--workers", type=int, default=0, help="batch workers (0: autotune)")
     ap.add_argument("--batch", type=int, default=1, help="witnesses per Ajtai commit")
+    ap.add_argument("--polys", type=int, default=8, help="batch_open: committed polynomials")
     ap.add_argument("--rate", type=float, default=0.0,
                     help="serve: mean arrival rate, proofs/s (0: 1.25 x the one-at-a-time capacity)")
     ap.add_argument("--hbm-budget-gib", type=int, default=96, help="serve: admission budget")
@@ -461,6 +463,96 @@ def bench_batch(args, M):
                          "traffic": None, "perms": perms, "valu_ops_per_perm": VALU_OPS_PER_PERM}}
 
 
+def batch_open_claims(polys):
+    """Claims of the batch-opening workload: poly i at point i % 2, and poly 0 at both points."""
+    return [(i, i % 2) for i in range(polys)] + [(0, 1)]
+
+
+def bench_batch_open(args, M):
+    """Batch opening (batch_open_ext2): `polys` committed polynomials of 2^nv evaluations opened
+    at 2 Ext2 points for polys + 1 claims in one proof, against one open_ext2 per claim."""
+    from morpheus_amd.prover import EvaluationExt2, batch_open_ext2, batch_verify_ext2
+    nv, lr, nq, k = args.nv or 20, 1, args.queries, args.polys
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    rng = np.random.default_rng(17)
+    polys = [gen_inputs(nv, seed=100 + i) for i in range(k)]
+    pts = rng.integers(0, M.P, (2, nv, 2), dtype=np.uint64)
+    comms = [M.BasefoldCommitment.commit(f, nv, lr, M.HASH_MONOLITH) for f in polys]
+    table = M.BasefoldTable.generate(nv, lr, nv, 42)
+    table.upload()
+    claims = batch_open_claims(k)
+    # claimed values: the batch verifier's sum check needs the true evaluations
+    vals = {}
+    for pi, qi in claims:
+        d = M.DeviceBuffer.from_numpy(polys[pi])
+        dr = M.DeviceBuffer.from_numpy(pts[qi].ravel())
+        out = M.DeviceBuffer(1 << nv)  # the first layer writes 2^(nv-1) Ext2 entries
+        M.check(M.lib.partial_eval_ext2_from_gl_ffi(d.ptr, out.ptr, dr.ptr, nv, nv), "partial_eval")
+        vals[(pi, qi)] = out.to_numpy(2)
+    evals = [EvaluationExt2(pi, qi, vals[(pi, qi)]) for pi, qi in claims]
+
+    def batch():
+        return batch_open_ext2(comms, pts, evals, table, M.Transcript(b"bench"), nq)
+
+    def singles():
+        for pi, qi in claims:
+            comms[pi].open_ext2(pts[qi], table, M.Transcript(b"bench"), nq)
+
+    for _ in range(max(args.warmup, 1)):
+        proof = batch()
+        singles()
+    bt, st = [], []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        proof = batch()
+        bt.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        singles()
+        st.append(time.perf_counter() - t0)
+    roots = np.array([c.root for c in comms])
+    ok = batch_verify_ext2(roots, pts, evals, proof, table, M.Transcript(b"bench"), nv, lr, nq)
+    b_s, s_s = float(np.mean(bt)), float(np.mean(st))
+    opened = k * (1 << nv)
+    perms = merkle_perms(nv, lr)  # the combined codeword's tree and the folded trees
+    ach = perms * VALU_OPS_PER_PERM / b_s / 1e12
+    return {"metric": METRIC, "value": opened / b_s, "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": b_s * 1e3, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u64 (Goldilocks, Ext2 challenges)", "data": "synthetic",
+            "config": {"workload": "basefold_batch_open", "num_vars": nv, "log_rate": lr, "polys": k, "points": 2,
+                       "claims": len(claims), "num_queries": nq, "merkle_hash": "monolith-w12",
+                       "parallelism": "single-gpu"},
+            "verified": ok, "proof_words": int(proof.size),
+            "one_open_per_claim": {"ms": s_s * 1e3, "value": opened / s_s},
+            "speedup_vs_one_open_per_claim": s_s / b_s,
+            "roofline": {"kernel": "Monolith Merkle work of the batch opening over its wall time", "bound": "valu",
+                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS, "unit": "Tops/s",
+                         "frac": ach / VALU_PEAK_TOPS, "traffic": None, "perms": perms}}
+
+
+def cpu_baseline_batch_open(nv, polys, nq):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    rng = np.random.default_rng(17)
+    fs = [gen_inputs(nv, seed=100 + i) for i in range(polys)]
+    comms = []
+    for f in fs:
+        cw, bh, tree = O.basefold_commit(f, nv, 1, O.MONOLITH)
+        comms.append((cw, bh, tree[-4:]))
+    pts = rng.integers(0, O.P, (2, nv, 2), dtype=np.uint64)
+    evals = []
+    for pi, qi in batch_open_claims(polys):
+        e = np.zeros((1 << nv, 2), dtype=np.uint64)
+        e[:, 0] = fs[pi]
+        evals.append((pi, qi, O.partial_eval_ext2(e, nv, pts[qi].ravel(), nv).ravel()[:2]))
+    table = O.basefold_table(nv, 1, nv, 42)
+    t0 = time.perf_counter()
+    O.basefold_batch_open_ext2(comms, nv, 1, O.MONOLITH, pts, evals, table, nv, O.Transcript(b"bench"), nq)
+    dt = time.perf_counter() - t0
+    return {"value": polys * (1 << nv) / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+            "sample": f"oracle batch_open_ext2 of {polys} polys x 2^{nv} evaluations, {polys + 1} claims at 2 points, "
+                      f"rate 1/2, Monolith, {nq} queries, single thread, {dt:.2f} s"}
+
+
 SERVE_DIMS = (256, 512, 1024, 2048, 4096)
 
 
@@ -658,6 +750,8 @@ def main():
         out = bench_hash(args, M)
     elif args.workload == "serve":
         out = bench_serve(args, M)
+    elif args.workload == "batch_open":
+        out = bench_batch_open(args, M)
     else:
         out = bench_ring_ajtai(args, M)
     if not args.no_cpu:
@@ -673,6 +767,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline_hash(args.cpu_sample_nv or 18)
         elif args.workload == "serve":
             out["cpu_baseline"] = cpu_baseline_serve(args.cpu_sample_nv or 8)
+        elif args.workload == "batch_open":
+            out["cpu_baseline"] = cpu_baseline_batch_open(args.cpu_sample_nv or 16, args.polys, args.queries)
         else:
             out["cpu_baseline"] = cpu_baseline_ring_ajtai(args.cpu_sample_nv or 13)
         out["cpu_baseline"]["host"] = platform.processor() or platform.machine()

@@ -80,6 +80,8 @@ int poseidon2_merkle_layer_ffi(const uint64_t *d_input, uint64_t *d_output, int 
 
 /* ---- partial evaluation and eq tables (ffi.rs:377-410) ---- */
 int partial_eval_gl_ffi(uint64_t *d_data, const uint64_t *d_r, int log_n, int m);              /* ffi.rs:377 */
+/* d_output holds 2^(log_n - 1) Ext2 entries (the first layer's output, as partial_eval.rs:116-117
+ * allocates it); the result is its first 2^(log_n - m) entries. */
 int partial_eval_ext2_from_gl_ffi(const uint64_t *d_input, uint64_t *d_output, const uint64_t *d_r, int log_n,
                                   int m);                                                      /* ffi.rs:384 */
 /* The result buffer is reported through d_result (always d_buf_a here). */
