@@ -116,6 +116,14 @@ typedef struct {
 int mam_prove_batch(mam_batch_instance *inst, int n, const uint64_t *d_table, int table_nv, int log_rate, int hash,
                     int num_queries, int workers);
 
+/* ---- proof wire format: bincode 1.x bytes of zk-torch-3's SumcheckProof
+ * (zk-torch-3/src/sumcheck/mod.rs:16-23): final_eval (c0, c1 as u64 LE) | u64 rounds |
+ * per round: u64 entries | entries x (c0, c1). Inputs in mam_sumcheck_prove's output layout.
+ * Returns the byte length, -2 on bad arguments or too small a buffer. */
+long mam_sumcheck_proof_bincode_len(int rounds, int entries);
+long mam_sumcheck_proof_bincode(const uint64_t *round_msgs, int rounds, int entries, const uint64_t *final_eval,
+                                uint8_t *out, long cap);
+
 /* ---- batch opening of several commitments at several Ext2 points ----
  * batch_open_ext2 / BasefoldVerifier::batch_verify_ext2 (goldilocks-cuda-rs/src/basefold.rs:
  * 1202-1525, 1931-2221): commitments of equal num_vars, log_rate and hash; points (host)
@@ -209,11 +217,14 @@ int mam_merkle_tree_async(const uint64_t *d_codeword, uint64_t *d_tree, size_t n
  * 1 opening fold+oracle (bytes), 2 commit transform (bytes), 3 sumcheck fold+message
  * (bytes), 4 single leaf-hashing launches over base-field codewords (permutations),
  * 5 int8 Ajtai GEMM launches (bytes), 6 dense ring-Ajtai launches (ring columns x witnesses).
- * collect() fills 7 entries of each array. */
+ * collect() fills 7 entries of each array. enable(1) times every category, enable_mask
+ * only the categories whose bit is set (each timed scope records two events). */
 int mam_profile_enable(int on);
+int mam_profile_enable_mask(unsigned mask);
 /* VALU issue-rate microbenchmark (no reference counterpart; used by bench.py for the
- * measured VALU peak): kind 0 = v_xad_u32 chains, 1 = v_mad_u64_u32 chains; each thread of
- * `blocks` x 256 runs iters x 8 single-instruction steps. *ms = the timed launch. */
+ * measured VALU peak): kind 0 = v_xad_u32 chains, 1 = v_mad_u64_u32, 2 = v_lshl_add_u64,
+ * 3 = v_add_u32; each thread of `blocks` x 256 runs iters x 8 single-instruction steps.
+ * *ms = the timed launch. */
 int mam_valu_probe(int kind, int blocks, int iters, double *ms);
 int mam_profile_collect(double *ms, long *count, double *work);
 

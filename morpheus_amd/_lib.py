@@ -207,10 +207,12 @@ _SIGS = {
     "mam_basefold_batch_proof_len": (ctypes.c_long, [ci, ci, ci, ci]),
     "mam_basefold_batch_open_ext2": (ci, [vp, ci, vp, ci, vp, ci, vp, ci, vp, ci, vp]),
     "mam_basefold_batch_verify_ext2": (ci, [vp, ci, ci, ci, ci, vp, ci, vp, ci, vp, ci, vp, ci, vp]),
+    "mam_sumcheck_proof_bincode_len": (ctypes.c_long, [ci, ci]),
+    "mam_sumcheck_proof_bincode": (ctypes.c_long, [vp, ci, ci, vp, vp, ctypes.c_long]),
     "mam_serve_matmul": (ci, [ctypes.POINTER(ServeRequest_t), ci, vp, ci, ci, ci, ci, vp, ci, ci, u64]),
     "fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "agl_fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
-    "mam_profile_enable": (ci, [ci]),
+    "mam_profile_enable": (ci, [ci]), "mam_profile_enable_mask": (ci, [ctypes.c_uint]),
     "mam_valu_probe": (ci, [ci, ci, ci, ctypes.POINTER(ctypes.c_double)]),
     "mam_release_scratch": (ci, []),
     "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),

@@ -251,10 +251,10 @@ struct ProfRec {
 };
 std::mutex g_prof_mu;
 std::vector<ProfRec> g_prof_recs;
-bool g_prof_on = false;
+unsigned g_prof_mask = 0;  // bit c enables category c
 }  // namespace
 
-bool mam_prof_enabled() { return g_prof_on; }
+bool mam_prof_enabled(int cat) { return cat >= 0 && cat < 32 && ((g_prof_mask >> cat) & 1u); }
 
 void mam_prof_push(int cat, double work, hipEvent_t a, hipEvent_t b) {
     std::lock_guard<std::mutex> g(g_prof_mu);
@@ -273,7 +273,14 @@ static void prof_clear() {
 MAM_API int mam_profile_enable(int on) {
     std::lock_guard<std::mutex> g(g_prof_mu);
     prof_clear();
-    g_prof_on = on != 0;
+    g_prof_mask = on ? ~0u : 0u;
+    return 0;
+}
+
+MAM_API int mam_profile_enable_mask(unsigned mask) {
+    std::lock_guard<std::mutex> g(g_prof_mu);
+    prof_clear();
+    g_prof_mask = mask;
     return 0;
 }
 

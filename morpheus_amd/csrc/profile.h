@@ -17,7 +17,7 @@ enum MamProfCat {
     PROF_NCAT = 7
 };
 
-bool mam_prof_enabled();
+bool mam_prof_enabled(int cat);
 void mam_prof_push(int cat, double work, hipEvent_t a, hipEvent_t b);
 
 struct ProfScope {
@@ -26,7 +26,7 @@ struct ProfScope {
     hipStream_t s;
     hipEvent_t a = nullptr;
     ProfScope(int c, double w, hipStream_t st) : cat(c), work(w), s(st) {
-        if (c >= 0 && mam_prof_enabled() && hipEventCreate(&a) == hipSuccess) {
+        if (mam_prof_enabled(c) && hipEventCreate(&a) == hipSuccess) {
             if (hipEventRecord(a, s) != hipSuccess) {
                 (void)hipEventDestroy(a);
                 a = nullptr;

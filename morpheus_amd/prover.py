@@ -293,3 +293,21 @@ def batch_verify_ext2(roots, points, evals, proof, table, transcript, num_vars, 
                                                    _ptr(pts), pts.size // (2 * num_vars), _claims(evals), len(evals),
                                                    _ptr(proof), num_queries, _ptr(table.entries), table.num_vars,
                                                    ctypes.byref(transcript.t)))
+
+
+def sumcheck_proof_bincode(proof):
+    """bincode 1.x bytes of zk-torch-3's SumcheckProof for a sumcheck_prove result."""
+    msgs = np.ascontiguousarray(proof["round_msgs"], dtype=np.uint64)
+    fe = np.ascontiguousarray(proof["final_eval"], dtype=np.uint64)
+    rounds, entries = msgs.shape[0], msgs.shape[1]
+    n = int(lib.mam_sumcheck_proof_bincode_len(rounds, entries))
+    out = (ctypes.c_uint8 * max(n, 1))()
+    got = lib.mam_sumcheck_proof_bincode(_ptr(msgs), rounds, entries, _ptr(fe), out, n)
+    check(0 if got == n else -1, "mam_sumcheck_proof_bincode")
+    return bytes(out)[:n]
+
+
+def fingerprint(data):
+    """ser_fingerprint (zk-torch-4/src/lib.rs:151-161): SHA-256 of the bytes, first 8 bytes as hex."""
+    import hashlib
+    return hashlib.sha256(data).digest()[:8].hex()

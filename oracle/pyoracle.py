@@ -409,3 +409,18 @@ def matmul_prove(a, b, label, table_entries, table_nv, lr, nq, key=None, rows=0,
         out[name] = basefold_open_ext2(cw, bh, root, nv, lr, hash_kind, pt.ravel(), table_entries, table_nv, t, nq)
     out.update(point=point, claim=claim, round_msgs=sc["round_msgs"], challenges=r_l, final_evals=sc["final_evals"])
     return out
+
+
+def sumcheck_proof_bincode(final_eval, round_msgs):
+    """bincode 1.x (fixed-width little-endian integers, u64 Vec lengths) of zk-torch-3's
+    SumcheckProof { final_eval: Ext2, round_messages: Vec<Vec<Ext2>> }
+    (zk-torch-3/src/sumcheck/mod.rs:16-23), field words as canonical residues."""
+    import struct
+    msgs = np.asarray(round_msgs, dtype=np.uint64)
+    out = struct.pack("<QQ", int(final_eval[0]) % P, int(final_eval[1]) % P)
+    out += struct.pack("<Q", msgs.shape[0])
+    for r in range(msgs.shape[0]):
+        out += struct.pack("<Q", msgs.shape[1])
+        for e in range(msgs.shape[1]):
+            out += struct.pack("<QQ", int(msgs[r, e, 0]) % P, int(msgs[r, e, 1]) % P)
+    return out
