@@ -12,6 +12,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <vector>
+
 #include "common.h"
 #include "sumcheck.h"
 #include "transcript.h"
@@ -164,6 +166,127 @@ int state_final(ScState &S) {
     return mam_publish_wait(S.finals, 2 * S.np, nullptr, 0, S.host, S.st);
 }
 
+// Whole proof with two rounds per device pass (np <= MAM_LOOK_MAX_POLYS): each pass
+// returns the round-r message and the round-(r+1) message as a polynomial in ch_r, so the
+// host draws both challenges from one readback (same transcript as one round at a time).
+int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_msg_len, const uint8_t *l_ch,
+                    long l_ch_len, u64 *round_msgs, u64 *challenges, u64 *final_evals) {
+    const int np = S.np, nv = S.nv, dp1 = np + 1;
+    // The passes are fixed by nv: pass k covers rounds r_k, r_k + 1 (look) or r_k alone.
+    struct Pass {
+        int r;
+        bool look;
+        int folds;
+        size_t items;
+        const u64 *src;
+        size_t src_stride;
+        bool src_base;
+        u64 *out;
+        size_t out_stride;
+    };
+    std::vector<Pass> passes;
+    const u64 *src = S.input;
+    size_t src_stride = S.in_stride, size = S.n;
+    bool src_base = S.input_base;
+    int folds = 0, nxt = 0;
+    for (int r = 0; r < nv;) {
+        Pass p;
+        p.r = r;
+        p.look = nv - r >= 2;
+        p.folds = folds;
+        const size_t m = size >> folds;  // elements per polynomial after the pending folds
+        p.items = m / (p.look ? 4 : 2);
+        p.src = src;
+        p.src_stride = src_stride;
+        p.src_base = src_base;
+        p.out = folds ? S.buf[nxt] : nullptr;
+        p.out_stride = folds ? S.stride[nxt] : 0;
+        passes.push_back(p);
+        if (folds) {
+            src = p.out;
+            src_stride = p.out_stride;
+            src_base = false;
+            size = m;
+            nxt ^= 1;
+        }
+        folds = p.look ? 2 : 1;
+        r += p.look ? 2 : 1;
+    }
+    const int P = (int)passes.size();
+    std::vector<u64> pub(P, 0), tick(P, 0);
+    auto words = [&](const Pass &p) { return 2 * dp1 * (p.look ? 1 + dp1 : 1); };
+    auto enqueue = [&](int k, const u64 *cp, ext2 ca, ext2 cb) -> int {
+        const Pass &p = passes[k];
+        int e = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride, p.items,
+                                   ca, cb, S.s, S.st, cp);
+        if (e) return e;
+        pub[k] = mam_publish_launch(S.s.msg, words(p), nullptr, 0, S.st);
+        return pub[k] ? 0 : -1;
+    };
+    // Pass k + 1 is queued (waiting for its two challenges) before pass k's result is read.
+    auto prequeue = [&](int k) -> int {
+        const u64 *d = nullptr;
+        tick[k] = mam_chal_prequeue(S.st, &d);
+        return tick[k] ? enqueue(k, d, e2_zero(), e2_zero()) : -1;
+    };
+    auto draw = [&](const u64 *m, int r) {
+        for (int c = 0; c < dp1; c++) mam::tr_append_ext2(t, l_msg, l_msg_len, m + 2 * c);
+        u64 ch[2];
+        mam::tr_challenge_ext2(t, l_ch, l_ch_len, ch);
+        challenges[2 * r] = ch[0];
+        challenges[2 * r + 1] = ch[1];
+        return ext2{ch[0], ch[1]};
+    };
+    ext2 ca = e2_zero(), cb = e2_zero();
+    int rc = enqueue(0, nullptr, ca, cb);
+    if (!rc && P > 1) rc = prequeue(1);
+    for (int k = 0; k < P && !rc; k++) {
+        const Pass &p = passes[k];
+        rc = mam_publish_collect(pub[k], words(p), S.host, S.st);
+        if (rc) break;
+        u64 *m0 = round_msgs + (size_t)p.r * dp1 * 2;
+        memcpy(m0, S.host, dp1 * 2 * sizeof(u64));
+        ca = draw(m0, p.r);
+        if (p.look) {
+            // round r + 1: msg(c) = sum_j ch_r^j sym[c][j] (Horner)
+            u64 *m1 = round_msgs + (size_t)(p.r + 1) * dp1 * 2;
+            const u64 *sym = S.host + 2 * dp1;
+            for (int c = 0; c < dp1; c++) {
+                ext2 v{gl_canon(sym[2 * (c * dp1 + np)]), gl_canon(sym[2 * (c * dp1 + np) + 1])};
+                for (int j = np - 1; j >= 0; j--)
+                    v = e2_add(e2_mul(v, ca), ext2{gl_canon(sym[2 * (c * dp1 + j)]), gl_canon(sym[2 * (c * dp1 + j) + 1])});
+                m1[2 * c] = v.c0;
+                m1[2 * c + 1] = v.c1;
+            }
+            cb = draw(m1, p.r + 1);
+        }
+        if (k + 1 < P) {
+            const u64 w[MAM_CHAL_WORDS] = {ca.c0, ca.c1, cb.c0, cb.c1};
+            mam_chal_release(tick[k + 1], w);
+            if (k + 2 < P) rc = prequeue(k + 2);
+        }
+    }
+    if (rc) {
+        mam_chal_release_all();  // never leave a queued wait behind
+        S.finished = true;
+        return rc;
+    }
+    const Pass &last = passes[P - 1];
+    const u64 *fsrc = last.folds ? last.out : last.src;
+    const size_t fstride = last.folds ? last.out_stride : last.src_stride;
+    const bool fbase = last.folds ? false : last.src_base;
+    rc = mam_sc_launch_pass_final(np, fsrc, fstride, fbase, last.look ? 2 : 1, ca, cb, S.finals, S.st);
+    const u64 *st = P > 1 ? mam_chal_status_ptr() : nullptr;
+    if (!rc) rc = mam_publish_wait(S.finals, 2 * np, st, st ? 1 : 0, S.host, S.st);
+    if (!rc && st && S.host[2 * np]) {
+        mam_chal_status_clear(S.st);
+        rc = -1;
+    }
+    if (!rc) memcpy(final_evals, S.host, 2 * np * sizeof(u64));
+    S.finished = true;
+    return rc;
+}
+
 const uint8_t L_NUM_VAR[] = "num_var";
 const uint8_t L_NUM_POLY[] = "num_poly";
 const uint8_t L_ROUND_MSG[] = "round_message";
@@ -190,6 +313,17 @@ extern "C" __attribute__((visibility("default"))) int mam_sumcheck_prove(
     mam::tr_append_u64(t, L_NUM_VAR, 7, (u64)num_vars);
     mam::tr_append_u64(t, L_NUM_POLY, 8, (u64)num_polys);
     const int dp1 = num_polys + 1;
+    if (num_polys <= MAM_LOOK_MAX_POLYS) {
+        rc = prove_lookahead(S, t, L_ROUND_MSG, 13, L_CHALLENGE, 9, round_msgs, challenges, final_evals);
+        if (!rc) {
+            ext2 fe = e2_one();
+            for (int i = 0; i < num_polys; i++) fe = e2_mul(fe, ext2{final_evals[2 * i], final_evals[2 * i + 1]});
+            final_eval[0] = fe.c0;
+            final_eval[1] = fe.c1;
+        }
+        state_free(S);
+        return rc;
+    }
     for (int r = 0; r < num_vars && !rc; r++) {
         rc = state_round_message(S);
         if (rc) break;

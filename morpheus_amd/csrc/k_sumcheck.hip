@@ -290,3 +290,201 @@ int mam_sc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_
         k_sc_fold_final<false><<<1, 64, 0, st>>>(in, in_stride, np, ch, finals);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+// ---------------------------------------------------------------------------
+// Two rounds per device pass (whole-proof prover, up to 3 polynomials).
+//
+// The round-(r+1) message is a polynomial in the round-r challenge: with the round-r
+// arrays grouped in fours (e0, e1, e2, e3), folding at ch gives the pair
+// (e0 + ch (e1 - e0), e2 + ch (e3 - e2)), and its factor at point c is
+//   u(c) + ch v(c),  u(c) = e0 + c (e2 - e0),  v(c) = (e1 - e0) + c ((e3 - e2) - (e1 - e0)).
+// One pass over the round-r arrays therefore yields the round-r message (from the pairs
+// (e0, e1), (e2, e3)) and the coefficients sym[c][k] of ch^k in the round-(r+1) message.
+// The host evaluates the round-(r+1) message itself once it has drawn ch_r, draws ch_{r+1},
+// and the next pass applies both challenges on read (four input elements per element).
+// The transcript is unchanged; the device is visited once per two rounds.
+//
+// k_sc_pass<NP, BASE_IN, FOLDS, LOOK>: FOLDS (0 or 2) pending challenges are applied on
+// read; each item is a group of G = 4 (LOOK) or 2 elements of the folded arrays, which are
+// written to `out` when FOLDS > 0. Reduced words: msg[c] (NP + 1 Ext2), then with LOOK
+// sym[c][k] (c, k = 0..NP).
+// ---------------------------------------------------------------------------
+template <int NP, bool LOOK>
+struct PassWords {
+    static constexpr int value = 2 * (NP + 1) + (LOOK ? 2 * (NP + 1) * (NP + 1) : 0);
+};
+
+template <bool BASE_IN, int FOLDS>
+__device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, ext2 cb) {
+    if (FOLDS == 0) return BASE_IN ? e2_base(gl_canon(p[j])) : e2_load(p, j);
+    ext2 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = BASE_IN ? e2_base(gl_canon(p[4 * j + k])) : e2_load(p, 4 * j + k);
+    const ext2 f0 = e2_add(v[0], e2_mul(ca, e2_sub(v[1], v[0])));
+    const ext2 f1 = e2_add(v[2], e2_mul(ca, e2_sub(v[3], v[2])));
+    return e2_add(f0, e2_mul(cb, e2_sub(f1, f0)));
+}
+
+template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
+__global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ in, size_t in_stride,
+                                                      u64 *__restrict__ out, size_t out_stride, size_t items,
+                                                      ext2 ca, ext2 cb, u64 *partials, u32 *counter, u64 *res,
+                                                      const u64 *cp) {
+    constexpr int G = LOOK ? 4 : 2;
+    if (cp) {
+        ca = ext2{gl_canon(cp[0]), gl_canon(cp[1])};
+        cb = ext2{gl_canon(cp[2]), gl_canon(cp[3])};
+    }
+    constexpr int W = PassWords<NP, LOOK>::value;
+    __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
+    u64 acc[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) acc[k] = 0;
+    for (size_t z = blockIdx.x * (size_t)blockDim.x + threadIdx.x; z < items; z += (size_t)gridDim.x * blockDim.x) {
+        ext2 e[NP][G];
+#pragma unroll
+        for (int i = 0; i < NP; i++) {
+            const u64 *p = in + i * in_stride;
+#pragma unroll
+            for (int t = 0; t < G; t++) e[i][t] = pass_load<BASE_IN, FOLDS>(p, G * z + t, ca, cb);
+            if (FOLDS > 0) {
+#pragma unroll
+                for (int t = 0; t < G; t++) e2_store(out + i * out_stride, G * z + t, e[i][t]);
+            }
+        }
+        // round-r message over the G / 2 pairs of the group
+#pragma unroll
+        for (int q = 0; q < G / 2; q++) {
+            ext2 v[NP], d[NP];
+#pragma unroll
+            for (int i = 0; i < NP; i++) {
+                v[i] = e[i][2 * q];
+                d[i] = e2_sub(e[i][2 * q + 1], e[i][2 * q]);
+            }
+#pragma unroll
+            for (int c = 0; c <= NP; c++) {
+                ext2 prod = v[0];
+#pragma unroll
+                for (int i = 1; i < NP; i++) prod = e2_mul(prod, v[i]);
+                acc[2 * c] = gl_add(acc[2 * c], prod.c0);
+                acc[2 * c + 1] = gl_add(acc[2 * c + 1], prod.c1);
+                if (c < NP) {
+#pragma unroll
+                    for (int i = 0; i < NP; i++) v[i] = e2_add(v[i], d[i]);
+                }
+            }
+        }
+        if (LOOK) {
+            // sym[c][k]: coefficient of ch^k in prod_i (u_i(c) + ch v_i(c))
+            ext2 u[NP], du[NP], w[NP], dw[NP];
+#pragma unroll
+            for (int i = 0; i < NP; i++) {
+                const ext2 d01 = e2_sub(e[i][1], e[i][0]), d23 = e2_sub(e[i][G - 1], e[i][G - 2]);
+                u[i] = e[i][0];
+                du[i] = e2_sub(e[i][G - 2], e[i][0]);
+                w[i] = d01;
+                dw[i] = e2_sub(d23, d01);
+            }
+#pragma unroll
+            for (int c = 0; c <= NP; c++) {
+                ext2 poly[NP + 1];
+                poly[0] = u[0];
+                poly[1] = w[0];
+#pragma unroll
+                for (int i = 1; i < NP; i++) {
+                    poly[i + 1] = e2_mul(poly[i], w[i]);
+#pragma unroll
+                    for (int k = i; k >= 1; k--) poly[k] = e2_add(e2_mul(poly[k], u[i]), e2_mul(poly[k - 1], w[i]));
+                    poly[0] = e2_mul(poly[0], u[i]);
+                }
+#pragma unroll
+                for (int k = 0; k <= NP; k++) {
+                    const int o = 2 * (NP + 1) + 2 * (c * (NP + 1) + k);
+                    acc[o] = gl_add(acc[o], poly[k].c0);
+                    acc[o + 1] = gl_add(acc[o + 1], poly[k].c1);
+                }
+                if (c < NP) {
+#pragma unroll
+                    for (int i = 0; i < NP; i++) {
+                        u[i] = e2_add(u[i], du[i]);
+                        w[i] = e2_add(w[i], dw[i]);
+                    }
+                }
+            }
+        }
+    }
+    block_sum<W>(acc, lds);
+    grid_finish<W>(acc, lds, partials, counter, res);
+}
+
+// The final values: arrays of 2^FOLDS elements folded at the pending challenges (one
+// thread per polynomial).
+template <bool BASE_IN, int FOLDS>
+__global__ void k_sc_pass_final(const u64 *__restrict__ in, size_t in_stride, int np, ext2 ca, ext2 cb,
+                                u64 *finals) {
+    const int i = threadIdx.x;
+    if (i >= np) return;
+    const u64 *p = in + i * in_stride;
+    ext2 v;
+    if (FOLDS == 2) {
+        v = pass_load<BASE_IN, 2>(p, 0, ca, cb);
+    } else {
+        const ext2 a = BASE_IN ? e2_base(gl_canon(p[0])) : e2_load(p, 0);
+        const ext2 b = BASE_IN ? e2_base(gl_canon(p[1])) : e2_load(p, 1);
+        v = e2_add(a, e2_mul(ca, e2_sub(b, a)));
+    }
+    e2_store(finals, i, v);
+}
+
+template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
+static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t items, ext2 ca,
+                         ext2 cb, const ScScratch &s, hipStream_t st, const u64 *cp) {
+    unsigned g = grid_reduce(items);
+    ProfScope prof(PROF_SUMCHECK,
+                   (double)items * NP * ((LOOK ? 4 : 2) * (FOLDS ? 4 : 1) * (BASE_IN ? 8.0 : 16.0) +
+                                         (FOLDS ? (LOOK ? 4 : 2) * 16.0 : 0.0)),
+                   st);
+    k_sc_pass<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca, cb,
+                                                                 s.partials, s.counter, s.msg, cp);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int NP>
+static int launch_pass_np(const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
+                          size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
+                          const u64 *cp) {
+#define MAM_PASS(B, F, L)                                                                            \
+    if (base_in == B && folds == F && look == L)                                                     \
+        return launch_pass_t<NP, B, F, L>(in, in_stride, out, out_stride, items, ca, cb, s, st, cp);
+    MAM_PASS(true, 0, true)
+    MAM_PASS(true, 0, false)
+    MAM_PASS(true, 2, true)
+    MAM_PASS(true, 2, false)
+    MAM_PASS(false, 0, true)
+    MAM_PASS(false, 0, false)
+    MAM_PASS(false, 2, true)
+    MAM_PASS(false, 2, false)
+#undef MAM_PASS
+    return -2;
+}
+
+int mam_sc_launch_pass(int np, const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
+                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
+                       const u64 *cp) {
+    switch (np) {
+        case 1: return launch_pass_np<1>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, cp);
+        case 2: return launch_pass_np<2>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, cp);
+        case 3: return launch_pass_np<3>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, cp);
+        default: return -2;
+    }
+}
+
+int mam_sc_launch_pass_final(int np, const u64 *in, size_t in_stride, bool base_in, int folds, ext2 ca, ext2 cb,
+                             u64 *finals, hipStream_t st) {
+    if (np < 1 || np > MAM_MAX_POLYS || (folds != 1 && folds != 2)) return -2;
+    if (base_in && folds == 2) k_sc_pass_final<true, 2><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
+    else if (base_in) k_sc_pass_final<true, 1><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
+    else if (folds == 2) k_sc_pass_final<false, 2><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
+    else k_sc_pass_final<false, 1><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}

@@ -257,7 +257,8 @@ def test_codeword_folds_and_sumcheck_pieces():
 
 
 @pytest.mark.parametrize("np_,nv,base", [(1, 1, True), (2, 2, True), (2, 9, True), (3, 8, False), (4, 6, True),
-                                         (2, 14, False), (8, 5, True)])
+                                         (2, 14, False), (8, 5, True), (1, 6, False), (1, 7, True), (2, 3, False),
+                                         (3, 1, False), (3, 2, True), (3, 7, True), (3, 11, False), (2, 17, True)])
 def test_sumcheck_prove_matches_oracle(np_, nv, base):
     say("sumcheck", np_, nv, base)
     rng = np.random.default_rng(800 + 10 * np_ + nv)
