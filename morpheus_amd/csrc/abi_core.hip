@@ -320,21 +320,13 @@ struct ThreadArenas {
     Arena host;
     u64 *mailbox = nullptr;  // fine-grained pinned host words, MAM_MAILBOX_WORDS + 1 (flag last)
     u64 seq = 0;
-    // pre-queued challenges: pinned [CHAL_SLOTS][MAM_CHAL_WORDS] words + release flag;
-    // device [CHAL_SLOTS][MAM_CHAL_WORDS] words + status word
-    u64 *chal_host = nullptr;
-    u64 *chal_dev = nullptr;
-    u64 chal_seq = 0, chal_released = 0;
     ~ThreadArenas() {
         for (auto &a : dev)
             if (a.p) (void)hipFree(a.p);
         if (host.p) (void)hipHostFree(host.p);
         if (mailbox) (void)hipHostFree(mailbox);
-        if (chal_host) (void)hipHostFree(chal_host);
-        if (chal_dev) (void)hipFree(chal_dev);
     }
 };
-constexpr int CHAL_SLOTS = 64;
 thread_local ThreadArenas t_arenas;
 }  // namespace
 
@@ -401,25 +393,21 @@ __global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, 
     if (t == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-u64 mam_publish_launch(const u64 *a, int na, const u64 *b, int nb, hipStream_t s) {
-    if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS) return 0;
+int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s) {
+    if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS) return -2;
     if (!t_arenas.mailbox) {
         if (hipHostMalloc((void **)&t_arenas.mailbox, (MAM_MAILBOX_WORDS + 1) * sizeof(u64),
                           hipHostMallocCoherent) != hipSuccess) {
             t_arenas.mailbox = nullptr;
-            return 0;
+            return -1;
         }
         t_arenas.mailbox[MAM_MAILBOX_WORDS] = 0;
         t_arenas.seq = 0;
     }
-    const u64 seq = ++t_arenas.seq;
-    k_publish<<<1, 64, 0, s>>>(a, na, b, nb, t_arenas.mailbox, seq);
-    return hipGetLastError() == hipSuccess ? seq : 0;
-}
-
-int mam_publish_collect(u64 seq, int words, u64 *out, hipStream_t s) {
     u64 *box = t_arenas.mailbox;
-    if (!seq || !box || words < 0 || words > MAM_MAILBOX_WORDS) return -1;
+    const u64 seq = ++t_arenas.seq;
+    k_publish<<<1, 64, 0, s>>>(a, na, b, nb, box, seq);
+    if (hipGetLastError() != hipSuccess) return -1;
     // Spin on the flag; every few thousand polls ask the stream whether it failed or drained.
     for (unsigned it = 1;; it++) {
         if (__atomic_load_n(&box[MAM_MAILBOX_WORDS], __ATOMIC_ACQUIRE) == seq) break;
@@ -433,107 +421,6 @@ int mam_publish_collect(u64 seq, int words, u64 *out, hipStream_t s) {
         }
         __builtin_ia32_pause();
     }
-    for (int i = 0; i < words; i++) out[i] = box[i];
-    return 0;
-}
-
-int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s) {
-    if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS) return -2;
-    const u64 seq = mam_publish_launch(a, na, b, nb, s);
-    return seq ? mam_publish_collect(seq, na + nb, out, s) : -1;
-}
-
-// One wave: lane 0 polls the release flag (system scope) until it reaches `ticket` or the
-// poll budget runs out (then the status word is set and zeros are copied), then the words
-// of the ticket's slot are copied to device memory.
-__global__ void k_chal_fetch(const u64 *flag, const u64 *slot, u64 ticket, u64 *dst, u64 *status,
-                             unsigned long long max_polls) {
-    __shared__ u64 w[MAM_CHAL_WORDS];
-    if (threadIdx.x == 0) {
-        int good = 0;
-        for (unsigned long long it = 0; it < max_polls; it++) {
-            if (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) >= ticket) {
-                good = 1;
-                break;
-            }
-            __builtin_amdgcn_s_sleep(2);
-        }
-        for (int i = 0; i < MAM_CHAL_WORDS; i++)
-            w[i] = good ? __hip_atomic_load(&slot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) : 0;
-        if (!good) *status = 1;
-    }
-    __syncthreads();
-    if (threadIdx.x < MAM_CHAL_WORDS) dst[threadIdx.x] = w[threadIdx.x];
-}
-
-u64 mam_chal_prequeue(hipStream_t s, const u64 **d_words) {
-    ThreadArenas &A = t_arenas;
-    if (!A.chal_host) {
-        if (hipHostMalloc((void **)&A.chal_host, (CHAL_SLOTS * MAM_CHAL_WORDS + 1) * sizeof(u64),
-                          hipHostMallocCoherent) != hipSuccess) {
-            A.chal_host = nullptr;
-            return 0;
-        }
-        memset(A.chal_host, 0, (CHAL_SLOTS * MAM_CHAL_WORDS + 1) * sizeof(u64));
-        A.chal_seq = A.chal_released = 0;
-    }
-    if (!A.chal_dev) {
-        if (hipMalloc((void **)&A.chal_dev, (CHAL_SLOTS * MAM_CHAL_WORDS + 1) * sizeof(u64)) != hipSuccess) {
-            A.chal_dev = nullptr;
-            return 0;
-        }
-        if (hipMemsetAsync(A.chal_dev, 0, (CHAL_SLOTS * MAM_CHAL_WORDS + 1) * sizeof(u64), s) != hipSuccess) return 0;
-    }
-    if (A.chal_seq - A.chal_released >= CHAL_SLOTS / 2) return 0;  // too many outstanding tickets
-    const u64 ticket = ++A.chal_seq;
-    const int slot = (int)(ticket % CHAL_SLOTS);
-    u64 *dst = A.chal_dev + slot * MAM_CHAL_WORDS;
-    // ~2^23 polls of a PCIe read plus a short sleep: several seconds, far beyond any host draw
-    k_chal_fetch<<<1, 64, 0, s>>>(A.chal_host + CHAL_SLOTS * MAM_CHAL_WORDS, A.chal_host + slot * MAM_CHAL_WORDS,
-                                  ticket, dst, A.chal_dev + CHAL_SLOTS * MAM_CHAL_WORDS, 1ull << 23);
-    if (hipGetLastError() != hipSuccess) {
-        A.chal_released = A.chal_seq;  // the kernel was not queued: nothing waits for this ticket
-        return 0;
-    }
-    *d_words = dst;
-    return ticket;
-}
-
-void mam_chal_release(u64 ticket, const u64 *words) {
-    ThreadArenas &A = t_arenas;
-    if (!A.chal_host || !ticket || ticket <= A.chal_released) return;
-    u64 *slot = A.chal_host + (ticket % CHAL_SLOTS) * MAM_CHAL_WORDS;
-    for (int i = 0; i < MAM_CHAL_WORDS; i++) slot[i] = words ? words[i] : 0;
-    __atomic_store_n(A.chal_host + CHAL_SLOTS * MAM_CHAL_WORDS, ticket, __ATOMIC_RELEASE);
-    A.chal_released = ticket;
-}
-
-void mam_chal_release_all() {
-    ThreadArenas &A = t_arenas;
-    if (!A.chal_host || A.chal_released == A.chal_seq) return;
-    __atomic_store_n(A.chal_host + CHAL_SLOTS * MAM_CHAL_WORDS, A.chal_seq, __ATOMIC_RELEASE);
-    A.chal_released = A.chal_seq;
-}
-
-const u64 *mam_chal_status_ptr() {
-    return t_arenas.chal_dev ? t_arenas.chal_dev + CHAL_SLOTS * MAM_CHAL_WORDS : nullptr;
-}
-
-void mam_chal_status_clear(hipStream_t s) {
-    if (t_arenas.chal_dev) (void)hipMemsetAsync(t_arenas.chal_dev + CHAL_SLOTS * MAM_CHAL_WORDS, 0, sizeof(u64), s);
-}
-
-int mam_chal_status(hipStream_t s) {
-    ThreadArenas &A = t_arenas;
-    if (!A.chal_dev) return 0;
-    u64 st = 0;
-    u64 *p = A.chal_dev + CHAL_SLOTS * MAM_CHAL_WORDS;
-    if (hipMemcpyAsync(&st, p, sizeof(u64), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess)
-        return -1;
-    if (st) {
-        (void)hipMemsetAsync(p, 0, sizeof(u64), s);
-        return -1;
-    }
+    for (int i = 0; i < na + nb; i++) out[i] = box[i];
     return 0;
 }

@@ -35,17 +35,13 @@ MAM_HD void mam_oracle_accumulate(u64 *acc, ext2 e0, ext2 e1, ext2 b0, ext2 b1) 
 
 int mam_bitrev(const u64 *src, u64 *dst, int nb, int words, hipStream_t s);
 int mam_commit_codeword(const u64 *d_evals, u64 *d_bh_evals, u64 *d_codeword, int nv, int lr, hipStream_t s);
-// Functions taking a challenge by value also take an optional device pointer to it (two
-// words); when the pointer is set the kernels read the challenge there at run time, so the
-// launch can be queued before the challenge is known (mam_chal_prequeue).
-int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s,
-                const u64 *alpha_ptr = nullptr);
+int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s);
 int mam_eq_table(const u64 *d_r, int nv, u64 *d_out, bool ext, hipStream_t s);
 int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, const RedScratch &rs,
                      hipStream_t s);
 int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,
-                         size_t half_out, const RedScratch &rs, hipStream_t s, const u64 *ch_ptr = nullptr);
+                         size_t half_out, const RedScratch &rs, hipStream_t s);
 // Fold a codeword (base or Ext2) at alpha into `fold_out` (Ext2) and build the Merkle tree
 // over the folded codeword in the same pass (leaf i = folded elements 2i, 2i+1).
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
-                      size_t folded_len, int hash, hipStream_t s, const u64 *alpha_ptr = nullptr);
+                      size_t folded_len, int hash, hipStream_t s);

@@ -75,27 +75,3 @@ u64 *mam_host_arena(size_t words);
 // synchronization. Returns 0, -1 (stream error) or -2 (na + nb > MAM_MAILBOX_WORDS).
 static constexpr int MAM_MAILBOX_WORDS = 63;
 int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s);
-// The same in two halves: launch returns a ticket (0 on failure), collect spins for it.
-// Between the two the caller may queue more work (which must not publish before the
-// ticket has been collected).
-u64 mam_publish_launch(const u64 *a, int na, const u64 *b, int nb, hipStream_t s);
-int mam_publish_collect(u64 ticket, int words, u64 *out, hipStream_t s);
-
-// Pre-queued challenges. mam_chal_prequeue queues on stream s a one-wave kernel that waits
-// (bounded spin on a pinned flag) until the host releases the ticket it returns, then copies
-// the MAM_CHAL_WORDS released words to device memory at *d_words; work queued after it reads
-// the challenge there. The host queues a round's kernels before it has drawn the round's
-// challenge, so they start as soon as it is released instead of after a launch from an idle
-// queue. Every ticket must be released (mam_chal_release) or abandoned
-// (mam_chal_release_all); a wait that runs out (~10 s) marks the thread's status, which
-// mam_chal_status reads (a synchronizing copy) and clears.
-static constexpr int MAM_CHAL_WORDS = 4;
-u64 mam_chal_prequeue(hipStream_t s, const u64 **d_words);
-void mam_chal_release(u64 ticket, const u64 *words);
-void mam_chal_release_all();
-int mam_chal_status(hipStream_t s);
-// Device address of the status word (nonzero after a wait ran out; nullptr before the first
-// prequeue), for callers that fold it into a readback they do anyway; mam_chal_status_clear
-// queues its reset.
-const u64 *mam_chal_status_ptr();
-void mam_chal_status_clear(hipStream_t s);

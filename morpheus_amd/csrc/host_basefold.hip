@@ -258,71 +258,42 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     const u64 *eq_cur = S.eq, *bh_cur = S.bh;
     bool bh_base = S.bh_base;
     size_t sc_size = n;
-    size_t flen = cw_len / 2;  // folded codeword length of the next round to queue
+    size_t flen = cw_len / 2;  // folded codeword length of this round
     const u64 *cw_in = S.cw;
     bool cw_base = S.cw_base;
-    // Round r's device work: fold eq / bh and the codeword at ch_r, the folded codeword's
-    // tree, then publish (next oracle, root). Rounds after the first are queued before their
-    // challenge is drawn (mam_chal_prequeue) and read it from device memory.
-    const int R = nv - 1;
-    std::vector<u64> pub(R > 0 ? R : 1, 0), tick(R > 0 ? R : 1, 0);
-    auto enqueue = [&](int round, ext2 ch, const u64 *chp) -> int {
+    for (int round = 0; round < nv - 1 && !rc; round++) {
+        ext2 ch = sample_ext2(t);
         const int pp = round & 1;
-        int e = mam_open_fold_oracle(eq_cur, bh_cur, bh_base, ch, eqp[pp], bhp[pp], sc_size / 4, rs, s, chp);
-        if (e) return e;
+        rc = mam_open_fold_oracle(eq_cur, bh_cur, bh_base, ch, eqp[pp], bhp[pp], sc_size / 4, rs, s);
+        if (rc) break;
         eq_cur = eqp[pp];
         bh_cur = bhp[pp];
         bh_base = false;
         sc_size /= 2;
         const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, round);
-        e = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s, chp);
-        if (e) return e;
-        pub[round] = mam_publish_launch(rs.result, 6, ftree[round] + mam_merkle_root_offset(flen / 2), 4, s);
-        if (!pub[round]) return -1;
-        cw_in = fcw[round];
-        cw_base = false;
-        flen /= 2;
-        return 0;
-    };
-    auto prequeue = [&](int round) -> int {
-        const u64 *d = nullptr;
-        tick[round] = mam_chal_prequeue(s, &d);
-        return tick[round] ? enqueue(round, e2_zero(), d) : -1;
-    };
-    if (!rc && R > 0) rc = enqueue(0, sample_ext2(t), nullptr);
-    if (!rc && R > 1) rc = prequeue(1);
-    for (int round = 0; round < R && !rc; round++) {
-        rc = mam_publish_collect(pub[round], 10, h, s);
+        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s);
         if (rc) break;
+        const size_t leaves = flen / 2;
+        // oracle (3 Ext2) and the new root in one mailbox round trip
+        if (mam_publish_wait(rs.result, 6, ftree[round] + mam_merkle_root_offset(leaves), 4, h, s)) {
+            rc = -1;
+            break;
+        }
         u64 *o = o_oracles + 6L * (round + 1);
         memcpy(o, h, 6 * sizeof(u64));
         for (int k = 0; k < 3; k++) observe_ext2(t, o + 2 * k);
         memcpy(o_roots + 4L * round, h + 6, 4 * sizeof(u64));
         observe_hash(t, o_roots + 4L * round);
-        if (round + 1 < R) {
-            const ext2 ch = sample_ext2(t);
-            const u64 w[MAM_CHAL_WORDS] = {ch.c0, ch.c1, 0, 0};
-            mam_chal_release(tick[round + 1], w);
-            if (round + 2 < R) rc = prequeue(round + 2);
-        }
+        cw_in = fcw[round];
+        cw_base = false;
+        flen /= 2;
     }
-    if (rc) mam_chal_release_all();  // never leave a queued wait behind
     if (!rc) {
-        // last fold, no tree; the readback carries the pre-queued waits' status word
+        // last fold, no tree
         ext2 ch = sample_ext2(t);
         const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, nv - 1);
         rc = mam_cw_fold(cw_in, lvl, ch, fcw[nv - 1], flen, cw_base, s);
-        const u64 *st = R > 1 ? mam_chal_status_ptr() : nullptr;
-        if (!rc && st && 2 * flen + 1 <= (size_t)MAM_MAILBOX_WORDS) {
-            rc = mam_publish_wait(fcw[nv - 1], (int)(2 * flen), st, 1, h, s);
-            if (!rc && h[2 * flen]) {
-                mam_chal_status_clear(s);
-                rc = -1;
-            }
-        } else {
-            if (!rc) rc = read_back(h, fcw[nv - 1], 2 * flen, s);
-            if (!rc && st) rc = mam_chal_status(s);
-        }
+        if (!rc) rc = read_back(h, fcw[nv - 1], 2 * flen, s);
         if (!rc) memcpy(o_final, h, 2 * flen * sizeof(u64));
     }
     if (!rc && nq > 0) {

@@ -213,22 +213,7 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
         r += p.look ? 2 : 1;
     }
     const int P = (int)passes.size();
-    std::vector<u64> pub(P, 0), tick(P, 0);
     auto words = [&](const Pass &p) { return 2 * dp1 * (p.look ? 1 + dp1 : 1); };
-    auto enqueue = [&](int k, const u64 *cp, ext2 ca, ext2 cb) -> int {
-        const Pass &p = passes[k];
-        int e = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride, p.items,
-                                   ca, cb, S.s, S.st, cp);
-        if (e) return e;
-        pub[k] = mam_publish_launch(S.s.msg, words(p), nullptr, 0, S.st);
-        return pub[k] ? 0 : -1;
-    };
-    // Pass k + 1 is queued (waiting for its two challenges) before pass k's result is read.
-    auto prequeue = [&](int k) -> int {
-        const u64 *d = nullptr;
-        tick[k] = mam_chal_prequeue(S.st, &d);
-        return tick[k] ? enqueue(k, d, e2_zero(), e2_zero()) : -1;
-    };
     auto draw = [&](const u64 *m, int r) {
         for (int c = 0; c < dp1; c++) mam::tr_append_ext2(t, l_msg, l_msg_len, m + 2 * c);
         u64 ch[2];
@@ -238,11 +223,12 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
         return ext2{ch[0], ch[1]};
     };
     ext2 ca = e2_zero(), cb = e2_zero();
-    int rc = enqueue(0, nullptr, ca, cb);
-    if (!rc && P > 1) rc = prequeue(1);
+    int rc = 0;
     for (int k = 0; k < P && !rc; k++) {
         const Pass &p = passes[k];
-        rc = mam_publish_collect(pub[k], words(p), S.host, S.st);
+        rc = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride, p.items, ca,
+                                cb, S.s, S.st);
+        if (!rc) rc = mam_publish_wait(S.s.msg, words(p), nullptr, 0, S.host, S.st);
         if (rc) break;
         u64 *m0 = round_msgs + (size_t)p.r * dp1 * 2;
         memcpy(m0, S.host, dp1 * 2 * sizeof(u64));
@@ -260,14 +246,8 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
             }
             cb = draw(m1, p.r + 1);
         }
-        if (k + 1 < P) {
-            const u64 w[MAM_CHAL_WORDS] = {ca.c0, ca.c1, cb.c0, cb.c1};
-            mam_chal_release(tick[k + 1], w);
-            if (k + 2 < P) rc = prequeue(k + 2);
-        }
     }
     if (rc) {
-        mam_chal_release_all();  // never leave a queued wait behind
         S.finished = true;
         return rc;
     }
@@ -276,12 +256,7 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
     const size_t fstride = last.folds ? last.out_stride : last.src_stride;
     const bool fbase = last.folds ? false : last.src_base;
     rc = mam_sc_launch_pass_final(np, fsrc, fstride, fbase, last.look ? 2 : 1, ca, cb, S.finals, S.st);
-    const u64 *st = P > 1 ? mam_chal_status_ptr() : nullptr;
-    if (!rc) rc = mam_publish_wait(S.finals, 2 * np, st, st ? 1 : 0, S.host, S.st);
-    if (!rc && st && S.host[2 * np]) {
-        mam_chal_status_clear(S.st);
-        rc = -1;
-    }
+    if (!rc) rc = mam_publish_wait(S.finals, 2 * np, nullptr, 0, S.host, S.st);
     if (!rc) memcpy(final_evals, S.host, 2 * np * sizeof(u64));
     S.finished = true;
     return rc;

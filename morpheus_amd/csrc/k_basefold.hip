@@ -283,10 +283,9 @@ int mam_commit_codeword(const u64 *d_evals, u64 *d_bh_evals, u64 *d_codeword, in
 // ---------------------------------------------------------------------------
 template <int KIND>  // 0: base cw, base alpha; 1: base cw, ext2 alpha; 2: ext2 cw, ext2 alpha
 __global__ void k_cw_fold(const u64 *__restrict__ cw, const u64 *__restrict__ table, ext2 alpha,
-                          u64 *__restrict__ out, size_t pairs, const u64 *alpha_ptr = nullptr) {
+                          u64 *__restrict__ out, size_t pairs) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= pairs) return;
-    if (alpha_ptr) alpha = ext2{gl_canon(alpha_ptr[0]), gl_canon(alpha_ptr[1])};
     const u64 x0 = gl_canon(table[2 * i]), w = gl_canon(table[2 * i + 1]);
     if (KIND == 0) {
         u64 v0 = gl_canon(cw[2 * i]), v1 = gl_canon(cw[2 * i + 1]);
@@ -330,12 +329,11 @@ MAM_API int basefold_fold_ext2_ffi(const u64 *d_codeword, const u64 *d_table, u6
     return mam_launch_status();
 }
 
-int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s,
-                const u64 *alpha_ptr) {
+int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s) {
     if (base_cw)
-        k_cw_fold<1><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs, alpha_ptr);
+        k_cw_fold<1><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs);
     else
-        k_cw_fold<2><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs, alpha_ptr);
+        k_cw_fold<2><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -710,11 +708,9 @@ template <bool BH_BASE>
 __global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(const u64 *__restrict__ eq, const u64 *__restrict__ bh,
                                                                 ext2 ch, u64 *__restrict__ eq_out,
                                                                 u64 *__restrict__ bh_out, size_t half_out,
-                                                                u64 *partials, u32 *counter, u64 *result,
-                                                                const u64 *ch_ptr) {
+                                                                u64 *partials, u32 *counter, u64 *result) {
     __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
     u64 acc[6] = {0, 0, 0, 0, 0, 0};
-    if (ch_ptr) ch = ext2{gl_canon(ch_ptr[0]), gl_canon(ch_ptr[1])};
     for (size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x; y < half_out; y += (size_t)gridDim.x * blockDim.x) {
         ext2 q0 = e2_load(eq, 4 * y), q1 = e2_load(eq, 4 * y + 1), q2 = e2_load(eq, 4 * y + 2),
              q3 = e2_load(eq, 4 * y + 3);
@@ -753,14 +749,14 @@ int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, c
 }
 
 int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,
-                         size_t half_out, const RedScratch &rs, hipStream_t s, const u64 *ch_ptr) {
+                         size_t half_out, const RedScratch &rs, hipStream_t s) {
     unsigned g = grid_reduce(half_out);
     ProfScope prof(PROF_OPEN_SUMCHECK, (double)half_out * (64.0 + (bh_base ? 32.0 : 64.0) + 128.0), s);
     if (bh_base)
         k_open_fold_oracle<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, ch, eq_out, bh_out, half_out, rs.partials, rs.counter,
-                                                         rs.result, ch_ptr);
+                                                         rs.result);
     else
         k_open_fold_oracle<false><<<g, MAM_BLOCK, 0, s>>>(eq, bh, ch, eq_out, bh_out, half_out, rs.partials,
-                                                          rs.counter, rs.result, ch_ptr);
+                                                          rs.counter, rs.result);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -41,15 +41,10 @@ static constexpr int TREE_BLOCK = 256;
 enum { SRC_GL = 0, SRC_EXT2 = 1, SRC_NODES = 2, SRC_FOLD_GL = 3, SRC_FOLD_EXT2 = 4 };
 
 struct FoldSrc {
-    const u64 *table;      // (x0, w) pairs of this fold level
+    const u64 *table;  // (x0, w) pairs of this fold level
     ext2 alpha;
-    u64 *out;              // folded codeword (Ext2)
-    const u64 *alpha_ptr;  // when set, alpha is read here at run time
+    u64 *out;          // folded codeword (Ext2)
 };
-
-__device__ __forceinline__ ext2 fold_alpha(const FoldSrc &fs) {
-    return fs.alpha_ptr ? ext2{gl_canon(fs.alpha_ptr[0]), gl_canon(fs.alpha_ptr[1])} : fs.alpha;
-}
 
 // Writes `nlev` consecutive levels. The first written level has `n_first` nodes and
 // starts at digest index `first_off` of `tree`; each next level follows the previous one.
@@ -83,7 +78,7 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_levels(const u64 *__restric
                     v1 = e2_load(src, 2 * k + 1);
                 }
                 const u64 x0 = gl_canon(fs.table[2 * k]), w = gl_canon(fs.table[2 * k + 1]);
-                f[h] = mam_fold_value(v0, v1, e2_sub(fold_alpha(fs), e2_base(x0)), w);
+                f[h] = mam_fold_value(v0, v1, e2_sub(fs.alpha, e2_base(x0)), w);
                 e2_store(fs.out, k, f[h]);
             }
             u64 ab[4] = {f[0].c0, f[0].c1, f[1].c0, f[1].c1};
@@ -156,7 +151,7 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
                 v1 = e2_load(src, 2 * k + 1);
             }
             const u64 x0 = gl_canon(fs.table[2 * k]), w = gl_canon(fs.table[2 * k + 1]);
-            const ext2 f = mam_fold_value(v0, v1, e2_sub(fold_alpha(fs), e2_base(x0)), w);
+            const ext2 f = mam_fold_value(v0, v1, e2_sub(fs.alpha, e2_base(x0)), w);
             x = (j & 1) ? f.c1 : f.c0;
             fs.out[2 * k + (j & 1)] = x;
         }
@@ -284,7 +279,7 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
 }
 
 int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, int ext2_leaves, hipStream_t s) {
-    FoldSrc fs = {nullptr, e2_zero(), nullptr, nullptr};
+    FoldSrc fs = {nullptr, e2_zero(), nullptr};
     int src = ext2_leaves ? SRC_EXT2 : SRC_GL;
     if (hash == MAM_HASH_MONOLITH) return build_tree<MonoHash>(d_cw, d_tree, num_leaves, src, fs, s);
     if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(d_cw, d_tree, num_leaves, src, fs, s);
@@ -292,9 +287,9 @@ int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, 
 }
 
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
-                      size_t folded_len, int hash, hipStream_t s, const u64 *alpha_ptr) {
+                      size_t folded_len, int hash, hipStream_t s) {
     if (folded_len < 2) return -2;
-    FoldSrc fs = {table, alpha, fold_out, alpha_ptr};
+    FoldSrc fs = {table, alpha, fold_out};
     int src = base_cw ? SRC_FOLD_GL : SRC_FOLD_EXT2;
     if (hash == MAM_HASH_MONOLITH) return build_tree<MonoHash>(cw, tree, folded_len / 2, src, fs, s);
     if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(cw, tree, folded_len / 2, src, fs, s);

@@ -328,13 +328,8 @@ __device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, ext2 
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ in, size_t in_stride,
                                                       u64 *__restrict__ out, size_t out_stride, size_t items,
-                                                      ext2 ca, ext2 cb, u64 *partials, u32 *counter, u64 *res,
-                                                      const u64 *cp) {
+                                                      ext2 ca, ext2 cb, u64 *partials, u32 *counter, u64 *res) {
     constexpr int G = LOOK ? 4 : 2;
-    if (cp) {
-        ca = ext2{gl_canon(cp[0]), gl_canon(cp[1])};
-        cb = ext2{gl_canon(cp[2]), gl_canon(cp[3])};
-    }
     constexpr int W = PassWords<NP, LOOK>::value;
     __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
     u64 acc[W];
@@ -438,24 +433,23 @@ __global__ void k_sc_pass_final(const u64 *__restrict__ in, size_t in_stride, in
 
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t items, ext2 ca,
-                         ext2 cb, const ScScratch &s, hipStream_t st, const u64 *cp) {
+                         ext2 cb, const ScScratch &s, hipStream_t st) {
     unsigned g = grid_reduce(items);
     ProfScope prof(PROF_SUMCHECK,
                    (double)items * NP * ((LOOK ? 4 : 2) * (FOLDS ? 4 : 1) * (BASE_IN ? 8.0 : 16.0) +
                                          (FOLDS ? (LOOK ? 4 : 2) * 16.0 : 0.0)),
                    st);
     k_sc_pass<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca, cb,
-                                                                 s.partials, s.counter, s.msg, cp);
+                                                                 s.partials, s.counter, s.msg);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int NP>
 static int launch_pass_np(const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
-                          size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
-                          const u64 *cp) {
+                          size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st) {
 #define MAM_PASS(B, F, L)                                                                            \
     if (base_in == B && folds == F && look == L)                                                     \
-        return launch_pass_t<NP, B, F, L>(in, in_stride, out, out_stride, items, ca, cb, s, st, cp);
+        return launch_pass_t<NP, B, F, L>(in, in_stride, out, out_stride, items, ca, cb, s, st);
     MAM_PASS(true, 0, true)
     MAM_PASS(true, 0, false)
     MAM_PASS(true, 2, true)
@@ -469,12 +463,11 @@ static int launch_pass_np(const u64 *in, size_t in_stride, bool base_in, int fol
 }
 
 int mam_sc_launch_pass(int np, const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
-                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
-                       const u64 *cp) {
+                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st) {
     switch (np) {
-        case 1: return launch_pass_np<1>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, cp);
-        case 2: return launch_pass_np<2>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, cp);
-        case 3: return launch_pass_np<3>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, cp);
+        case 1: return launch_pass_np<1>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st);
+        case 2: return launch_pass_np<2>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st);
+        case 3: return launch_pass_np<3>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st);
         default: return -2;
     }
 }

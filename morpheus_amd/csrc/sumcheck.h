@@ -23,9 +23,7 @@ int mam_sc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_
                              hipStream_t st);
 // Two rounds per pass (whole-proof prover, np <= MAM_LOOK_MAX_POLYS): see k_sumcheck.hip.
 #define MAM_LOOK_MAX_POLYS 3
-// cp: optional device pointer to (ca, cb), read at run time when set.
 int mam_sc_launch_pass(int np, const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
-                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
-                       const u64 *cp = nullptr);
+                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st);
 int mam_sc_launch_pass_final(int np, const u64 *in, size_t in_stride, bool base_in, int folds, ext2 ca, ext2 cb,
                              u64 *finals, hipStream_t st);
