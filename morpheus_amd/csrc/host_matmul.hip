@@ -117,6 +117,7 @@ __global__ void k_ones_ext2(u64 *p, size_t k) {
 MAM_API int mam_matmul_prove(mam_matmul_instance *I, const u64 *d_table, int table_nv, int log_rate, int hash,
                              int num_queries, void *ajtai_handle, int ajtai_rows) {
     if (!I || !d_table || I->lm < 1 || I->lk < 1 || I->ln < 1) return -2;
+    if (log_rate < 0 || log_rate > 4 || (hash != 0 && hash != 1) || num_queries < 0) return -2;
     const int lm = I->lm, lk = I->lk, ln = I->ln;
     const int nvs[3] = {lm + lk, lk + ln, lm + ln};
     for (int x = 0; x < 3; x++)

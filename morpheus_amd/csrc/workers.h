@@ -1,7 +1,7 @@
 // Persistent host worker threads for the multi-instance entry points (batched proving, the
 // proving service). Each worker keeps its per-thread stream, arenas, mailbox and buffers
 // between calls; a pool is never torn down and its idle threads wait on a condition
-// variable. One run() at a time per pool.
+// variable. Concurrent run() calls on one pool are serialized.
 #pragma once
 
 #include <condition_variable>
@@ -13,6 +13,7 @@ class WorkerPool {
    public:
     // Run f on `n` pool threads (created on first need) and wait until every one returned.
     void run(int n, const std::function<void()> &f) {
+        std::lock_guard<std::mutex> one_run(run_mu_);
         std::unique_lock<std::mutex> lk(m_);
         while ((int)count_ < n) {
             std::thread(&WorkerPool::loop, this, (int)count_).detach();
@@ -42,7 +43,7 @@ class WorkerPool {
             if (--remaining_ == 0) done_.notify_all();
         }
     }
-    std::mutex m_;
+    std::mutex m_, run_mu_;
     std::condition_variable cv_, done_;
     std::function<void()> job_;
     unsigned gen_ = 0, count_ = 0;

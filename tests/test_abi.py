@@ -117,3 +117,21 @@ def test_host_verifier_agrees_with_oracle(nv, lr, nq, hash_kind):
     assert M.proof_len(nv, lr, nq) == O.basefold_proof_len(nv, lr, nq)
     parts = M.split_proof(proof, nv, lr, nq)
     assert len(parts["queries"]) == nq
+
+
+def test_argument_checks_before_device_work():
+    # rejected arguments return -2 before any device call (no GPU needed)
+    import ctypes
+    from morpheus_amd._lib import MatmulInstance_t
+    s = MatmulInstance_t()
+    s.lm, s.lk, s.ln = 3, 3, 3
+    fake = ctypes.c_void_p(16)
+    assert M.lib.mam_matmul_prove(ctypes.byref(s), fake, 6, 9, 0, 4, None, 0) == -2   # log_rate too large
+    assert M.lib.mam_matmul_prove(ctypes.byref(s), fake, 6, 1, 2, 4, None, 0) == -2   # unknown hash
+    assert M.lib.mam_matmul_prove(ctypes.byref(s), fake, 5, 1, 0, 4, None, 0) == -2   # table too small
+    s.lk = 0
+    assert M.lib.mam_matmul_prove(ctypes.byref(s), fake, 6, 1, 0, 4, None, 0) == -2
+    assert M.lib.mam_sumcheck_proof_bincode_len(-1, 3) == -2
+    buf = (ctypes.c_uint8 * 8)()
+    fe = np.zeros(2, dtype=np.uint64)
+    assert M.lib.mam_sumcheck_proof_bincode(None, 0, 3, fe.ctypes.data, buf, 8) == -2  # 24 bytes needed
