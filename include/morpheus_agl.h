@@ -99,6 +99,13 @@ int ajtai_commit_sparse_ffi(const uint32_t *d_key, const uint64_t *d_positions, 
 /* No reference counterpart: host copy of M_{row, col} (64 coefficients) for checks. */
 int mam_ajtai_ring_matrix_entry(const uint8_t *key32, uint32_t row, uint64_t col, uint64_t *out64);
 
+/* Tensor-core commit probe (ffi.rs:627-633): int8 GEMM partials
+ * partial[kb][nt][r * 16 + c] = sum over k in [1024 kb, min(1024 (kb + 1), K_total)) of
+ * z[r * K_total + k] * M[(nt * 16 + c) * K_total + k], nt < 60, r, c < 16 (z row-major 16 x K,
+ * M column-major K x 960); K_total a multiple of 16. Asynchronous; 0 / -1. */
+int ajtai_tc_commit_probe_ffi(const int8_t *d_z_int8, const int8_t *d_M_int8, int32_t *d_partial, int K_total,
+                              int num_K_chunks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -209,6 +209,7 @@ _SIGS = {
     "mam_basefold_batch_verify_ext2": (ci, [vp, ci, ci, ci, ci, vp, ci, vp, ci, vp, ci, vp, ci, vp]),
     "mam_sumcheck_proof_bincode_len": (ctypes.c_long, [ci, ci]),
     "mam_sumcheck_proof_bincode": (ctypes.c_long, [vp, ci, ci, vp, vp, ctypes.c_long]),
+    "ajtai_tc_commit_probe_ffi": (ci, [vp, vp, vp, ci, ci]),
     "mam_serve_matmul": (ci, [ctypes.POINTER(ServeRequest_t), ci, vp, ci, ci, ci, ci, vp, ci, ci, u64]),
     "fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "agl_fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),

@@ -410,3 +410,48 @@ MAM_API int mam_ajtai_i8_free(void *handle) {
     delete A;
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Tensor-core commit probe (almost-goldilocks-cuda-rs/src/ffi.rs:627-633,
+// cuda_almost_goldilocks/ajtai_tc_commit_probe.cuh:38-99): int8 GEMM partials
+//   partial[kb][nt][r * 16 + c] = sum_{k in chunk kb} z[r][k] * M[k][nt * 16 + c]
+// z row-major [16][K_total], M column-major [K_total][960] (entry (k, n) at n * K_total + k),
+// chunk kb = [1024 kb, min(1024 (kb + 1), K_total)), nt < 60; the reference's throughput
+// gate for a tensor-core commit. One wave per 16 x 16 output tile and chunk, on
+// v_mfma_i32_16x16x64_i8: lane l = i + 16 g loads the 16 bytes k0 + 16 g .. + 15 of row i
+// of z (A) and of column i of M (B); the same (g, byte) -> k assignment on both sides makes
+// the instruction's internal k order irrelevant. K_total must be a multiple of 16.
+// ---------------------------------------------------------------------------
+static constexpr int TC_OUT_TILES = 60;   // 960 outputs = 42 ring rows x 64 coefficients, rounded to 16
+static constexpr int TC_CHUNK_K = 1024;
+
+__global__ __launch_bounds__(64) void k_tc_probe(const int8_t *__restrict__ z, const int8_t *__restrict__ M,
+                                                 int32_t *__restrict__ partial, long K) {
+    const int nt = blockIdx.x, kb = blockIdx.y, lane = threadIdx.x;
+    const int i = lane & 15, g = lane >> 4;
+    const long k_lo = (long)kb * TC_CHUNK_K;
+    long k_hi = k_lo + TC_CHUNK_K;
+    if (k_hi > K) k_hi = K;
+    const int8_t *zr = z + (long)i * K;
+    const int8_t *mc = M + (long)(nt * 16 + i) * K;
+    v4i acc = v4i{0, 0, 0, 0};
+    for (long k0 = k_lo; k0 < k_hi; k0 += 64) {
+        const long k = k0 + 16 * g;
+        v4i a = v4i{0, 0, 0, 0}, b = v4i{0, 0, 0, 0};
+        if (k < k_hi) {
+            a = *(const v4i *)(zr + k);
+            b = *(const v4i *)(mc + k);
+        }
+        acc = __builtin_amdgcn_mfma_i32_16x16x64_i8(a, b, acc, 0, 0, 0);
+    }
+    int32_t *o = partial + ((long)kb * TC_OUT_TILES + nt) * 256;
+#pragma unroll
+    for (int v = 0; v < 4; v++) o[(4 * g + v) * 16 + i] = acc[v];
+}
+
+MAM_API int ajtai_tc_commit_probe_ffi(const int8_t *d_z_int8, const int8_t *d_M_int8, int32_t *d_partial, int K_total,
+                                      int num_K_chunks) {
+    if (!d_z_int8 || !d_M_int8 || !d_partial || K_total <= 0 || K_total % 16 || num_K_chunks <= 0) return -1;
+    k_tc_probe<<<dim3(TC_OUT_TILES, num_K_chunks), 64, 0, mam_stream()>>>(d_z_int8, d_M_int8, d_partial, K_total);
+    return mam_launch_status();
+}
