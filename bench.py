@@ -282,7 +282,7 @@ def bench_sumcheck(args, M):
     for _ in range(args.warmup):
         step()
     M.synchronize()
-    lib.mam_profile_enable(1)
+    lib.mam_profile_enable_mask(1 << 3)  # the pass kernels only
     lat = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -304,7 +304,9 @@ def bench_sumcheck(args, M):
                        "transcript": "poseidon2-w8", "parallelism": "single-gpu"},
             "proofs_per_s": args.steps / elapsed, "p50_ms": float(np.percentile(lat_ms, 50)),
             "p99_ms": float(np.percentile(lat_ms, 99)),
-            "roofline": {"kernel": "k_sc_fold_msg (fold + next message)", "bound": "hbm", "achieved": gbs,
+            "roofline": {"kernel": "k_sc_pass (two rounds per pass: folds on read + message + next message "
+                                   "as a polynomial in the challenge); 2^20 is latency-bound, the host transcript "
+                                   "(20 rounds x ~20 us) dominates", "bound": "hbm", "achieved": gbs,
                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
                          "avg_ms": ms / max(n, 1), "launches": n}}
 
