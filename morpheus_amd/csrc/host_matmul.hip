@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "common.h"
+#include "matmul.h"
 #include "transcript.h"
 
 extern "C" {
@@ -45,22 +46,6 @@ int mam_ajtai_i8_commit_prefix(void *handle, const u64 *d_z, int batch, uint64_t
 }
 int mam_eq_table(const u64 *d_r, int nv, u64 *d_out, bool ext, hipStream_t s);
 
-struct mam_matmul_instance {
-    const u64 *d_a, *d_b, *d_c;  // device, canonical words, row-major
-    int lm, lk, ln;
-    const uint8_t *label;
-    long label_len;
-    u64 *roots;        // [3][4] A, B, C
-    u64 *ajtai;        // [ajtai_rows], or null to skip the Ajtai commitment
-    u64 *point;        // [ln + lm][2]: r_j then r_i
-    u64 *claim;        // [2]
-    u64 *round_msgs;   // [lk][4][2]
-    u64 *challenges;   // [lk][2]
-    u64 *final_evals;  // [3][2]
-    u64 *proof_a, *proof_b, *proof_c;  // Basefold proofs (mam_basefold_proof_len words each)
-    double start_ms, end_ms;
-    int status;
-};
 
 namespace {
 

@@ -23,17 +23,9 @@
 #include <vector>
 
 #include "common.h"
+#include "matmul.h"
 #include "workers.h"
 
-struct mam_matmul_instance;  // host_matmul.hip
-extern "C" int mam_matmul_prove(mam_matmul_instance *inst, const u64 *d_table, int table_nv, int log_rate, int hash,
-                                int num_queries, void *ajtai_handle, int ajtai_rows);
-
-// Matches the layout of mam_matmul_instance's leading fields (host_matmul.hip).
-struct MatmulHead {
-    const u64 *d_a, *d_b, *d_c;
-    int lm, lk, ln;
-};
 
 struct mam_serve_request {
     mam_matmul_instance *inst;  // statement and output buffers
@@ -50,7 +42,7 @@ size_t poly_footprint(int nv, int lr) {
     return cw * 8 + ((size_t)1 << nv) * 8 + (cw - 1) * 32 + cw * 16 + cw * 32;
 }
 
-size_t matmul_footprint(const MatmulHead &h, int lr) {
+size_t matmul_footprint(const mam_matmul_instance &h, int lr) {
     return poly_footprint(h.lm + h.lk, lr) + poly_footprint(h.lk + h.ln, lr) + poly_footprint(h.lm + h.ln, lr);
 }
 
@@ -66,7 +58,7 @@ MAM_API int mam_serve_matmul(mam_serve_request *req, int n, const u64 *d_table, 
     for (int i = 0; i < n; i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return req[a].arrival_ms < req[b].arrival_ms; });
     std::vector<size_t> foot(n);
-    for (int i = 0; i < n; i++) foot[i] = matmul_footprint(*(const MatmulHead *)req[i].inst, log_rate);
+    for (int i = 0; i < n; i++) foot[i] = matmul_footprint(*req[i].inst, log_rate);
 
     std::mutex mu;
     std::condition_variable cv;
