@@ -54,11 +54,11 @@ def valu_peak(M):
 # VALU lane-operations of one Monolith permutation in the leaf kernel: SQ_INSTS_VALU x 64
 # over the permutations of one k_tree_levels<MonoHash, SRC_GL> dispatch (rocprofv3 --pmc,
 # profiles/merkle_leaf_kernel_2p22.md).
-VALU_OPS_PER_PERM = 6451.0
+VALU_OPS_PER_PERM = 4982.0
 # VALU instructions one lane executes for one full Monolith permutation with every lane
 # active: SQ_INSTS_VALU / SQ_WAVES of k_mono_permute_n (rocprofv3 --pmc,
 # profiles/monolith_permutation_valu.md). Lane-operations/s = permutations/s x this.
-MONO_PERM_VALU = 6968.0
+MONO_PERM_VALU = 5291.0
 # HBM bytes of that dispatch at 2^22 leaves: FETCH_SIZE x 2 (a 256 MiB copy reads back as
 # 128 MiB on this counter) + WRITE_SIZE (exact for the copy), same profile file.
 LEAF_LAUNCH_TRAFFIC_BYTES = (65.9 + 224.0) * 2 ** 20

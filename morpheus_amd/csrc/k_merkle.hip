@@ -127,8 +127,8 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
     __shared__ u64 sh[WIDE_GROUPS * 4];
     __shared__ __align__(16) u64 wst[WIDE_GROUPS][16];  // per-group permutation slots
     const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
-    u32 coef[12];
-    mono_wide::lane_coeffs(j, coef);
+    mono_wide::LaneConsts lc;
+    mono_wide::lane_consts(j, lc);
     const size_t i = (size_t)blockIdx.x * WIDE_GROUPS + g;
     const bool active = i < n_first;
     u64 x = 0;
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
             fs.out[2 * k + (j & 1)] = x;
         }
     }
-    x = mono_wide::permute(x, j, wst[g], coef);
+    x = mono_wide::permute(x, j, wst[g], lc);
     if (active && j < 4) {
         tree[4 * (first_off + i) + j] = x;
         sh[4 * g + j] = x;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
         u64 y = (g < block_n && j < 8) ? sh[8 * g + j] : 0;
         __syncthreads();
         // wave-uniform: a wave (4 groups) runs the permutation if its first group is active
-        if ((g & ~3) < block_n) y = mono_wide::permute(y, j, wst[g], coef);
+        if ((g & ~3) < block_n) y = mono_wide::permute(y, j, wst[g], lc);
         if (g < block_n && j < 4) {
             tree[4 * (level_off + (size_t)blockIdx.x * (WIDE_GROUPS >> l) + g) + j] = y;
             sh[4 * g + j] = y;

@@ -10,7 +10,7 @@
 //
 // Layout of the computation here (all integer, exact):
 //   concrete: y_r = sum_c ROW0[(c - r) mod 12] * x_c + RC[round][r]  (mod p)
-//             computed on 32-bit halves with 64-bit multiply-accumulate
+//             computed on 32-bit halves with 32x32+64 multiply-adds
 //   bars:     per byte b: b ^ (rotl1(~b) & rotl2(b) & rotl3(b)), then rotl1, on x_0..x_3
 //   bricks:   x_i += x_{i-1}^2 for i = 11..1 (old values)
 #pragma once
@@ -65,78 +65,34 @@ __host__ __device__ __forceinline__ u64 rc(int round, int r) {
 // residues; only the four bar inputs (bars act on the bits of the canonical value) and the
 // outputs are reduced to [0, p).
 
-// a + b for any 64-bit representatives: a wrap-around of 2^64 is worth 2^32 - 1.
-MAM_HD u64 add_nc(u64 a, u64 b) {
-    u64 s = a + b;
-    if (s < a) {
-        s += GL_EPS;
-        if (s < GL_EPS) s += GL_EPS;
-    }
-    return s;
+// y = a + 2^32 b (mod p) as a 64-bit representative, for a, b < 2^42. With
+// b = bh 2^32 + bl and 2^64 = 2^32 - 1 (mod p): y = (a + bh (2^32 - 1)) + bl 2^32. The first
+// sum stays below 2^43 (one 32x32+64 multiply-add); the second may wrap 2^64 once, and a
+// wrap-around is worth 2^32 - 1.
+MAM_HD u64 fold_halves(u64 a, u64 b) {
+    const u64 t = a + (b >> 32) * GL_EPS;
+    const u64 s = t + (b << 32);
+    return s < t ? s + GL_EPS : s;
 }
 
-// x^2 reduced to a 64-bit representative (not necessarily canonical).
-// With x = l + 2^32 h: x^2 = l^2 + 2^33 l h + 2^64 h^2 (three 32x32 products).
-MAM_HD u64 sqr_nc(u64 x) {
-    const u64 l = (u32)x, h = x >> 32;
-    const u64 ll = l * l, lh = l * h, hh2 = h * h;
-    u64 lo = ll + (lh << 33);
-    u64 hi = hh2 + (lh >> 31) + (lo < ll);
-    u64 hh = hi >> 32, hl = hi & GL_EPS;
-    u64 t0 = lo - hh;
-    if (lo < hh) t0 -= GL_EPS;
-    u64 t1 = hl * GL_EPS;
-    u64 t2 = t0 + t1;
-    if (t2 < t1) t2 += GL_EPS;
-    return t2;
-}
-
-// lo + 2^64 * hi with hi < 2^32, reduced to a 64-bit representative: 2^64 = 2^32 - 1 (mod p)
-MAM_HD u64 fold_hi_nc(u64 lo, u64 hi) {
-    u64 t = (hi << 32) - hi;
-    u64 s = lo + t;
-    if (s < t) s += GL_EPS;
-    return s;
-}
-
-// Canonical residue of lo + 2^64 * hi with hi < 2^32.
-MAM_HD u64 reduce_small_hi(u64 lo, u64 hi) { return gl_canon(fold_hi_nc(lo, hi)); }
-
-// y = a + 2^32 b + k (mod p) for a, b < 2^41 and the 64-bit constant k, as a 64-bit
-// representative, with 32-bit add-with-carry steps:
-//   y = (a0 + k0) + 2^32 (a1 + b0 + k1) + 2^64 b1 = (w1:w0) + 2^64 w2,
-//   then 2^64 w2 = w2 (2^32 - 1) and one wrap-around of the final 64-bit add is 2^32 - 1.
-MAM_HD u32 addc32(u32 x, u32 y, u32 cin, u32 &cout) {
+// A multiplier the compiler has to treat as unknown: x * 8 then stays one 32x32+64
+// multiply-add instead of becoming a shift of a zero-extended word plus an add.
+MAM_HD u32 opaque_u32(u32 v) {
 #ifdef __HIP_DEVICE_COMPILE__
-    return __builtin_addc(x, y, cin, &cout);
-#else
-    u64 s = (u64)x + y + cin;
-    cout = (u32)(s >> 32);
-    return (u32)s;
+    asm("" : "+s"(v));
 #endif
-}
-
-MAM_HD u64 combine(u64 a, u64 b, u64 k) {
-    u32 c0, c1, c2;
-    const u32 w0 = addc32((u32)a, (u32)k, 0, c0);
-    u32 w1 = addc32((u32)b, (u32)(k >> 32), c0, c1);
-    w1 = addc32(w1, (u32)(a >> 32), 0, c2);
-    const u64 w2 = (u64)((u32)(b >> 32) + c1 + c2);  // < 2^10
-    return fold_hi_nc(((u64)w1 << 32) | w0, w2);
-}
-
-template <int ROUND>
-MAM_HD u64 combine_row(u64 a, u64 b, int r) {
-    return combine(a, b, (ROUND >= 1 && ROUND <= 5) ? rc(ROUND, r) : 0);
+    return v;
 }
 
 // y_r = sum_c ROW0[(c - r) mod 12] * x_c + rc_r on 32-bit halves of any representatives:
-// sum_c m * lo_c and sum_c m * hi_c are each below 12 * 26 * 2^32 < 2^41 (one 32x32+64
-// multiply-add per term), then y = A + 2^32 B + rc is folded back below 2^64.
-// CANON_ROWS leading rows are made canonical (the next layer's bar inputs).
+// the round constant's halves start the two accumulators, and sum_c m * lo_c, sum_c m * hi_c
+// (one 32x32+64 multiply-add per term) stay below 161 * 2^32 < 2^40 (the row sums to 160),
+// then y = A + 2^32 B is folded back below 2^64. CANON_ROWS leading rows are made canonical
+// (the next layer's bar inputs).
 template <int ROUND, int CANON_ROWS>
 MAM_HD void concrete(u64 *x) {
     u32 lo[WIDTH], hi[WIDTH];
+    const u32 m8 = opaque_u32(8);
 #pragma unroll
     for (int c = 0; c < WIDTH; c++) {
         lo[c] = (u32)x[c];
@@ -144,33 +100,58 @@ MAM_HD void concrete(u64 *x) {
     }
 #pragma unroll
     for (int r = 0; r < WIDTH; r++) {
-        u64 a = 0, b = 0;
+        const u64 k = (ROUND >= 1 && ROUND <= 5) ? rc(ROUND, r) : 0;
+        u64 a = (u32)k, b = k >> 32;
 #pragma unroll
         for (int c = 0; c < WIDTH; c++) {
-            const u64 m = row0((c - r + WIDTH) % WIDTH);
+            u32 m = row0((c - r + WIDTH) % WIDTH);
+            if (m == 8) m = m8;
             a += (u64)lo[c] * m;
             b += (u64)hi[c] * m;
         }
-        x[r] = combine_row<ROUND>(a, b, r);
+        x[r] = fold_halves(a, b);
         if (r < CANON_ROWS) x[r] = gl_canon(x[r]);
     }
 }
 
-MAM_HD u64 bar(u64 v) {
-    const u64 m1h = 0x8080808080808080ULL, m1l = 0x7F7F7F7F7F7F7F7FULL;
-    const u64 m2h = 0xC0C0C0C0C0C0C0C0ULL, m2l = 0x3F3F3F3F3F3F3F3FULL;
-    const u64 m3h = 0xE0E0E0E0E0E0E0E0ULL, m3l = 0x1F1F1F1F1F1F1F1FULL;
-    u64 nv = ~v;
-    u64 r1 = ((nv & m1h) >> 7) | ((nv & m1l) << 1);
-    u64 r2 = ((v & m2h) >> 6) | ((v & m2l) << 2);
-    u64 r3 = ((v & m3h) >> 5) | ((v & m3l) << 3);
-    u64 t = v ^ (r1 & r2 & r3);
-    return ((t & m1h) >> 7) | ((t & m1l) << 1);
+// Each byte of w rotated left by k: the byte's top 8 - k bits move up (mask m keeps them),
+// its low k bits come round from the top (a bit-insert of the two shifts).
+MAM_HD u32 rotl_bytes(u32 w, int k, u32 m) { return ((w << k) & m) | ((w >> (8 - k)) & ~m); }
+
+// One bar on a 32-bit half: per byte b ^ (rotl1(~b) & rotl2(b) & rotl3(b)), then rotl1;
+// rotl1(~b) = ~rotl1(b). Bytes never straddle the halves, so each half is done on its own.
+MAM_HD u32 bar_half(u32 v) {
+    const u32 r1 = rotl_bytes(v, 1, 0xFEFEFEFEu), r2 = rotl_bytes(v, 2, 0xFCFCFCFCu);
+    const u32 r3 = rotl_bytes(v, 3, 0xF8F8F8F8u);
+    return rotl_bytes(v ^ (~r1 & r2 & r3), 1, 0xFEFEFEFEu);
+}
+
+MAM_HD u64 bar(u64 v) { return ((u64)bar_half((u32)(v >> 32)) << 32) | bar_half((u32)v); }
+
+// y + x^2 (mod p) as a 64-bit representative, for any representatives x, y.
+// With x = l + 2^32 h: y + x^2 = y + l^2 + 2^33 lh + 2^64 h^2 = lo + 2^64 hi (hi < 2^64, three
+// 32x32 products and two carries). Then 2^64 = 2^32 - 1 and 2^96 = -1 (mod p) give, with
+// hi = H1 2^32 + H0, lo + 2^64 hi = lo + H0 (2^32 - 1) - H1. t = H0 (2^32 - 1) + (2^32 - 1 - H1)
+// lies in [0, 2^64 - 2^32], so the result is lo + t - (2^32 - 1): a wrap-around of lo + t
+// (worth 2^32 - 1) cancels the subtraction, and a borrow of the subtraction (worth
+// -(2^32 - 1)) is paid by subtracting 2^32 - 1 once more.
+MAM_HD u64 sqr_add(u64 y, u64 x) {
+    const u64 l = (u32)x, h = x >> 32;
+    const u64 lh = l * h, hh = h * h;
+    const u64 a = l * l + y;
+    u64 hi = hh + (lh >> 31) + (a < y);
+    const u64 lo = a + (lh << 33);
+    hi += (lo < a);
+    const u64 t = (hi & GL_EPS) * GL_EPS + ((~(hi >> 32)) & GL_EPS);
+    const u64 r = lo + t;
+    if (r < lo) return r;
+    const u64 s = r - GL_EPS;
+    return r < GL_EPS ? s - GL_EPS : s;
 }
 
 MAM_HD void bricks(u64 *x) {
 #pragma unroll
-    for (int i = WIDTH - 1; i >= 1; i--) x[i] = add_nc(x[i], sqr_nc(x[i - 1]));
+    for (int i = WIDTH - 1; i >= 1; i--) x[i] = sqr_add(x[i], x[i - 1]);
 }
 
 // bar inputs x[0..3] are canonical (made so by the previous concrete layer)
