@@ -23,6 +23,26 @@ extern "C" {
 
 int almost_goldilocks_cuda_init(void); /* ffi.rs:13 */
 
+/* memory and streams of the q crate beyond the shared ones in morpheus_gl.h (ffi.rs:61-100,
+ * 342-350, 687). cuda_malloc / cuda_free cache freed blocks per device (best fit within
+ * 1.25 x + 2 MiB, event-guarded reuse; AGL_ALLOC_CACHE=0 turns the cache off), as the
+ * q crate's allocator does (wrapper.cu:118-248). */
+size_t agl_cached_bytes(void);                                           /* ffi.rs:61 */
+int agl_d2h_staged(void *h_dst, const void *d_src, size_t bytes);        /* ffi.rs:63 */
+int agl_h2d_staged(void *d_dst, const void *h_src, size_t bytes);        /* ffi.rs:64 */
+const char *agl_cuda_error_string(int code);                            /* ffi.rs:80 */
+int cuda_memset(void *dst, int value, size_t size);                     /* ffi.rs:84 */
+int cuda_pool_trim(size_t min_bytes_to_keep);                           /* ffi.rs:85 */
+int cuda_sync_mine(void);                                               /* ffi.rs:87; AGL_DEVICE_SYNC=1: device-wide */
+int cuda_stream_create(void **stream);                                  /* ffi.rs:96 */
+int cuda_stream_destroy(void *stream);                                  /* ffi.rs:97 */
+int cuda_stream_synchronize(void *stream);                              /* ffi.rs:98 */
+int cuda_memcpy_dtod_async(void *dst, const void *src, size_t size, void *stream); /* ffi.rs:99 */
+int cuda_memcpy_htod_async(void *dst, const void *src, size_t size, void *stream); /* ffi.rs:100 */
+int cuda_memcpy_peer(void *dst, int dst_dev, const void *src, int src_dev, size_t size); /* ffi.rs:342-348 */
+int cuda_enable_peer_access(int peer);                                  /* ffi.rs:350 */
+int agl_zero_buffer_ffi(uint64_t *d_buf, size_t n);                     /* ffi.rs:687; returns the runtime error code */
+
 /* base field batch ops, ffi.rs:103-112 */
 int agl_batch_add_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
 int agl_batch_sub_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
@@ -34,6 +54,8 @@ int agl_batch_inverse_ffi(const uint64_t *a, uint64_t *r, int n);
 int agl_batch_exp_ffi(const uint64_t *a, uint64_t exp, uint64_t *r, int n);
 int agl_batch_mul_scalar_ffi(uint64_t scalar, const uint64_t *a, uint64_t *r, int n);
 int agl_batch_div_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
+/* out[x] = in[y], bit b of x = bit inv_map[b] of y (ffi.rs:113-119), total = 2^n_bits */
+int agl_bit_permute_ffi(const uint64_t *d_input, uint64_t *d_output, const int *d_perm_map, int n_bits, int total);
 
 /* Ext2 batch ops, ffi.rs:122-138 */
 int aext2_batch_add_ffi(const uint64_t *a, const uint64_t *b, uint64_t *r, int n);
@@ -85,6 +107,22 @@ int aext2_sumcheck_batched_round_message_ffi(const uint64_t *d_polys, uint64_t *
                                              size_t half, int num_leaves, int num_blocks_x);
 int aext2_sumcheck_batched_fold_ffi(const uint64_t *d_input, uint64_t *d_output, uint64_t challenge_c0,
                                     uint64_t challenge_c1, size_t original_size, size_t half, int num_leaves);
+/* round 0 of the same state with a binary f given as bits (packed_size_u64 words per leaf,
+ * bit i of word i / 64 = f(i)): message from the eq slot and the bits, ffi.rs:402-411; fold
+ * writing eq' and the Ext2 f' = f0 + ch (f1 - f0) at half size, ffi.rs:413-423 */
+int aext2_sumcheck_batched_round0_binary_msg_ffi(const uint64_t *d_polys, const uint64_t *d_packed,
+                                                 uint64_t *d_partial, size_t original_size, size_t half,
+                                                 int num_leaves, int num_blocks_x, size_t packed_size_u64);
+int aext2_sumcheck_batched_round0_binary_fold_ffi(const uint64_t *d_polys, const uint64_t *d_packed,
+                                                  uint64_t *d_output, uint64_t challenge_c0, uint64_t challenge_c1,
+                                                  size_t original_size, size_t half, int num_leaves,
+                                                  size_t packed_size_u64);
+/* lifts into the f slot of that layout: bits (ffi.rs:293-299), or pos - neg in {0, 1, q - 1}
+ * (ffi.rs:274-281) */
+int aext2_batched_lift_binary_ffi(const uint64_t *d_packed, uint64_t *d_polys, size_t original_size,
+                                  int num_leaves, size_t packed_size_u64);
+int aext2_batched_lift_ternary_single_ffi(const uint64_t *d_pos, const uint64_t *d_neg, uint64_t *d_polys,
+                                          size_t original_size, int num_leaves, size_t packed_size_u64);
 
 /* ring Ajtai over F_q[X]/(X^64 + 1), KAPPA = 42 (ajtai.rs:138-386); d_key: 8 u32 on the
  * device; outputs [B][42][64] canonical coefficients. */

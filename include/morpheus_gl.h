@@ -31,7 +31,7 @@ int goldilocks_cuda_init(void);                                     /* ffi.rs:11
 int poseidon2_cuda_init(void);                                      /* ffi.rs:12 */
 int monolith_cuda_init(void);                                       /* ffi.rs:565 */
 int cuda_malloc(void **ptr, size_t size);                           /* ffi.rs:18 */
-int cuda_free(void *ptr);                                           /* ffi.rs:19 */
+int cuda_free(void *ptr);                                           /* ffi.rs:19; caching, see morpheus_agl.h */
 int cuda_memcpy_htod(void *dst, const void *src, size_t size);      /* ffi.rs:20 */
 int cuda_memcpy_dtoh(void *dst, const void *src, size_t size);      /* ffi.rs:21 */
 int cuda_memcpy_dtod(void *dst, const void *src, size_t size);      /* ffi.rs:22 */

@@ -190,6 +190,26 @@ _SIGS = {
     "aext2_sumcheck_fold_ffi": (ci, [vp, vp, u64, u64, ci, csz, csz]),
     "aext2_sumcheck_batched_round_message_ffi": (ci, [vp, vp, csz, csz, ci, ci]),
     "aext2_sumcheck_batched_fold_ffi": (ci, [vp, vp, u64, u64, csz, csz, ci]),
+    "aext2_sumcheck_batched_round0_binary_msg_ffi": (ci, [vp, vp, vp, csz, csz, ci, ci, csz]),
+    "aext2_sumcheck_batched_round0_binary_fold_ffi": (ci, [vp, vp, vp, u64, u64, csz, csz, ci, csz]),
+    "aext2_batched_lift_binary_ffi": (ci, [vp, vp, csz, ci, csz]),
+    "aext2_batched_lift_ternary_single_ffi": (ci, [vp, vp, vp, csz, ci, csz]),
+    "agl_bit_permute_ffi": (ci, [vp, vp, vp, ci, ci]),
+    "agl_cached_bytes": (csz, []),
+    "agl_d2h_staged": (ci, [vp, vp, csz]),
+    "agl_h2d_staged": (ci, [vp, vp, csz]),
+    "agl_cuda_error_string": (ctypes.c_char_p, [ci]),
+    "cuda_memset": (ci, [vp, ci, csz]),
+    "cuda_pool_trim": (ci, [csz]),
+    "cuda_sync_mine": (ci, []),
+    "cuda_stream_create": (ci, [ctypes.POINTER(vp)]),
+    "cuda_stream_destroy": (ci, [vp]),
+    "cuda_stream_synchronize": (ci, [vp]),
+    "cuda_memcpy_dtod_async": (ci, [vp, vp, csz, vp]),
+    "cuda_memcpy_htod_async": (ci, [vp, vp, csz, vp]),
+    "cuda_memcpy_peer": (ci, [vp, ci, vp, ci, csz]),
+    "cuda_enable_peer_access": (ci, [ci]),
+    "agl_zero_buffer_ffi": (ci, [vp, csz]),
     "ajtai_commit_dense_batched_ffi": (ci, [vp, vp, u64, ci, ci, vp]),
     "ajtai_commit_dense_batched_at_ffi": (ci, [vp, vp, u64, ci, ci, u64, vp]),
     "ajtai_commit_wide_ffi": (ci, [vp, vp, u64, u64, ci, vp]),

@@ -26,16 +26,7 @@ MAM_API int monolith_cuda_init() { return goldilocks_cuda_init(); }
 // Memory management (wrapper.cu:51-97); copies are ordered on the per-thread stream and
 // complete before returning, as the reference's synchronous cudaMemcpy does.
 // ---------------------------------------------------------------------------
-MAM_API int cuda_malloc(void **ptr, size_t size) {
-    if (!ptr) return -1;
-    if (size == 0) size = 1;
-    return hipMalloc(ptr, size) == hipSuccess ? 0 : -1;
-}
-
-MAM_API int cuda_free(void *ptr) {
-    if (!ptr) return 0;
-    return hipFree(ptr) == hipSuccess ? 0 : -1;
-}
+// cuda_malloc / cuda_free and the rest of the memory entry points: mem_pool.hip.
 
 static int copy_sync(void *dst, const void *src, size_t size, hipMemcpyKind kind) {
     if (size == 0) return 0;
@@ -89,6 +80,16 @@ MAM_API int gl_memset_zero(u64 *d_buf, size_t n_u64) {
 }
 
 MAM_API int gl_zero_buffer(u64 *d_buf, int n) { return n <= 0 ? 0 : gl_memset_zero(d_buf, (size_t)n); }
+
+// almost-goldilocks-cuda-rs/src/ffi.rs:687: n zero words, complete on return; returns the
+// runtime's error code (0 on success).
+MAM_API int agl_zero_buffer_ffi(u64 *d_buf, size_t n) {
+    if (n == 0) return 0;
+    k_fill_zero<<<grid_1d(n), MAM_BLOCK, 0, mam_stream()>>>(d_buf, n);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(mam_stream());
+    return (int)e;
+}
 
 enum BinOp { OP_ADD, OP_SUB, OP_MUL, OP_DIV };
 enum UnOp { OP_NEG, OP_SQR, OP_DBL, OP_INV, OP_EXP, OP_SCALE, OP_FROB };

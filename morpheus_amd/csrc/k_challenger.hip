@@ -211,3 +211,8 @@ MAM_API int bit_permute_gl_ffi(const u64 *d_input, u64 *d_output, const int *d_p
     k_bit_permute<<<grid_1d(total), MAM_BLOCK, 0, mam_stream()>>>(d_input, d_output, d_perm_map, n_bits, total);
     return mam_launch_status();
 }
+// The q crate's twin (almost-goldilocks-cuda-rs/src/ffi.rs:113-119): the gather moves words and
+// does no field arithmetic, so it is the same launch.
+MAM_API int agl_bit_permute_ffi(const u64 *d_input, u64 *d_output, const int *d_perm_map, int n_bits, int total) {
+    return bit_permute_gl_ffi(d_input, d_output, d_perm_map, n_bits, total);
+}

@@ -524,6 +524,152 @@ __global__ void k_batched_fold_f(const u64 *__restrict__ in, u64 *__restrict__ o
     }
 }
 
+// Round 0 with a binary f (GpuBatchedSamePointState, sumcheck_prover.rs:468-1027): f is still
+// a bit vector (`packed_size_u64` words per leaf, bit i of word i/64 = f(i)), so the message
+// and the fold read the bits instead of a lifted Ext2 f-slot. The message uses the same
+// grid-stride split as k_batched_msg_f (identical per-block partials); the fold writes both
+// slots at half size: eq' = e0 + ch (e1 - e0) and f' = f0 + ch (f1 - f0) in {0, ch, 1 - ch, 1}.
+// Pairs (2y, 2y+1) never straddle a packed word.
+template <class F>
+__global__ __launch_bounds__(MAM_BLOCK) void k_batched_r0bin_msg_f(const u64 *__restrict__ polys,
+                                                                   const u64 *__restrict__ packed,
+                                                                   u64 *__restrict__ partial, size_t original_size,
+                                                                   size_t half, int num_leaves, size_t packed_words) {
+    using E = typename F::E;
+    __shared__ u64 lds[3 * 2 * MAM_BLOCK / 64];
+    const int leaf = blockIdx.y;
+    const u64 *e_p = polys + (size_t)leaf * 4 * original_size;
+    const u64 *bits = packed + (size_t)leaf * packed_words;
+    E acc[3] = {F::e_zero(), F::e_zero(), F::e_zero()};
+    for (size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x; y < half; y += (size_t)gridDim.x * blockDim.x) {
+        const E e0 = F::e_load(e_p, 2 * y), e1 = F::e_load(e_p, 2 * y + 1);
+        const u64 w = bits[y >> 5] >> ((2 * y) & 63);
+        const bool f0 = w & 1, f1 = (w >> 1) & 1;
+        if (f0) acc[0] = F::e_add(acc[0], e0);
+        if (f1) acc[1] = F::e_add(acc[1], e1);
+        // (2 e1 - e0) (2 f1 - f0), 2 f1 - f0 in {-1, 0, 1, 2}
+        if (f0 != f1 || f1) {
+            const E e2v = F::e_sub(F::e_add(e1, e1), e0);
+            if (f1 && !f0)
+                acc[2] = F::e_add(acc[2], F::e_add(e2v, e2v));
+            else if (f1)
+                acc[2] = F::e_add(acc[2], e2v);
+            else
+                acc[2] = F::e_sub(acc[2], e2v);
+        }
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int c = 0; c < 3; c++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            const u64 o0 = ((u64)__shfl_xor((u32)(acc[c].c0 >> 32), off) << 32) | (u32)__shfl_xor((u32)acc[c].c0, off);
+            const u64 o1 = ((u64)__shfl_xor((u32)(acc[c].c1 >> 32), off) << 32) | (u32)__shfl_xor((u32)acc[c].c1, off);
+            acc[c] = F::e_add(acc[c], F::e(o0, o1));
+        }
+        if (lane == 0) {
+            lds[(c * (MAM_BLOCK / 64) + wave) * 2] = acc[c].c0;
+            lds[(c * (MAM_BLOCK / 64) + wave) * 2 + 1] = acc[c].c1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x / 64;
+        for (int c = 0; c < 3; c++) {
+            E s = F::e_zero();
+            for (int w = 0; w < nw; w++)
+                s = F::e_add(s, F::e(lds[(c * (MAM_BLOCK / 64) + w) * 2], lds[(c * (MAM_BLOCK / 64) + w) * 2 + 1]));
+            F::e_store(partial, ((size_t)blockIdx.x * num_leaves + leaf) * 3 + c, s);
+        }
+    }
+}
+
+template <class F>
+__global__ void k_batched_r0bin_fold_f(const u64 *__restrict__ polys, const u64 *__restrict__ packed,
+                                       u64 *__restrict__ out, typename F::E ch, size_t original_size, size_t half,
+                                       size_t packed_words) {
+    using E = typename F::E;
+    const int leaf = blockIdx.y;
+    const size_t ps = 2 * original_size;
+    const u64 *e_p = polys + (size_t)leaf * 2 * ps;
+    const u64 *bits = packed + (size_t)leaf * packed_words;
+    u64 *dst = out + (size_t)leaf * 2 * ps;
+    const E one = F::e_one();
+    for (size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x; y < half; y += (size_t)gridDim.x * blockDim.x) {
+        const E a = F::e_load(e_p, 2 * y), b = F::e_load(e_p, 2 * y + 1);
+        F::e_store(dst, y, F::e_add(a, F::e_mul(ch, F::e_sub(b, a))));
+        const u64 w = bits[y >> 5] >> ((2 * y) & 63);
+        const bool f0 = w & 1, f1 = (w >> 1) & 1;
+        const E f = f0 ? (f1 ? one : F::e_sub(one, ch)) : (f1 ? ch : F::e_zero());
+        F::e_store(dst + ps, y, f);
+    }
+}
+
+// Lifts of a packed witness into the f-slot of the batched layout (the eq slot is left
+// alone): binary f(i) = bit i; ternary f(i) = pos_i - neg_i in {0, 1, -1} (both set gives 0).
+template <class F, bool TERNARY>
+__global__ void k_batched_lift_f(const u64 *__restrict__ pos, const u64 *__restrict__ neg, u64 *__restrict__ polys,
+                                 size_t original_size, size_t packed_words) {
+    const int leaf = blockIdx.y;
+    const u64 *pb = pos + (size_t)leaf * packed_words;
+    const u64 *nb = TERNARY ? neg + (size_t)leaf * packed_words : nullptr;
+    u64 *f = polys + (size_t)leaf * 4 * original_size + 2 * original_size;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < original_size;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const u64 p = (pb[i >> 6] >> (i & 63)) & 1;
+        u64 v = p;
+        if (TERNARY) v = F::sub(p, (nb[i >> 6] >> (i & 63)) & 1);
+        F::e_store(f, i, F::e_base(v));
+    }
+}
+
+// The witness bits a launch reads: a leaf's packed block must hold original_size bits.
+static bool packed_ok(size_t original_size, size_t packed_words) {
+    return packed_words >= (original_size + 63) / 64;
+}
+
+// ffi.rs:402-411
+MAM_API int aext2_sumcheck_batched_round0_binary_msg_ffi(const u64 *d_polys, const u64 *d_packed, u64 *d_partial,
+                                                         size_t original_size, size_t half, int num_leaves,
+                                                         int num_blocks_x, size_t packed_size_u64) {
+    if (num_leaves <= 0) return 0;
+    if (num_blocks_x <= 0 || num_leaves > 65535 || 2 * half > original_size ||
+        !packed_ok(original_size, packed_size_u64))
+        return -1;
+    k_batched_r0bin_msg_f<FieldQ><<<dim3(num_blocks_x, num_leaves), MAM_BLOCK, 0, mam_stream()>>>(
+        d_polys, d_packed, d_partial, original_size, half, num_leaves, packed_size_u64);
+    return mam_sync_status();
+}
+// ffi.rs:413-423
+MAM_API int aext2_sumcheck_batched_round0_binary_fold_ffi(const u64 *d_polys, const u64 *d_packed, u64 *d_output,
+                                                          u64 c0, u64 c1, size_t original_size, size_t half,
+                                                          int num_leaves, size_t packed_size_u64) {
+    if (num_leaves <= 0 || half == 0) return 0;
+    if (num_leaves > 65535 || 2 * half > original_size || !packed_ok(original_size, packed_size_u64)) return -1;
+    unsigned gx = grid_stride(half, MAM_BLOCK, 1024);
+    k_batched_r0bin_fold_f<FieldQ><<<dim3(gx, num_leaves), MAM_BLOCK, 0, mam_stream()>>>(
+        d_polys, d_packed, d_output, qe2{fq::canon(c0), fq::canon(c1)}, original_size, half, packed_size_u64);
+    return mam_sync_status();
+}
+// ffi.rs:293-299
+MAM_API int aext2_batched_lift_binary_ffi(const u64 *d_packed, u64 *d_polys, size_t original_size, int num_leaves,
+                                          size_t packed_size_u64) {
+    if (num_leaves <= 0 || original_size == 0) return 0;
+    if (num_leaves > 65535 || !packed_ok(original_size, packed_size_u64)) return -1;
+    unsigned gx = grid_stride(original_size, MAM_BLOCK, 1024);
+    k_batched_lift_f<FieldQ, false><<<dim3(gx, num_leaves), MAM_BLOCK, 0, mam_stream()>>>(
+        d_packed, nullptr, d_polys, original_size, packed_size_u64);
+    return mam_sync_status();
+}
+// ffi.rs:274-281
+MAM_API int aext2_batched_lift_ternary_single_ffi(const u64 *d_pos, const u64 *d_neg, u64 *d_polys,
+                                                  size_t original_size, int num_leaves, size_t packed_size_u64) {
+    if (num_leaves <= 0 || original_size == 0) return 0;
+    if (num_leaves > 65535 || !packed_ok(original_size, packed_size_u64)) return -1;
+    unsigned gx = grid_stride(original_size, MAM_BLOCK, 1024);
+    k_batched_lift_f<FieldQ, true><<<dim3(gx, num_leaves), MAM_BLOCK, 0, mam_stream()>>>(
+        d_pos, d_neg, d_polys, original_size, packed_size_u64);
+    return mam_sync_status();
+}
+
 // ffi.rs:265-272, :320-328
 MAM_API int aext2_sumcheck_batched_round_message_ffi(const u64 *d_polys, u64 *d_partial, size_t original_size,
                                                      size_t half, int num_leaves, int num_blocks_x) {

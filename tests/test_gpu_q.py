@@ -2,6 +2,7 @@
 field and Ext2 batch ops, eq tables (plain, stream, batched), partial evaluation,
 sumcheck round messages and folds (plain and batched), and the ring Ajtai commitments."""
 import ctypes
+import gc
 
 import numpy as np
 import pytest
@@ -269,3 +270,120 @@ def test_ring_commit_sparse():
     for chunk in (64, 4096):
         assert np.array_equal(R.commit_sparse(seed, pos, chunk).rows, O.ring_commit_sparse(KEY, pos))
     assert np.array_equal(R.commit_sparse(seed, []).rows, np.zeros((42, 64), dtype=np.uint64))
+
+
+def lifted(polys, bits, n, leaves, pw):
+    """the batched layout with each leaf's f slot replaced by its bits as Ext2 (c1 = 0)"""
+    out = polys.copy()
+    for l in range(leaves):
+        b = bits[l * pw:(l + 1) * pw]
+        f = np.array([(int(b[i >> 6]) >> (i & 63)) & 1 for i in range(n)], dtype=np.uint64)
+        s = l * 4 * n + 2 * n
+        out[s:s + 2 * n:2] = f
+        out[s + 1:s + 2 * n:2] = 0
+    return out
+
+
+def test_sumcheck_batched_round0_binary_and_lifts():
+    print("q sumcheck batched round 0 binary", flush=True)
+    rng = np.random.default_rng(51)
+    n, leaves, bx = 1 << 9, 5, 3
+    pw = n // 64 + 1  # a leaf's packed block may be longer than its bits
+    polys = words(rng, leaves * 2 * 2 * n)  # the f slots hold garbage the round-0 kernels must ignore
+    bits = rng.integers(0, 1 << 64, leaves * pw, dtype=np.uint64)
+    want_layout = lifted(polys, bits, n, leaves, pw)
+    dp, db = dev(polys), dev(bits)
+    part = M.DeviceBuffer(bx * leaves * 3 * 2)
+    assert lib.aext2_sumcheck_batched_round0_binary_msg_ffi(dp.ptr, db.ptr, part.ptr, n, n // 2, leaves, bx, pw) == 0
+    p = part.to_numpy().reshape(bx, leaves, 3, 2)
+    # the same per-block partials as the plain kernel on the lifted layout
+    dl, part2 = dev(want_layout), M.DeviceBuffer(bx * leaves * 3 * 2)
+    assert lib.aext2_sumcheck_batched_round_message_ffi(dl.ptr, part2.ptr, n, n // 2, leaves, bx) == 0
+    assert np.array_equal(part.to_numpy(), part2.to_numpy())
+    ch = [int(x) for x in words(rng, 2)]
+    out = M.DeviceBuffer(leaves * 2 * 2 * n)
+    assert lib.aext2_sumcheck_batched_round0_binary_fold_ffi(dp.ptr, db.ptr, out.ptr, ch[0], ch[1], n, n // 2, leaves,
+                                                             pw) == 0
+    fo = out.to_numpy()
+    for l in range(leaves):
+        leaf = want_layout[l * 4 * n:(l + 1) * 4 * n]
+        want = O.q_sumcheck_msg(1, leaf, 2, n, n // 2).reshape(-1, 2)
+        for c in range(3):
+            got = [sum(int(p[b, l, c, k]) for b in range(bx)) % Q for k in range(2)]
+            assert got == [int(want[c][0]), int(want[c][1])], (l, c)
+        fw = O.q_sumcheck_fold(1, leaf, ch, 2, n, n // 2)
+        for i in range(2):
+            s = l * 4 * n + i * 2 * n
+            assert np.array_equal(fo[s:s + n], fw[i * 2 * n:i * 2 * n + n]), (l, i)
+    # binary lift writes the f slots only
+    assert lib.aext2_batched_lift_binary_ffi(db.ptr, dp.ptr, n, leaves, pw) == 0
+    assert np.array_equal(dp.to_numpy(), want_layout)
+    # ternary lift: pos - neg, a bit set in both gives 0
+    neg = rng.integers(0, 1 << 64, leaves * pw, dtype=np.uint64)
+    dn = dev(neg)
+    assert lib.aext2_batched_lift_ternary_single_ffi(db.ptr, dn.ptr, dp.ptr, n, leaves, pw) == 0
+    got = dp.to_numpy()
+    for l in range(leaves):
+        s = l * 4 * n + 2 * n
+        for i in range(0, n, 7):
+            pb = (int(bits[l * pw + (i >> 6)]) >> (i & 63)) & 1
+            nb = (int(neg[l * pw + (i >> 6)]) >> (i & 63)) & 1
+            assert int(got[s + 2 * i]) == (pb - nb) % Q and int(got[s + 2 * i + 1]) == 0, (l, i)
+        assert np.array_equal(got[l * 4 * n:l * 4 * n + 2 * n], polys[l * 4 * n:l * 4 * n + 2 * n])
+    # a packed block shorter than the bits is rejected
+    assert lib.aext2_batched_lift_binary_ffi(db.ptr, dp.ptr, n, leaves, n // 64 - 1) == -1
+
+
+def test_bit_permute_twin_and_memory_entry_points():
+    print("q bit permute, memory", flush=True)
+    rng = np.random.default_rng(52)
+    nb = 10
+    x = words(rng, 1 << nb)
+    inv = np.array(rng.permutation(nb), dtype=np.int32)
+    dx, dy = dev(x), M.DeviceBuffer(1 << nb)
+    dm = M.DeviceBuffer(nb)
+    assert lib.cuda_memcpy_htod(dm.ptr, inv.ctypes.data, inv.nbytes) == 0
+    assert lib.agl_bit_permute_ffi(dx.ptr, dy.ptr, dm.ptr, nb, 1 << nb) == 0
+    y = dy.to_numpy()
+    for i in range(0, 1 << nb, 37):
+        src = sum(((i >> b) & 1) << int(inv[b]) for b in range(nb))
+        assert y[i] == x[src]
+    # caching allocator: a freed block is reused for a request of the same size and trimmed on demand
+    gc.collect()
+    assert lib.cuda_pool_trim(0) == 0
+    p = ctypes.c_void_p()
+    assert lib.cuda_malloc(ctypes.byref(p), 3 << 20) == 0
+    first = p.value
+    assert lib.cuda_free(first) == 0
+    assert lib.agl_cached_bytes() >= 3 << 20
+    assert lib.cuda_malloc(ctypes.byref(p), 3 << 20) == 0
+    assert p.value == first
+    # memset, zero buffer, staged copies, async copies on a caller stream
+    h = np.arange(1 << 16, dtype=np.uint64)
+    assert lib.agl_h2d_staged(p.value, h.ctypes.data, h.nbytes) == 0
+    back = np.zeros_like(h)
+    assert lib.agl_d2h_staged(back.ctypes.data, p.value, h.nbytes) == 0
+    assert np.array_equal(back, h)
+    assert lib.cuda_memset(p.value, 0xAB, 64) == 0
+    assert lib.agl_d2h_staged(back.ctypes.data, p.value, h.nbytes) == 0
+    assert all(int(v) == 0xABABABABABABABAB for v in back[:8]) and int(back[8]) == 8
+    assert lib.agl_zero_buffer_ffi(p.value, 1 << 16) == 0
+    assert lib.agl_d2h_staged(back.ctypes.data, p.value, h.nbytes) == 0
+    assert not back.any()
+    s = ctypes.c_void_p()
+    assert lib.cuda_stream_create(ctypes.byref(s)) == 0
+    q = ctypes.c_void_p()
+    assert lib.cuda_malloc(ctypes.byref(q), h.nbytes) == 0
+    assert lib.cuda_memcpy_htod_async(p.value, h.ctypes.data, h.nbytes, s.value) == 0
+    assert lib.cuda_memcpy_dtod_async(q.value, p.value, h.nbytes, s.value) == 0
+    assert lib.cuda_stream_synchronize(s.value) == 0
+    assert lib.cuda_stream_destroy(s.value) == 0
+    assert lib.cuda_memcpy_peer(p.value, 0, q.value, 0, h.nbytes) == 0
+    assert lib.cuda_sync_mine() == 0
+    assert lib.cuda_memcpy_dtoh(back.ctypes.data, p.value, h.nbytes) == 0
+    assert np.array_equal(back, h)
+    assert isinstance(lib.agl_cuda_error_string(0), bytes)
+    assert lib.cuda_free(q.value) == 0 and lib.cuda_free(p.value) == 0
+    gc.collect()
+    assert lib.cuda_pool_trim(0) == 0
+    assert lib.agl_cached_bytes() == 0
