@@ -61,7 +61,7 @@ VALU_OPS_PER_PERM = 4982.0
 MONO_PERM_VALU = 5291.0
 # HBM bytes of that dispatch at 2^22 leaves: FETCH_SIZE x 2 (a 256 MiB copy reads back as
 # 128 MiB on this counter) + WRITE_SIZE (exact for the copy), same profile file.
-LEAF_LAUNCH_TRAFFIC_BYTES = (65.9 + 224.0) * 2 ** 20
+LEAF_LAUNCH_TRAFFIC_BYTES = (65.2 + 224.0) * 2 ** 20
 # HBM bytes of one int8 Ajtai GEMM launch (witness 2^22, 256 rows, batch 1): FETCH_SIZE
 # 540765 KiB x 2.0 (copy calibration) + WRITE_SIZE 8192 KiB, from the rocprofv3 --pmc runs.
 AJTAI_GEMM_TRAFFIC_BYTES = (540765.0 * 2.0 + 8192.0) * 1024
