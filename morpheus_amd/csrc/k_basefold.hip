@@ -23,6 +23,7 @@
 //   256-byte coalesced.
 #include "basefold.h"
 #include "common.h"
+#include "eq_table.h"
 #include "profile.h"
 #include "reduce.h"
 
@@ -527,64 +528,12 @@ MAM_API int basefold_dot_product_mixed_ffi(const u64 *d_a, const u64 *d_b, u64 *
 }
 
 // ---------------------------------------------------------------------------
-// eq tables in one pass: a workgroup covers 2^EQ_LOW consecutive indices; it builds the
-// table of the low variables in LDS by doubling, multiplies its one high-variable factor
-// in, and streams the result out once.
+// eq tables (eq_table.h): one pass, one product per entry.
 // ---------------------------------------------------------------------------
-static constexpr int EQ_LOW = 10;
-
-template <bool EXT>
-__global__ __launch_bounds__(256) void k_eq_table(const u64 *__restrict__ r, int nv, u64 *__restrict__ out) {
-    __shared__ u64 lt[2 << EQ_LOW];
-    __shared__ u64 hf[2];
-    const int low = nv < EQ_LOW ? nv : EQ_LOW;
-    const int L = 1 << low;
-    auto ld = [&](int i) { return EXT ? e2_load(r, i) : e2_base(gl_canon(r[i])); };
-    if (threadIdx.x == 0) {
-        // factor of the high variables for this workgroup's index range
-        ext2 h = e2_one();
-        size_t hb = blockIdx.x;
-        for (int i = low; i < nv; i++) {
-            ext2 ri = ld(i);
-            h = e2_mul(h, ((hb >> (i - low)) & 1) ? ri : e2_sub(e2_one(), ri));
-        }
-        hf[0] = h.c0;
-        hf[1] = h.c1;
-        lt[0] = 1;
-        lt[1] = 0;
-    }
-    __syncthreads();
-    for (int i = 0; i < low; i++) {
-        const int half = 1 << i;
-        ext2 ri = ld(i);
-        for (int j = threadIdx.x; j < half; j += blockDim.x) {
-            ext2 a = e2(lt[2 * j], lt[2 * j + 1]);
-            ext2 ar = e2_mul(a, ri);
-            ext2 lo = e2_sub(a, ar);
-            // entry j is read and rewritten only by the thread that owns j
-            lt[2 * (j + half)] = ar.c0;
-            lt[2 * (j + half) + 1] = ar.c1;
-            lt[2 * j] = lo.c0;
-            lt[2 * j + 1] = lo.c1;
-        }
-        __syncthreads();
-    }
-    const ext2 h = e2(hf[0], hf[1]);
-    const size_t base = (size_t)blockIdx.x << low;
-    for (int j = threadIdx.x; j < L; j += blockDim.x) {
-        ext2 v = e2_mul(e2(lt[2 * j], lt[2 * j + 1]), h);
-        if (EXT) e2_store(out, base + j, v);
-        else out[base + j] = v.c0;
-    }
-}
-
 int mam_eq_table(const u64 *d_r, int nv, u64 *d_out, bool ext, hipStream_t s) {
     if (nv < 0 || nv > 40) return -2;
-    const int low = nv < EQ_LOW ? nv : EQ_LOW;
-    unsigned grid = (unsigned)((size_t)1 << (nv - low));
-    if (ext) k_eq_table<true><<<grid, 256, 0, s>>>(d_r, nv, d_out);
-    else k_eq_table<false><<<grid, 256, 0, s>>>(d_r, nv, d_out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    return ext ? eqt::launch<FieldGL, true>(d_r, 0, nv, d_out, 0, 1, s)
+               : eqt::launch<FieldGL, false>(d_r, 0, nv, d_out, 0, 1, s);
 }
 
 MAM_API int eq_dp_all_ffi(const u64 *d_r, u64 *d_buf_a, u64 *d_buf_b, int log_n, u64 **d_result) {

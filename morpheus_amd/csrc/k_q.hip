@@ -7,6 +7,7 @@
 // almost_extension_kernels.cu, almost_eq_lagrange.cuh:25-222, almost_partial_eval.cuh,
 // almost_sumcheck_prover.cuh:19-368); outputs are canonical residues.
 #include "common.h"
+#include "eq_table.h"
 #include "fpolicy.h"
 #include "sumcheck.h"
 
@@ -175,68 +176,13 @@ MAM_API int aext2_to_agl_batch_ffi(const u64 *in, u64 *out, int n) {
 }
 
 // ---------------------------------------------------------------------------
-// eq tables, one pass (same scheme as the Goldilocks k_eq_table): a workgroup owns
-// 2^QEQ_LOW consecutive indices, doubles the low variables' table in LDS, multiplies in its
-// high-variable factor and streams the result out once. Index bit i <-> r_i (LSB first).
-// grid.y = table (batched form); r and out advance by r_stride / out_stride words per table.
+// eq tables (eq_table.h), one pass; grid.y = table in the batched form, where r and out
+// advance by r_stride / out_stride words per table.
 // ---------------------------------------------------------------------------
-static constexpr int QEQ_LOW = 10;
-
-template <class F, bool EXT>
-__global__ __launch_bounds__(256) void k_eq_f(const u64 *__restrict__ r_all, size_t r_stride, int nv,
-                                              u64 *__restrict__ out_all, size_t out_stride) {
-    using E = typename F::E;
-    __shared__ u64 lt[2 << QEQ_LOW];
-    __shared__ u64 hf[2];
-    const u64 *r = r_all + blockIdx.y * r_stride;
-    u64 *out = out_all + blockIdx.y * out_stride;
-    const int low = nv < QEQ_LOW ? nv : QEQ_LOW;
-    const int L = 1 << low;
-    auto ld = [&](int i) { return EXT ? F::e_load(r, i) : F::e_base(F::canon(r[i])); };
-    if (threadIdx.x == 0) {
-        E h = F::e_one();
-        const size_t hb = blockIdx.x;
-        for (int i = low; i < nv; i++) {
-            const E ri = ld(i);
-            h = F::e_mul(h, ((hb >> (i - low)) & 1) ? ri : F::e_sub(F::e_one(), ri));
-        }
-        hf[0] = h.c0;
-        hf[1] = h.c1;
-        lt[0] = 1;
-        lt[1] = 0;
-    }
-    __syncthreads();
-    for (int i = 0; i < low; i++) {
-        const int half = 1 << i;
-        const E ri = ld(i);
-        for (int j = threadIdx.x; j < half; j += blockDim.x) {
-            const E a = F::e(lt[2 * j], lt[2 * j + 1]);
-            const E ar = F::e_mul(a, ri);
-            const E lo = F::e_sub(a, ar);
-            lt[2 * (j + half)] = ar.c0;
-            lt[2 * (j + half) + 1] = ar.c1;
-            lt[2 * j] = lo.c0;
-            lt[2 * j + 1] = lo.c1;
-        }
-        __syncthreads();
-    }
-    const E h = F::e(hf[0], hf[1]);
-    const size_t base = (size_t)blockIdx.x << low;
-    for (int j = threadIdx.x; j < L; j += blockDim.x) {
-        const E v = F::e_mul(F::e(lt[2 * j], lt[2 * j + 1]), h);
-        if (EXT) F::e_store(out, base + j, v);
-        else out[base + j] = v.c0;
-    }
-}
-
 template <class F, bool EXT>
 static int eq_launch(const u64 *d_r, size_t r_stride, int nv, u64 *d_out, size_t out_stride, int tables,
                      hipStream_t s) {
-    if (nv < 0 || nv > 40 || tables <= 0 || tables > 65535) return -1;
-    const int low = nv < QEQ_LOW ? nv : QEQ_LOW;
-    const dim3 grid((unsigned)((size_t)1 << (nv - low)), (unsigned)tables);
-    k_eq_f<F, EXT><<<grid, 256, 0, s>>>(d_r, r_stride, nv, d_out, out_stride);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    return eqt::launch<F, EXT>(d_r, r_stride, nv, d_out, out_stride, tables, s);
 }
 
 // The reference ping-pongs log_n layers between the two buffers, so its result lands in
