@@ -23,8 +23,23 @@ MAM_HD ext2 mam_fold_value(ext2 v0, ext2 v1, ext2 cx, u64 w) {
 // c0 += e0 b0, c1 += e0 b1 + e1 b0, c2 += e1 b1 (acc holds c0, c1, c2 as Ext2 word pairs).
 MAM_HD void mam_oracle_accumulate(u64 *acc, ext2 e0, ext2 e1, ext2 b0, ext2 b1) {
     ext2 t0 = e2_mul(e0, b0);
-    ext2 t1 = e2_add(e2_mul(e0, b1), e2_mul(e1, b0));
     ext2 t2 = e2_mul(e1, b1);
+    // e0 b1 + e1 b0 = (e0 + e1)(b0 + b1) - e0 b0 - e1 b1: three products instead of four
+    ext2 t1 = e2_sub(e2_sub(e2_mul(e2_add(e0, e1), e2_add(b0, b1)), t0), t2);
+    acc[0] = gl_add(acc[0], t0.c0);
+    acc[1] = gl_add(acc[1], t0.c1);
+    acc[2] = gl_add(acc[2], t1.c0);
+    acc[3] = gl_add(acc[3], t1.c1);
+    acc[4] = gl_add(acc[4], t2.c0);
+    acc[5] = gl_add(acc[5], t2.c1);
+}
+
+// The same terms from the two end values of each factor (e(0) = e0, e(1) = e1, likewise b):
+// c1 = e1 b1 - c0 - c2 needs no separate slope products.
+MAM_HD void mam_oracle_accumulate_ends(u64 *acc, ext2 e0, ext2 e1, ext2 b0, ext2 b1) {
+    ext2 t0 = e2_mul(e0, b0);
+    ext2 t2 = e2_mul(e2_sub(e1, e0), e2_sub(b1, b0));
+    ext2 t1 = e2_sub(e2_sub(e2_mul(e1, b1), t0), t2);
     acc[0] = gl_add(acc[0], t0.c0);
     acc[1] = gl_add(acc[1], t0.c1);
     acc[2] = gl_add(acc[2], t1.c0);

@@ -122,6 +122,12 @@ MAM_HD ext2 e2_mul(ext2 a, ext2 b) {
     return ext2{gl_add(m00, gl_mul7(m11)), gl_add(m01, m10)};
 }
 
+// Product with a value k fixed for a whole loop, whose 7 k.c1 (k7 = gl_mul7(k.c1)) was
+// formed once: (k0 + k1 X)(x0 + x1 X) = (k0 x0 + k7 x1) + (k0 x1 + k1 x0) X.
+MAM_HD ext2 e2_mul_fixed(ext2 k, u64 k7, ext2 x) {
+    return ext2{gl_add(gl_mul(k.c0, x.c0), gl_mul(k7, x.c1)), gl_add(gl_mul(k.c0, x.c1), gl_mul(k.c1, x.c0))};
+}
+
 MAM_HD ext2 e2_sqr(ext2 a) {
     u64 m00 = gl_mul(a.c0, a.c0);
     u64 m11 = gl_mul(a.c1, a.c1);

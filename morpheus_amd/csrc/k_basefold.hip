@@ -641,10 +641,10 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_oracle0(const u64 *__restric
         ext2 e0 = e2_load(eq, 2 * i), ehi = e2_load(eq, 2 * i + 1);
         if (BH_BASE) {
             u64 b0 = gl_canon(bh[2 * i]), bhi = gl_canon(bh[2 * i + 1]);
-            mam_oracle_accumulate(acc, e0, e2_sub(ehi, e0), e2_base(b0), e2_base(gl_sub(bhi, b0)));
+            mam_oracle_accumulate_ends(acc, e0, ehi, e2_base(b0), e2_base(bhi));
         } else {
             ext2 b0 = e2_load(bh, 2 * i), bhi = e2_load(bh, 2 * i + 1);
-            mam_oracle_accumulate(acc, e0, e2_sub(ehi, e0), b0, e2_sub(bhi, b0));
+            mam_oracle_accumulate_ends(acc, e0, ehi, b0, bhi);
         }
     }
     block_sum<6>(acc, lds);
@@ -660,11 +660,12 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(const u64 *__res
                                                                 u64 *partials, u32 *counter, u64 *result) {
     __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
     u64 acc[6] = {0, 0, 0, 0, 0, 0};
+    const u64 ch7 = gl_mul7(ch.c1);
     for (size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x; y < half_out; y += (size_t)gridDim.x * blockDim.x) {
         ext2 q0 = e2_load(eq, 4 * y), q1 = e2_load(eq, 4 * y + 1), q2 = e2_load(eq, 4 * y + 2),
              q3 = e2_load(eq, 4 * y + 3);
-        ext2 ea = e2_add(q0, e2_mul(ch, e2_sub(q1, q0)));
-        ext2 eb = e2_add(q2, e2_mul(ch, e2_sub(q3, q2)));
+        ext2 ea = e2_add(q0, e2_mul_fixed(ch, ch7, e2_sub(q1, q0)));
+        ext2 eb = e2_add(q2, e2_mul_fixed(ch, ch7, e2_sub(q3, q2)));
         ext2 ba, bb;
         if (BH_BASE) {
             u64 a0 = gl_canon(bh[4 * y]), a1 = gl_canon(bh[4 * y + 1]);
@@ -674,14 +675,14 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(const u64 *__res
         } else {
             ext2 b0 = e2_load(bh, 4 * y), b1 = e2_load(bh, 4 * y + 1), b2 = e2_load(bh, 4 * y + 2),
                  b3 = e2_load(bh, 4 * y + 3);
-            ba = e2_add(b0, e2_mul(ch, e2_sub(b1, b0)));
-            bb = e2_add(b2, e2_mul(ch, e2_sub(b3, b2)));
+            ba = e2_add(b0, e2_mul_fixed(ch, ch7, e2_sub(b1, b0)));
+            bb = e2_add(b2, e2_mul_fixed(ch, ch7, e2_sub(b3, b2)));
         }
         e2_store(eq_out, 2 * y, ea);
         e2_store(eq_out, 2 * y + 1, eb);
         e2_store(bh_out, 2 * y, ba);
         e2_store(bh_out, 2 * y + 1, bb);
-        mam_oracle_accumulate(acc, ea, e2_sub(eb, ea), ba, e2_sub(bb, ba));
+        mam_oracle_accumulate_ends(acc, ea, eb, ba, bb);
     }
     block_sum<6>(acc, lds);
     grid_finish<6>(acc, lds, partials, counter, result);

@@ -314,15 +314,16 @@ struct PassWords {
     static constexpr int value = 2 * (NP + 1) + (LOOK ? 2 * (NP + 1) * (NP + 1) : 0);
 };
 
+// ca7 = 7 ca.c1, cb7 = 7 cb.c1 (formed once per thread, see e2_mul_fixed)
 template <bool BASE_IN, int FOLDS>
-__device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, ext2 cb) {
+__device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, u64 ca7, ext2 cb, u64 cb7) {
     if (FOLDS == 0) return BASE_IN ? e2_base(gl_canon(p[j])) : e2_load(p, j);
     ext2 v[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) v[k] = BASE_IN ? e2_base(gl_canon(p[4 * j + k])) : e2_load(p, 4 * j + k);
-    const ext2 f0 = e2_add(v[0], e2_mul(ca, e2_sub(v[1], v[0])));
-    const ext2 f1 = e2_add(v[2], e2_mul(ca, e2_sub(v[3], v[2])));
-    return e2_add(f0, e2_mul(cb, e2_sub(f1, f0)));
+    const ext2 f0 = e2_add(v[0], e2_mul_fixed(ca, ca7, e2_sub(v[1], v[0])));
+    const ext2 f1 = e2_add(v[2], e2_mul_fixed(ca, ca7, e2_sub(v[3], v[2])));
+    return e2_add(f0, e2_mul_fixed(cb, cb7, e2_sub(f1, f0)));
 }
 
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
@@ -335,13 +336,14 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ i
     u64 acc[W];
 #pragma unroll
     for (int k = 0; k < W; k++) acc[k] = 0;
+    const u64 ca7 = gl_mul7(ca.c1), cb7 = gl_mul7(cb.c1);
     for (size_t z = blockIdx.x * (size_t)blockDim.x + threadIdx.x; z < items; z += (size_t)gridDim.x * blockDim.x) {
         ext2 e[NP][G];
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             const u64 *p = in + i * in_stride;
 #pragma unroll
-            for (int t = 0; t < G; t++) e[i][t] = pass_load<BASE_IN, FOLDS>(p, G * z + t, ca, cb);
+            for (int t = 0; t < G; t++) e[i][t] = pass_load<BASE_IN, FOLDS>(p, G * z + t, ca, ca7, cb, cb7);
             if (FOLDS > 0) {
 #pragma unroll
                 for (int t = 0; t < G; t++) e2_store(out + i * out_stride, G * z + t, e[i][t]);
@@ -422,7 +424,7 @@ __global__ void k_sc_pass_final(const u64 *__restrict__ in, size_t in_stride, in
     const u64 *p = in + i * in_stride;
     ext2 v;
     if (FOLDS == 2) {
-        v = pass_load<BASE_IN, 2>(p, 0, ca, cb);
+        v = pass_load<BASE_IN, 2>(p, 0, ca, gl_mul7(ca.c1), cb, gl_mul7(cb.c1));
     } else {
         const ext2 a = BASE_IN ? e2_base(gl_canon(p[0])) : e2_load(p, 0);
         const ext2 b = BASE_IN ? e2_base(gl_canon(p[1])) : e2_load(p, 1);
