@@ -394,21 +394,28 @@ __global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, 
     if (t == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s) {
-    if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS) return -2;
+u64 *mam_mailbox_arm(u64 *seq) {
     if (!t_arenas.mailbox) {
         if (hipHostMalloc((void **)&t_arenas.mailbox, (MAM_MAILBOX_WORDS + 1) * sizeof(u64),
                           hipHostMallocCoherent) != hipSuccess) {
             t_arenas.mailbox = nullptr;
-            return -1;
+            return nullptr;
         }
         t_arenas.mailbox[MAM_MAILBOX_WORDS] = 0;
         t_arenas.seq = 0;
     }
-    u64 *box = t_arenas.mailbox;
-    const u64 seq = ++t_arenas.seq;
+    *seq = ++t_arenas.seq;
+    return t_arenas.mailbox;
+}
+
+int mam_mailbox_post(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq, hipStream_t s) {
+    if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS || !box) return -2;
     k_publish<<<1, 64, 0, s>>>(a, na, b, nb, box, seq);
-    if (hipGetLastError() != hipSuccess) return -1;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mam_mailbox_wait(const u64 *box, u64 seq, int n, u64 *out, hipStream_t s) {
+    if (!box || n < 0 || n > MAM_MAILBOX_WORDS) return -2;
     // Spin on the flag; every few thousand polls ask the stream whether it failed or drained.
     for (unsigned it = 1;; it++) {
         if (__atomic_load_n(&box[MAM_MAILBOX_WORDS], __ATOMIC_ACQUIRE) == seq) break;
@@ -422,6 +429,15 @@ int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipSt
         }
         __builtin_ia32_pause();
     }
-    for (int i = 0; i < na + nb; i++) out[i] = box[i];
+    for (int i = 0; i < n; i++) out[i] = box[i];
     return 0;
+}
+
+int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s) {
+    if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS) return -2;
+    u64 seq = 0;
+    u64 *box = mam_mailbox_arm(&seq);
+    if (!box) return -1;
+    if (mam_mailbox_post(a, na, b, nb, box, seq, s)) return -1;
+    return mam_mailbox_wait(box, seq, na + nb, out, s);
 }

@@ -57,6 +57,13 @@ int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, c
 int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,
                          size_t half_out, const RedScratch &rs, hipStream_t s);
 // Fold a codeword (base or Ext2) at alpha into `fold_out` (Ext2) and build the Merkle tree
-// over the folded codeword in the same pass (leaf i = folded elements 2i, 2i+1).
+// over the folded codeword in the same pass (leaf i = folded elements 2i, 2i+1). `oj`
+// (open_fold.h, may be null): the round's eq / bh fold and oracle job, run along with the
+// tree's first launch where it can be, else launched before it. `mj` (may be null): a
+// mailbox post of mj->a's words followed by the root, made by the tree's final launch when
+// it can (*mail_posted true); otherwise the caller posts.
+struct OracleJob;
+struct MailJob;
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
-                      size_t folded_len, int hash, hipStream_t s);
+                      size_t folded_len, int hash, hipStream_t s, const OracleJob *oj = nullptr,
+                      const MailJob *mj = nullptr, bool *mail_posted = nullptr);

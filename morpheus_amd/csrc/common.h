@@ -75,3 +75,27 @@ u64 *mam_host_arena(size_t words);
 // synchronization. Returns 0, -1 (stream error) or -2 (na + nb > MAM_MAILBOX_WORDS).
 static constexpr int MAM_MAILBOX_WORDS = 63;
 int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipStream_t s);
+
+// The same in three steps, for a kernel that posts its own result: arm reserves the next
+// sequence value of the calling thread's mailbox (nullptr on failure); post queues the
+// copy kernel; a kernel may instead write the words itself and call mam_mailbox_raise;
+// wait spins until the value arrived and copies n words.
+u64 *mam_mailbox_arm(u64 *seq);
+int mam_mailbox_post(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq, hipStream_t s);
+int mam_mailbox_wait(const u64 *box, u64 seq, int n, u64 *out, hipStream_t s);
+
+// Device side of a post by one workgroup whose threads have written the mailbox words:
+// make them visible to the host, then release the sequence flag.
+__device__ __forceinline__ void mam_mailbox_raise(u64 *box, u64 seq) {
+    __threadfence_system();
+    __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// A mailbox post handed to a kernel: na words from a, then the kernel's own words.
+struct MailJob {
+    const u64 *a;
+    int na;
+    u64 *box;  // nullptr: no post
+    u64 seq;
+};

@@ -31,6 +31,7 @@
 #include "common.h"
 #include "merkle.h"
 #include "monolith.h"
+#include "open_fold.h"
 #include "poseidon2.h"
 #include "transcript.h"
 
@@ -264,18 +265,31 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     for (int round = 0; round < nv - 1 && !rc; round++) {
         ext2 ch = sample_ext2(t);
         const int pp = round & 1;
-        rc = mam_open_fold_oracle(eq_cur, bh_cur, bh_base, ch, eqp[pp], bhp[pp], sc_size / 4, rs, s);
-        if (rc) break;
+        // the eq / bh fold with the next oracle, run along with the folded codeword's tree
+        const OracleJob job = {eq_cur, bh_cur, ch, eqp[pp], bhp[pp], sc_size / 4, rs.partials,
+                               rs.counter, rs.result, 0, bh_base ? 1 : 0};
         eq_cur = eqp[pp];
         bh_cur = bhp[pp];
         bh_base = false;
         sc_size /= 2;
         const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, round);
-        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s);
+        // the oracle (3 Ext2) and the new root come back in one mailbox round trip, posted by
+        // the tree's final launch where it can
+        u64 seq = 0;
+        u64 *box = mam_mailbox_arm(&seq);
+        if (!box) {
+            rc = -1;
+            break;
+        }
+        const MailJob mail = {rs.result, 6, box, seq};
+        bool posted = false;
+        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s, &job, &mail,
+                               &posted);
         if (rc) break;
         const size_t leaves = flen / 2;
-        // oracle (3 Ext2) and the new root in one mailbox round trip
-        if (mam_publish_wait(rs.result, 6, ftree[round] + mam_merkle_root_offset(leaves), 4, h, s)) {
+        if ((!posted && mam_mailbox_post(rs.result, 6, ftree[round] + mam_merkle_root_offset(leaves), 4, box, seq,
+                                         s)) ||
+            mam_mailbox_wait(box, seq, 10, h, s)) {
             rc = -1;
             break;
         }

@@ -24,6 +24,7 @@
 #include "basefold.h"
 #include "common.h"
 #include "eq_table.h"
+#include "open_fold.h"
 #include "profile.h"
 #include "reduce.h"
 
@@ -651,41 +652,11 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_oracle0(const u64 *__restric
     grid_finish<6>(acc, lds, partials, counter, result);
 }
 
-// Fold eq (Ext2) and bh (base or Ext2) at ch: each output pair y comes from inputs
-// 4y..4y+3; writes the folded arrays and reduces the next oracle in-launch.
+// Fold eq (Ext2) and bh (base or Ext2) at ch and reduce the next oracle (open_fold.h).
 template <bool BH_BASE>
-__global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(const u64 *__restrict__ eq, const u64 *__restrict__ bh,
-                                                                ext2 ch, u64 *__restrict__ eq_out,
-                                                                u64 *__restrict__ bh_out, size_t half_out,
-                                                                u64 *partials, u32 *counter, u64 *result) {
+__global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(OracleJob j) {
     __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
-    u64 acc[6] = {0, 0, 0, 0, 0, 0};
-    const u64 ch7 = gl_mul7(ch.c1);
-    for (size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x; y < half_out; y += (size_t)gridDim.x * blockDim.x) {
-        ext2 q0 = e2_load(eq, 4 * y), q1 = e2_load(eq, 4 * y + 1), q2 = e2_load(eq, 4 * y + 2),
-             q3 = e2_load(eq, 4 * y + 3);
-        ext2 ea = e2_add(q0, e2_mul_fixed(ch, ch7, e2_sub(q1, q0)));
-        ext2 eb = e2_add(q2, e2_mul_fixed(ch, ch7, e2_sub(q3, q2)));
-        ext2 ba, bb;
-        if (BH_BASE) {
-            u64 a0 = gl_canon(bh[4 * y]), a1 = gl_canon(bh[4 * y + 1]);
-            u64 a2 = gl_canon(bh[4 * y + 2]), a3 = gl_canon(bh[4 * y + 3]);
-            ba = e2_add(e2_base(a0), e2_scale(gl_sub(a1, a0), ch));
-            bb = e2_add(e2_base(a2), e2_scale(gl_sub(a3, a2), ch));
-        } else {
-            ext2 b0 = e2_load(bh, 4 * y), b1 = e2_load(bh, 4 * y + 1), b2 = e2_load(bh, 4 * y + 2),
-                 b3 = e2_load(bh, 4 * y + 3);
-            ba = e2_add(b0, e2_mul_fixed(ch, ch7, e2_sub(b1, b0)));
-            bb = e2_add(b2, e2_mul_fixed(ch, ch7, e2_sub(b3, b2)));
-        }
-        e2_store(eq_out, 2 * y, ea);
-        e2_store(eq_out, 2 * y + 1, eb);
-        e2_store(bh_out, 2 * y, ba);
-        e2_store(bh_out, 2 * y + 1, bb);
-        mam_oracle_accumulate_ends(acc, ea, eb, ba, bb);
-    }
-    block_sum<6>(acc, lds);
-    grid_finish<6>(acc, lds, partials, counter, result);
+    open_fold_oracle_run<BH_BASE>(j, blockIdx.x, gridDim.x, lds);
 }
 
 int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, const RedScratch &rs,
@@ -698,15 +669,18 @@ int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, c
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+int mam_open_fold_oracle_job(const OracleJob &j, hipStream_t s) {
+    const unsigned g = grid_reduce(j.half_out);
+    ProfScope prof(PROF_OPEN_SUMCHECK, (double)j.half_out * (64.0 + (j.bh_base ? 32.0 : 64.0) + 128.0), s);
+    if (j.bh_base)
+        k_open_fold_oracle<true><<<g, MAM_BLOCK, 0, s>>>(j);
+    else
+        k_open_fold_oracle<false><<<g, MAM_BLOCK, 0, s>>>(j);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,
                          size_t half_out, const RedScratch &rs, hipStream_t s) {
-    unsigned g = grid_reduce(half_out);
-    ProfScope prof(PROF_OPEN_SUMCHECK, (double)half_out * (64.0 + (bh_base ? 32.0 : 64.0) + 128.0), s);
-    if (bh_base)
-        k_open_fold_oracle<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, ch, eq_out, bh_out, half_out, rs.partials, rs.counter,
-                                                         rs.result);
-    else
-        k_open_fold_oracle<false><<<g, MAM_BLOCK, 0, s>>>(eq, bh, ch, eq_out, bh_out, half_out, rs.partials,
-                                                          rs.counter, rs.result);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    const OracleJob j = {eq, bh, ch, eq_out, bh_out, half_out, rs.partials, rs.counter, rs.result, 0, bh_base ? 1 : 0};
+    return mam_open_fold_oracle_job(j, s);
 }

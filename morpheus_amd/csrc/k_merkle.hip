@@ -18,6 +18,7 @@
 #include "merkle.h"
 #include "monolith.h"
 #include "monolith_wide.h"
+#include "open_fold.h"
 #include "poseidon2.h"
 #include "profile.h"
 
@@ -120,12 +121,26 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_levels(const u64 *__restric
 // ---------------------------------------------------------------------------
 static constexpr int WIDE_GROUPS = 32;
 
+// The last oj.blocks workgroups of a launch run the round's eq / bh fold and oracle job
+// (open_fold.h) instead of tree work; the tree part is the first gridDim.x - oj.blocks.
+// mj.box (only in a one-workgroup launch that ends at the root): post mj.a's words and the
+// root to the host mailbox once the root is written.
 template <int SRC>
 __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__restrict__ src, u64 *__restrict__ tree,
                                                                 size_t first_off, size_t n_first, int nlev,
-                                                                FoldSrc fs) {
+                                                                FoldSrc fs, OracleJob oj, MailJob mj) {
     __shared__ u64 sh[WIDE_GROUPS * 4];
     __shared__ __align__(16) u64 wst[WIDE_GROUPS][16];  // per-group permutation slots
+    if (oj.blocks) {
+        const unsigned tree_blocks = gridDim.x - oj.blocks;
+        if (blockIdx.x >= tree_blocks) {  // workgroup-uniform
+            if (oj.bh_base)
+                open_fold_oracle_run<true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
+            else
+                open_fold_oracle_run<false>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
+            return;
+        }
+    }
     const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
     mono_wide::LaneConsts lc;
     mono_wide::lane_consts(j, lc);
@@ -177,6 +192,12 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
             sh[4 * g + j] = y;
         }
     }
+    if (mj.box) {
+        __syncthreads();  // the root is in sh[0..3]
+        const int t = threadIdx.x;
+        if (t < mj.na + 4) mj.box[t] = t < mj.na ? mj.a[t] : sh[t - mj.na];
+        mam_mailbox_raise(mj.box, mj.seq);
+    }
 }
 
 static size_t narrow_min() {
@@ -202,10 +223,28 @@ static int levels_per_launch(size_t n) {
     return 3;
 }
 
+// oj (may be null): an opening round's fold + oracle job. It rides along the first launch
+// when that launch is lane-parallel (workgroups of the same size), else it is launched on
+// its own before the tree. mj (may be null): a mailbox post that the final launch makes
+// after the root when that launch is one lane-parallel workgroup without the oracle job;
+// *mail_posted tells the caller whether it did.
 template <class H>
 static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_src, const FoldSrc &fs,
-                      hipStream_t s) {
+                      hipStream_t s, const OracleJob *oj = nullptr, const MailJob *mj = nullptr,
+                      bool *mail_posted = nullptr) {
     if (num_leaves == 0 || (num_leaves & (num_leaves - 1))) return -2;
+    if (mail_posted) *mail_posted = false;
+    const OracleJob none = {};
+    const MailJob no_mail = {};
+    OracleJob ride = none;
+    if (oj) {
+        if (H::kWide && num_leaves < narrow_min()) {
+            ride = *oj;
+            ride.blocks = grid_stride(oj->half_out, WIDE_GROUPS * 16, MAM_RED_GRID);
+        } else if (mam_open_fold_oracle_job(*oj, s)) {
+            return -1;
+        }
+    }
     ProfScope prof(PROF_MERKLE, 2.0 * (double)num_leaves - 1.0, s);
     size_t n = num_leaves, off = 0;
     int total_levels = ilog2_sz(num_leaves) + 1;
@@ -218,21 +257,31 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
             if (nl > total_levels - done) nl = total_levels - done;
             unsigned grid = (unsigned)((n + WIDE_GROUPS - 1) / WIDE_GROUPS);
             const int threads = WIDE_GROUPS * 16;
+            const OracleJob &j = first ? ride : none;
+            // the final launch posts the mailbox when it is one workgroup and carries no job
+            const bool post = mj && mj->box && done + nl == total_levels && grid == 1 && j.blocks == 0;
+            const MailJob &m = post ? *mj : no_mail;
             if (first) {
+                const unsigned g2 = grid + j.blocks;
                 switch (leaf_src) {
-                    case SRC_GL: k_tree_wide<SRC_GL><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs); break;
-                    case SRC_EXT2: k_tree_wide<SRC_EXT2><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs); break;
+                    case SRC_GL:
+                        k_tree_wide<SRC_GL><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
+                        break;
+                    case SRC_EXT2:
+                        k_tree_wide<SRC_EXT2><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
+                        break;
                     case SRC_FOLD_GL:
-                        k_tree_wide<SRC_FOLD_GL><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                        k_tree_wide<SRC_FOLD_GL><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
                         break;
                     default:
-                        k_tree_wide<SRC_FOLD_EXT2><<<grid, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                        k_tree_wide<SRC_FOLD_EXT2><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
                         break;
                 }
             } else {
                 const u64 *child = d_tree + 4 * (off - 2 * n);
-                k_tree_wide<SRC_NODES><<<grid, threads, 0, s>>>(child, d_tree, off, n, nl, fs);
+                k_tree_wide<SRC_NODES><<<grid, threads, 0, s>>>(child, d_tree, off, n, nl, fs, none, m);
             }
+            if (post && mail_posted) *mail_posted = true;
             if (hipGetLastError() != hipSuccess) return -1;
             for (int l = 0; l < nl; l++) {
                 off += n;
@@ -287,12 +336,15 @@ int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, 
 }
 
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
-                      size_t folded_len, int hash, hipStream_t s) {
+                      size_t folded_len, int hash, hipStream_t s, const OracleJob *oj, const MailJob *mj,
+                      bool *mail_posted) {
+    if (mail_posted) *mail_posted = false;
     if (folded_len < 2) return -2;
     FoldSrc fs = {table, alpha, fold_out};
     int src = base_cw ? SRC_FOLD_GL : SRC_FOLD_EXT2;
-    if (hash == MAM_HASH_MONOLITH) return build_tree<MonoHash>(cw, tree, folded_len / 2, src, fs, s);
-    if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(cw, tree, folded_len / 2, src, fs, s);
+    if (hash == MAM_HASH_MONOLITH)
+        return build_tree<MonoHash>(cw, tree, folded_len / 2, src, fs, s, oj, mj, mail_posted);
+    if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(cw, tree, folded_len / 2, src, fs, s, oj, mj, mail_posted);
     return -2;
 }
 

@@ -44,13 +44,16 @@ __device__ __forceinline__ void block_sum(u64 (&v)[M], u64 *lds) {
     }
 }
 
-// Grid-wide completion of a reduction. Every thread of every workgroup calls this after
-// block_sum (thread 0 holds the workgroup's sums). `partials` holds gridDim.x * M words.
-// Returns true in exactly one workgroup (the last to finish), whose thread 0 then holds
-// the grid total in v and has written it to `result`.
+// Completion of a reduction over `nblocks` workgroups numbered `bid` = 0 .. nblocks - 1
+// (grid_finish: every workgroup of the launch; grid_finish_n also serves a subset of a
+// launch whose other workgroups do other work). Every thread of each of those workgroups
+// calls it after block_sum (thread 0 holds the workgroup's sums). `partials` holds
+// nblocks * M words. Returns true in exactly one workgroup (the last to finish), whose
+// thread 0 then holds the total in v and has written it to `result`.
 template <int M>
-__device__ __forceinline__ bool grid_finish(u64 (&v)[M], u64 *lds, u64 *partials, u32 *counter, u64 *result) {
-    if (gridDim.x == 1) {  // single workgroup: no ticket, no device-scope fences
+__device__ __forceinline__ bool grid_finish_n(u64 (&v)[M], u64 *lds, u64 *partials, u32 *counter, u64 *result,
+                                              unsigned bid, unsigned nblocks) {
+    if (nblocks == 1) {  // single workgroup: no ticket, no device-scope fences
         if (threadIdx.x == 0) {
 #pragma unroll
             for (int k = 0; k < M; k++) result[k] = v[k];
@@ -60,17 +63,17 @@ __device__ __forceinline__ bool grid_finish(u64 (&v)[M], u64 *lds, u64 *partials
     __shared__ u32 s_last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < M; k++) partials[(size_t)blockIdx.x * M + k] = v[k];
+        for (int k = 0; k < M; k++) partials[(size_t)bid * M + k] = v[k];
         __threadfence();
         u32 ticket = atomicAdd(counter, 1u);
-        s_last = (ticket == gridDim.x - 1) ? 1u : 0u;
+        s_last = (ticket == nblocks - 1) ? 1u : 0u;
     }
     __syncthreads();
     if (!s_last) return false;
     __threadfence();
 #pragma unroll
     for (int k = 0; k < M; k++) v[k] = 0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    for (unsigned b = threadIdx.x; b < nblocks; b += blockDim.x) {
 #pragma unroll
         for (int k = 0; k < M; k++) v[k] = gl_add(v[k], __hip_atomic_load(&partials[(size_t)b * M + k], __ATOMIC_RELAXED,
                                                                    __HIP_MEMORY_SCOPE_AGENT));
@@ -83,4 +86,9 @@ __device__ __forceinline__ bool grid_finish(u64 (&v)[M], u64 *lds, u64 *partials
         __threadfence();
     }
     return true;
+}
+
+template <int M>
+__device__ __forceinline__ bool grid_finish(u64 (&v)[M], u64 *lds, u64 *partials, u32 *counter, u64 *result) {
+    return grid_finish_n<M>(v, lds, partials, counter, result, blockIdx.x, gridDim.x);
 }
