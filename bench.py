@@ -63,8 +63,8 @@ MONO_PERM_VALU = 5291.0
 # 128 MiB on this counter) + WRITE_SIZE (exact for the copy), same profile file.
 LEAF_LAUNCH_TRAFFIC_BYTES = (65.2 + 224.0) * 2 ** 20
 # HBM bytes of one int8 Ajtai GEMM launch (witness 2^22, 256 rows, batch 1): FETCH_SIZE
-# 540765 KiB x 2.0 (copy calibration) + WRITE_SIZE 8192 KiB, from the rocprofv3 --pmc runs.
-AJTAI_GEMM_TRAFFIC_BYTES = (540765.0 * 2.0 + 8192.0) * 1024
+# 540758 KiB x 2.0 (copy calibration) + WRITE_SIZE 8192 KiB, from the rocprofv3 --pmc runs.
+AJTAI_GEMM_TRAFFIC_BYTES = (540758.0 * 2.0 + 8192.0) * 1024
 # Wave-level VALU instructions of k_ring_dense (from its gfx950 ISA): one column fill
 # (ChaCha8 block, rejection check, two LDS rows) and one set bit of one 7-row group.
 RING_PRG_VALU = 440.0
