@@ -272,6 +272,30 @@ def test_ring_commit_sparse():
     assert np.array_equal(R.commit_sparse(seed, []).rows, np.zeros((42, 64), dtype=np.uint64))
 
 
+# (row, column, block) whose first ChaCha8 block holds a word >= q for REJ_KEY (a search of
+# the first 8 M columns): the matrix entries on which the rejection path runs.
+REJ_KEY = bytes((11 * i + 5) % 256 for i in range(32))
+REJECTS = [(40, 2108465, 3), (31, 2660596, 5), (21, 6196264, 3), (0, 7120001, 0)]
+
+
+def test_ring_commit_rejected_samples():
+    print("ring rejection path", flush=True)
+    seed = R.Seed.from_bytes(REJ_KEY)
+    rng = np.random.default_rng(71)
+    for row, col, blk in REJECTS:
+        assert np.array_equal(R.matrix_entry(seed, row, col), O.ring_matrix_entry(REJ_KEY, row, col))
+        off, N = col - 2, 5
+        z = rng.integers(0, 1 << 64, 2 * N, dtype=np.uint64)
+        got = R.commit_batched_at(seed, [z[:N], z[N:]], off, 64)
+        want = O.ring_commit_binary(REJ_KEY, z, N, 2, off)
+        for b in range(2):
+            assert np.array_equal(got[b].rows, want[b]), (row, col, b)
+        zw = words(rng, 64 * N)
+        assert np.array_equal(R.commit_wide(seed, zw, off, 128).rows, O.ring_commit_wide(REJ_KEY, zw, N, off))
+        pos = np.array([64 * col + 8 * blk, 64 * col + 63, 64 * (col - 1) + 5], dtype=np.uint64)
+        assert np.array_equal(R.commit_sparse(seed, pos, 64).rows, O.ring_commit_sparse(REJ_KEY, pos))
+
+
 def lifted(polys, bits, n, leaves, pw):
     """the batched layout with each leaf's f slot replaced by its bits as Ext2 (c1 = 0)"""
     out = polys.copy()

@@ -131,6 +131,35 @@ def test_ring_matrix_is_the_chacha8_stream():
                 assert [int(v) for v in m[8 * blk:8 * blk + 8]] == words
 
 
+# (row, column, block) whose first ChaCha8 block holds a word >= q for REJ_KEY: the only
+# matrix entries on which the rejection path runs (a search of the first 8 M columns).
+REJ_KEY = bytes((11 * i + 5) % 256 for i in range(32))
+REJECTS = [(40, 2108465, 3), (31, 2660596, 5), (21, 6196264, 3), (0, 7120001, 0)]
+
+
+def stream_words(kw, row, col, counter):
+    out = np.zeros(16, dtype=np.uint32)
+    nonce = np.array([row, col & 0xFFFFFFFF, col >> 32], dtype=np.uint32)
+    O.L.or_chacha_block(8, O._u32p(kw), counter, O._u32p(nonce), O._u32p(out))
+    return [(int(out[2 * k + 1]) << 32) | int(out[2 * k]) for k in range(8)]
+
+
+def test_ring_matrix_rejection_path():
+    """A rejected word is skipped and the block's coefficients continue with the accepted
+    words of the blocks at counters blk + 8, blk + 16, ... (ajtai_chacha8.cuh:115-178)."""
+    kw = O.ring_key_words(REJ_KEY)
+    for row, col, blk in REJECTS:
+        words = stream_words(kw, row, col, blk)
+        assert any(w >= Q for w in words), (row, col, blk)
+        accepted = [w for w in words + stream_words(kw, row, col, blk + 8) if w < Q][:8]
+        m = O.ring_matrix_entry(REJ_KEY, row, col)
+        assert [int(v) for v in m[8 * blk:8 * blk + 8]] == accepted
+        assert all(int(v) < Q for v in m)
+        for b in range(8):
+            if b != blk:
+                assert [int(v) for v in m[8 * b:8 * b + 8]] == stream_words(kw, row, col, b)
+
+
 def negacyclic_shift(a, l):
     out = [0] * 64
     for r in range(64):
