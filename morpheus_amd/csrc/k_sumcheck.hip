@@ -12,6 +12,8 @@
 // reduced inside the launch (wave shuffles, LDS, last-workgroup finish), so the host
 // reads back (d+1) Ext2 values per round and never reduces block partials itself.
 // Evaluation points c = 0..d are walked with additions (v_{c+1} = v_c + d).
+#include <stdlib.h>
+
 #include "common.h"
 #include "profile.h"
 #include "reduce.h"
@@ -414,6 +416,197 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ i
     grid_finish<W>(acc, lds, partials, counter, res);
 }
 
+// ---------------------------------------------------------------------------
+// k_sc_pass_lp: the same pass for small arrays, one item spread over LP_LANES lanes. In
+// k_sc_pass one thread carries a whole item (the folds of NP x G inputs, the message products
+// and the round-(r+1) coefficient products, about 10^4 instructions for NP = 2), so a pass
+// over a few thousand items costs that chain's latency however few the items are. Here lane
+// (i, t) of an item forms input e[i][t], then lane k forms one product task: the message
+// term of pair q at point c (k = q (NP + 1) + c), or the round-(r+1) polynomial in ch at
+// point c (k = G / 2 (NP + 1) + c). Each lane keeps the sums of its own task; the lanes of
+// one task are added across the wave, the workgroup and the grid into the same words as
+// k_sc_pass writes.
+// ---------------------------------------------------------------------------
+static constexpr int LP_LANES = 16;
+static constexpr int LP_ITEMS = MAM_BLOCK / LP_LANES;
+
+template <int NP, bool LOOK>
+struct LpTasks {
+    static constexpr int G = LOOK ? 4 : 2;
+    static constexpr int MSG = (G / 2) * (NP + 1);
+    static constexpr int ALL = MSG + (LOOK ? NP + 1 : 0);
+    static constexpr int WORDS = LOOK ? 2 * (NP + 1) : 2;  // accumulator words per lane
+};
+
+template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
+__global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict__ in, size_t in_stride,
+                                                         u64 *__restrict__ out, size_t out_stride, size_t items,
+                                                         ext2 ca, ext2 cb, u64 *partials, u32 *counter, u64 *res) {
+    using T = LpTasks<NP, LOOK>;
+    constexpr int G = T::G, AW = T::WORDS, W = PassWords<NP, LOOK>::value;
+    static_assert(NP * G <= LP_LANES && T::ALL <= LP_LANES, "an item must fit its lanes");
+    static_assert(W <= MAM_BLOCK, "one thread per reduced word");
+    __shared__ u64 ev[LP_ITEMS][NP * G][2];
+    __shared__ u64 red[MAM_BLOCK / WAVE][LP_LANES][AW];
+    __shared__ u64 fin[MAM_BLOCK];
+    __shared__ u32 s_last;
+    const int sub = threadIdx.x % LP_LANES, slot = threadIdx.x / LP_LANES;
+    u64 acc[AW];
+#pragma unroll
+    for (int k = 0; k < AW; k++) acc[k] = 0;
+    const u64 ca7 = gl_mul7(ca.c1), cb7 = gl_mul7(cb.c1);
+    for (size_t base = (size_t)blockIdx.x * LP_ITEMS; base < items; base += (size_t)gridDim.x * LP_ITEMS) {
+        const size_t z = base + slot;
+        const bool live = z < items;
+        if (live && sub < NP * G) {
+            const int i = sub / G, t = sub % G;
+            const ext2 e = pass_load<BASE_IN, FOLDS>(in + i * in_stride, G * z + t, ca, ca7, cb, cb7);
+            if (FOLDS > 0) e2_store(out + i * out_stride, G * z + t, e);
+            ev[slot][sub][0] = e.c0;
+            ev[slot][sub][1] = e.c1;
+        }
+        __syncthreads();
+        if (live && sub < T::ALL) {
+            auto E = [&](int i, int t) { return ext2{ev[slot][i * G + t][0], ev[slot][i * G + t][1]}; };
+            if (sub < T::MSG) {
+                const int q = sub / (NP + 1), c = sub % (NP + 1);
+                ext2 prod{0, 0};
+#pragma unroll
+                for (int i = 0; i < NP; i++) {
+                    const ext2 a = E(i, 2 * q), d = e2_sub(E(i, 2 * q + 1), a);
+                    ext2 v = a;
+                    for (int k = 0; k < c; k++) v = e2_add(v, d);
+                    prod = i == 0 ? v : e2_mul(prod, v);
+                }
+                acc[0] = gl_add(acc[0], prod.c0);
+                acc[1] = gl_add(acc[1], prod.c1);
+            } else if (LOOK) {
+                // coefficients of ch^k in prod_i (u_i(c) + ch w_i(c)), as in k_sc_pass
+                const int c = sub - T::MSG;
+                ext2 u[NP], w[NP];
+#pragma unroll
+                for (int i = 0; i < NP; i++) {
+                    const ext2 e0 = E(i, 0), d01 = e2_sub(E(i, 1), e0);
+                    const ext2 du = e2_sub(E(i, G - 2), e0), dw = e2_sub(e2_sub(E(i, G - 1), E(i, G - 2)), d01);
+                    u[i] = e0;
+                    w[i] = d01;
+                    for (int k = 0; k < c; k++) {
+                        u[i] = e2_add(u[i], du);
+                        w[i] = e2_add(w[i], dw);
+                    }
+                }
+                ext2 poly[NP + 1];
+                poly[0] = u[0];
+                poly[1] = w[0];
+#pragma unroll
+                for (int i = 1; i < NP; i++) {
+                    poly[i + 1] = e2_mul(poly[i], w[i]);
+#pragma unroll
+                    for (int k = i; k >= 1; k--) poly[k] = e2_add(e2_mul(poly[k], u[i]), e2_mul(poly[k - 1], w[i]));
+                    poly[0] = e2_mul(poly[0], u[i]);
+                }
+#pragma unroll
+                for (int k = 0; k <= NP; k++) {
+                    acc[2 * k] = gl_add(acc[2 * k], poly[k].c0);
+                    acc[2 * k + 1] = gl_add(acc[2 * k + 1], poly[k].c1);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    // lanes of one task: across the wave (lane offsets 16, 32), then across the waves
+#pragma unroll
+    for (int k = 0; k < AW; k++) {
+        acc[k] = gl_add(acc[k], shfl_xor_u64(acc[k], 16));
+        acc[k] = gl_add(acc[k], shfl_xor_u64(acc[k], 32));
+    }
+    const int lane = threadIdx.x % WAVE, wave = threadIdx.x / WAVE;
+    if (lane < LP_LANES) {
+#pragma unroll
+        for (int k = 0; k < AW; k++) red[wave][lane][k] = acc[k];
+    }
+    __syncthreads();
+    // thread t < W forms word t of the workgroup's sums: msg[c] (t = 2 c + h), then
+    // sym[c][k] (t = 2 (NP + 1) + 2 (c (NP + 1) + k) + h)
+    u64 word = 0;
+    if ((int)threadIdx.x < W) {
+        const int t = threadIdx.x;
+        for (int wv = 0; wv < MAM_BLOCK / WAVE; wv++) {
+            if (t < 2 * (NP + 1)) {
+                for (int q = 0; q < G / 2; q++) word = gl_add(word, red[wv][q * (NP + 1) + t / 2][t % 2]);
+            } else {
+                const int o = t - 2 * (NP + 1);
+                word = gl_add(word, red[wv][T::MSG + o / (2 * (NP + 1))][o % (2 * (NP + 1))]);
+            }
+        }
+    }
+    if (gridDim.x == 1) {
+        if ((int)threadIdx.x < W) res[threadIdx.x] = word;
+        return;
+    }
+    // ticketed grid reduction (as grid_finish), the last workgroup adding with W x L threads
+    if ((int)threadIdx.x < W) partials[(size_t)blockIdx.x * W + threadIdx.x] = word;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = atomicAdd(counter, 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();
+    constexpr int L = MAM_BLOCK / W;  // threads per word
+    u64 v = 0;
+    if ((int)threadIdx.x < L * W) {
+        // eight loads in flight per trip: the workgroups' partials are read at L2 latency
+        const int k = threadIdx.x % W;
+        unsigned b = threadIdx.x / W;
+        for (; b + 7 * L < gridDim.x; b += 8 * L) {
+            u64 t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+                t[u] = __hip_atomic_load(&partials[(size_t)(b + u * L) * W + k], __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+            for (int u = 0; u < 8; u++) v = gl_add(v, t[u]);
+        }
+        for (; b < gridDim.x; b += L)
+            v = gl_add(v, __hip_atomic_load(&partials[(size_t)b * W + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    fin[threadIdx.x] = v;
+    __syncthreads();
+    if ((int)threadIdx.x < W) {
+        u64 sum = 0;
+        for (int j = 0; j < L; j++) sum = gl_add(sum, fin[j * W + threadIdx.x]);
+        res[threadIdx.x] = sum;
+    }
+    if (threadIdx.x == 0) {
+        *counter = 0;
+        __threadfence();
+    }
+}
+
+// Workgroups of a pass launch: MAM_SC_GRID (default MAM_RED_GRID), at most what the partials
+// scratch holds for W reduced words per workgroup.
+static unsigned pass_grid_cap(int w) {
+    static long v = -1;
+    if (v < 0) {
+        const char *e = getenv("MAM_SC_GRID");
+        v = e ? strtol(e, nullptr, 10) : (long)MAM_RED_GRID;
+        if (v < 1) v = 1;
+    }
+    const long room = (long)MAM_RED_MAX_BLOCKS * 2 * (MAM_MAX_POLYS + 1) / w;
+    return (unsigned)(v < room ? v : room);
+}
+
+// Largest item count that runs on k_sc_pass_lp (MAM_SC_LP_MAX overrides; 0 disables it).
+static size_t lp_max_items() {
+    static long v = -1;
+    if (v < 0) {
+        const char *e = getenv("MAM_SC_LP_MAX");
+        v = e ? strtol(e, nullptr, 10) : (1L << 14);
+        if (v < 0) v = 0;
+    }
+    return (size_t)v;
+}
+
 // The final values: arrays of 2^FOLDS elements folded at the pending challenges (one
 // thread per polynomial).
 template <bool BASE_IN, int FOLDS>
@@ -436,13 +629,22 @@ __global__ void k_sc_pass_final(const u64 *__restrict__ in, size_t in_stride, in
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t items, ext2 ca,
                          ext2 cb, const ScScratch &s, hipStream_t st) {
-    unsigned g = grid_reduce(items);
+    unsigned g = grid_stride(items, MAM_BLOCK, MAM_RED_MAX_BLOCKS);
     ProfScope prof(PROF_SUMCHECK,
                    (double)items * NP * ((LOOK ? 4 : 2) * (FOLDS ? 4 : 1) * (BASE_IN ? 8.0 : 16.0) +
                                          (FOLDS ? (LOOK ? 4 : 2) * 16.0 : 0.0)),
                    st);
-    k_sc_pass<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca, cb,
-                                                                 s.partials, s.counter, s.msg);
+    const unsigned cap = pass_grid_cap(PassWords<NP, LOOK>::value);
+    if (g > cap) g = cap;
+    if (items <= lp_max_items()) {
+        const size_t want = (items + LP_ITEMS - 1) / LP_ITEMS;
+        const unsigned glp = (unsigned)(want < cap ? (want ? want : 1) : cap);
+        k_sc_pass_lp<NP, BASE_IN, FOLDS, LOOK><<<glp, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca,
+                                                                         cb, s.partials, s.counter, s.msg);
+    } else {
+        k_sc_pass<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca, cb,
+                                                                     s.partials, s.counter, s.msg);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
