@@ -635,7 +635,7 @@ MAM_API int partial_eval_ext2_from_gl_ffi(const u64 *d_input, u64 *d_output, con
 // Round 0: eq (Ext2) and bh (base or Ext2), natural order; oracle coefficients reduced in-launch.
 template <bool BH_BASE>
 __global__ __launch_bounds__(MAM_BLOCK) void k_open_oracle0(const u64 *__restrict__ eq, const u64 *__restrict__ bh,
-                                                            size_t pairs, u64 *partials, u32 *counter, u64 *result) {
+                                                            size_t pairs, u64 *partials, u64 *result) {
     __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
     u64 acc[6] = {0, 0, 0, 0, 0, 0};
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < pairs; i += (size_t)gridDim.x * blockDim.x) {
@@ -649,23 +649,24 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_oracle0(const u64 *__restric
         }
     }
     block_sum<6>(acc, lds);
-    grid_finish<6>(acc, lds, partials, counter, result);
+    grid_partials<6>(acc, partials, result);
 }
 
 // Fold eq (Ext2) and bh (base or Ext2) at ch and reduce the next oracle (open_fold.h).
 template <bool BH_BASE>
 __global__ __launch_bounds__(MAM_BLOCK) void k_open_fold_oracle(OracleJob j) {
     __shared__ u64 lds[6 * (MAM_BLOCK / WAVE)];
-    open_fold_oracle_run<BH_BASE>(j, blockIdx.x, gridDim.x, lds);
+    open_fold_oracle_run<BH_BASE, false>(j, blockIdx.x, gridDim.x, lds);
 }
 
 int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, const RedScratch &rs,
                      hipStream_t s) {
     unsigned g = grid_reduce(pairs);
     if (bh_base)
-        k_open_oracle0<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.counter, rs.result);
+        k_open_oracle0<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.result);
     else
-        k_open_oracle0<false><<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.counter, rs.result);
+        k_open_oracle0<false><<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.result);
+    launch_sum_partials<6>(rs.partials, g, rs.result, s);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -676,6 +677,7 @@ int mam_open_fold_oracle_job(const OracleJob &j, hipStream_t s) {
         k_open_fold_oracle<true><<<g, MAM_BLOCK, 0, s>>>(j);
     else
         k_open_fold_oracle<false><<<g, MAM_BLOCK, 0, s>>>(j);
+    launch_sum_partials<6>(j.partials, g, j.result, s);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

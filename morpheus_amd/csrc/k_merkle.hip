@@ -135,9 +135,9 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
         const unsigned tree_blocks = gridDim.x - oj.blocks;
         if (blockIdx.x >= tree_blocks) {  // workgroup-uniform
             if (oj.bh_base)
-                open_fold_oracle_run<true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
+                open_fold_oracle_run<true, true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
             else
-                open_fold_oracle_run<false>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
+                open_fold_oracle_run<false, true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
             return;
         }
     }

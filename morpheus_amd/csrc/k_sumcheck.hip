@@ -112,7 +112,7 @@ MAM_API int sumcheck_fold_ext2_ffi(const u64 *d_input, u64 *d_output, u64 challe
 // Round-0 message straight from the input polynomials (base field or Ext2).
 template <int NP, bool BASE>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_msg(const u64 *__restrict__ in, size_t stride, size_t half,
-                                                     u64 *partials, u32 *counter, u64 *msg) {
+                                                     u64 *partials, u64 *msg) {
     __shared__ u64 lds[2 * (NP + 1) * (MAM_BLOCK / WAVE)];
     u64 acc[2 * (NP + 1)];
 #pragma unroll
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_msg(const u64 *__restrict__ in
         }
     }
     block_sum<2 * (NP + 1)>(acc, lds);
-    grid_finish<2 * (NP + 1)>(acc, lds, partials, counter, msg);
+    grid_partials<2 * (NP + 1)>(acc, partials, msg);
 }
 
 // Fold the current arrays (each 4*half_out elements) at `ch` into `out` (2*half_out
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_msg(const u64 *__restrict__ in
 template <int NP, bool BASE_IN>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_fold_msg(const u64 *__restrict__ in, size_t in_stride,
                                                           u64 *__restrict__ out, size_t out_stride, size_t half_out,
-                                                          ext2 ch, u64 *partials, u32 *counter, u64 *msg) {
+                                                          ext2 ch, u64 *partials, u64 *msg) {
     __shared__ u64 lds[2 * (NP + 1) * (MAM_BLOCK / WAVE)];
     u64 acc[2 * (NP + 1)];
 #pragma unroll
@@ -214,7 +214,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_fold_msg(const u64 *__restrict
         }
     }
     block_sum<2 * (NP + 1)>(acc, lds);
-    grid_finish<2 * (NP + 1)>(acc, lds, partials, counter, msg);
+    grid_partials<2 * (NP + 1)>(acc, partials, msg);
 }
 
 // Last fold: two elements per polynomial -> one final value each, written to `finals`.
@@ -232,9 +232,10 @@ template <int NP>
 static int launch_msg(const u64 *in, size_t stride, size_t half, bool base, const ScScratch &s, hipStream_t st) {
     unsigned g = grid_reduce(half);
     if (base)
-        k_sc_msg<NP, true><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.counter, s.msg);
+        k_sc_msg<NP, true><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.msg);
     else
-        k_sc_msg<NP, false><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.counter, s.msg);
+        k_sc_msg<NP, false><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.msg);
+    launch_sum_partials<2 * (NP + 1)>(s.partials, g, s.msg, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -246,10 +247,11 @@ static int launch_fold_msg(const u64 *in, size_t in_stride, u64 *out, size_t out
     ProfScope prof(PROF_SUMCHECK, (double)half_out * NP * ((base_in ? 32.0 : 64.0) + 32.0), st);
     if (base_in)
         k_sc_fold_msg<NP, true><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, half_out, ch, s.partials,
-                                                         s.counter, s.msg);
+                                                         s.msg);
     else
         k_sc_fold_msg<NP, false><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, half_out, ch, s.partials,
-                                                          s.counter, s.msg);
+                                                          s.msg);
+    launch_sum_partials<2 * (NP + 1)>(s.partials, g, s.msg, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -331,7 +333,7 @@ __device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, u64 c
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ in, size_t in_stride,
                                                       u64 *__restrict__ out, size_t out_stride, size_t items,
-                                                      ext2 ca, ext2 cb, u64 *partials, u32 *counter, u64 *res) {
+                                                      ext2 ca, ext2 cb, u64 *partials, u64 *res) {
     constexpr int G = LOOK ? 4 : 2;
     constexpr int W = PassWords<NP, LOOK>::value;
     __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
@@ -339,6 +341,74 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ i
 #pragma unroll
     for (int k = 0; k < W; k++) acc[k] = 0;
     const u64 ca7 = gl_mul7(ca.c1), cb7 = gl_mul7(cb.c1);
+    if constexpr (BASE_IN && FOLDS == 0) {
+        // unfolded base inputs: every product is a base-field product (the c1 words stay 0)
+        for (size_t z = blockIdx.x * (size_t)blockDim.x + threadIdx.x; z < items;
+             z += (size_t)gridDim.x * blockDim.x) {
+            u64 e[NP][G];
+#pragma unroll
+            for (int i = 0; i < NP; i++) {
+#pragma unroll
+                for (int t = 0; t < G; t++) e[i][t] = gl_canon(in[i * in_stride + G * z + t]);
+            }
+#pragma unroll
+            for (int q = 0; q < G / 2; q++) {
+                u64 v[NP], d[NP];
+#pragma unroll
+                for (int i = 0; i < NP; i++) {
+                    v[i] = e[i][2 * q];
+                    d[i] = gl_sub(e[i][2 * q + 1], e[i][2 * q]);
+                }
+#pragma unroll
+                for (int c = 0; c <= NP; c++) {
+                    u64 prod = v[0];
+#pragma unroll
+                    for (int i = 1; i < NP; i++) prod = gl_mul(prod, v[i]);
+                    acc[2 * c] = gl_add(acc[2 * c], prod);
+                    if (c < NP) {
+#pragma unroll
+                        for (int i = 0; i < NP; i++) v[i] = gl_add(v[i], d[i]);
+                    }
+                }
+            }
+            if (LOOK) {
+                u64 u[NP], du[NP], w[NP], dw[NP];
+#pragma unroll
+                for (int i = 0; i < NP; i++) {
+                    const u64 d01 = gl_sub(e[i][1], e[i][0]), d23 = gl_sub(e[i][G - 1], e[i][G - 2]);
+                    u[i] = e[i][0];
+                    du[i] = gl_sub(e[i][G - 2], e[i][0]);
+                    w[i] = d01;
+                    dw[i] = gl_sub(d23, d01);
+                }
+#pragma unroll
+                for (int c = 0; c <= NP; c++) {
+                    u64 poly[NP + 1];
+                    poly[0] = u[0];
+                    poly[1] = w[0];
+#pragma unroll
+                    for (int i = 1; i < NP; i++) {
+                        poly[i + 1] = gl_mul(poly[i], w[i]);
+#pragma unroll
+                        for (int k = i; k >= 1; k--) poly[k] = gl_add(gl_mul(poly[k], u[i]), gl_mul(poly[k - 1], w[i]));
+                        poly[0] = gl_mul(poly[0], u[i]);
+                    }
+#pragma unroll
+                    for (int k = 0; k <= NP; k++) {
+                        const int o = 2 * (NP + 1) + 2 * (c * (NP + 1) + k);
+                        acc[o] = gl_add(acc[o], poly[k]);
+                    }
+                    if (c < NP) {
+#pragma unroll
+                        for (int i = 0; i < NP; i++) {
+                            u[i] = gl_add(u[i], du[i]);
+                            w[i] = gl_add(w[i], dw[i]);
+                        }
+                    }
+                }
+            }
+        }
+    } else
     for (size_t z = blockIdx.x * (size_t)blockDim.x + threadIdx.x; z < items; z += (size_t)gridDim.x * blockDim.x) {
         ext2 e[NP][G];
 #pragma unroll
@@ -413,7 +483,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ i
         }
     }
     block_sum<W>(acc, lds);
-    grid_finish<W>(acc, lds, partials, counter, res);
+    grid_partials<W>(acc, partials, res);
 }
 
 // ---------------------------------------------------------------------------
@@ -441,15 +511,13 @@ struct LpTasks {
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict__ in, size_t in_stride,
                                                          u64 *__restrict__ out, size_t out_stride, size_t items,
-                                                         ext2 ca, ext2 cb, u64 *partials, u32 *counter, u64 *res) {
+                                                         ext2 ca, ext2 cb, u64 *partials, u64 *res) {
     using T = LpTasks<NP, LOOK>;
     constexpr int G = T::G, AW = T::WORDS, W = PassWords<NP, LOOK>::value;
     static_assert(NP * G <= LP_LANES && T::ALL <= LP_LANES, "an item must fit its lanes");
     static_assert(W <= MAM_BLOCK, "one thread per reduced word");
     __shared__ u64 ev[LP_ITEMS][NP * G][2];
     __shared__ u64 red[MAM_BLOCK / WAVE][LP_LANES][AW];
-    __shared__ u64 fin[MAM_BLOCK];
-    __shared__ u32 s_last;
     const int sub = threadIdx.x % LP_LANES, slot = threadIdx.x / LP_LANES;
     u64 acc[AW];
 #pragma unroll
@@ -540,47 +608,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict_
             }
         }
     }
-    if (gridDim.x == 1) {
-        if ((int)threadIdx.x < W) res[threadIdx.x] = word;
-        return;
-    }
-    // ticketed grid reduction (as grid_finish), the last workgroup adding with W x L threads
-    if ((int)threadIdx.x < W) partials[(size_t)blockIdx.x * W + threadIdx.x] = word;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_last = atomicAdd(counter, 1u) == gridDim.x - 1 ? 1u : 0u;
-    __syncthreads();
-    if (!s_last) return;
-    __threadfence();
-    constexpr int L = MAM_BLOCK / W;  // threads per word
-    u64 v = 0;
-    if ((int)threadIdx.x < L * W) {
-        // eight loads in flight per trip: the workgroups' partials are read at L2 latency
-        const int k = threadIdx.x % W;
-        unsigned b = threadIdx.x / W;
-        for (; b + 7 * L < gridDim.x; b += 8 * L) {
-            u64 t[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++)
-                t[u] = __hip_atomic_load(&partials[(size_t)(b + u * L) * W + k], __ATOMIC_RELAXED,
-                                         __HIP_MEMORY_SCOPE_AGENT);
-#pragma unroll
-            for (int u = 0; u < 8; u++) v = gl_add(v, t[u]);
-        }
-        for (; b < gridDim.x; b += L)
-            v = gl_add(v, __hip_atomic_load(&partials[(size_t)b * W + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-    }
-    fin[threadIdx.x] = v;
-    __syncthreads();
-    if ((int)threadIdx.x < W) {
-        u64 sum = 0;
-        for (int j = 0; j < L; j++) sum = gl_add(sum, fin[j * W + threadIdx.x]);
-        res[threadIdx.x] = sum;
-    }
-    if (threadIdx.x == 0) {
-        *counter = 0;
-        __threadfence();
-    }
+    if ((int)threadIdx.x < W) (gridDim.x == 1 ? res : partials + (size_t)blockIdx.x * W)[threadIdx.x] = word;
 }
 
 // Workgroups of a pass launch: MAM_SC_GRID (default MAM_RED_GRID), at most what the partials
@@ -639,12 +667,14 @@ static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_s
     if (items <= lp_max_items()) {
         const size_t want = (items + LP_ITEMS - 1) / LP_ITEMS;
         const unsigned glp = (unsigned)(want < cap ? (want ? want : 1) : cap);
-        k_sc_pass_lp<NP, BASE_IN, FOLDS, LOOK><<<glp, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca,
-                                                                         cb, s.partials, s.counter, s.msg);
+        g = glp;
+        k_sc_pass_lp<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca,
+                                                                       cb, s.partials, s.msg);
     } else {
         k_sc_pass<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca, cb,
-                                                                     s.partials, s.counter, s.msg);
+                                                                     s.partials, s.msg);
     }
+    launch_sum_partials<PassWords<NP, LOOK>::value>(s.partials, g, s.msg, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

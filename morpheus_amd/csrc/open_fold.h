@@ -24,7 +24,10 @@ struct OracleJob {
 };
 
 // `lds` holds 6 * (blockDim.x / 64) words; bid / nblocks number the job's workgroups.
-template <bool BH_BASE>
+// TICKET: finish the oracle with the one-launch ticketed reduction (the job rides along a
+// tree launch); otherwise leave the workgroups' partials for k_sum_partials (a launch of its
+// own, whose caller adds them in a second launch).
+template <bool BH_BASE, bool TICKET>
 __device__ __forceinline__ void open_fold_oracle_run(const OracleJob &j, unsigned bid, unsigned nblocks, u64 *lds) {
     u64 acc[6] = {0, 0, 0, 0, 0, 0};
     const ext2 ch = j.ch;
@@ -53,7 +56,10 @@ __device__ __forceinline__ void open_fold_oracle_run(const OracleJob &j, unsigne
         mam_oracle_accumulate_ends(acc, ea, eb, ba, bb);
     }
     block_sum<6>(acc, lds);
-    grid_finish_n<6>(acc, lds, j.partials, j.counter, j.result, bid, nblocks);
+    if (TICKET)
+        grid_finish_n<6>(acc, lds, j.partials, j.counter, j.result, bid, nblocks);
+    else
+        grid_partials<6>(acc, j.partials, j.result);
 }
 
 // The job as its own launch when it cannot ride along a tree launch (k_basefold.hip).

@@ -1,10 +1,16 @@
 // Workgroup and grid reductions of Goldilocks sums for 64-wide wavefronts.
 //
 // block_sum: wave-level butterfly with 32-bit lane shuffles (DPP/ds_swizzle on CDNA),
-// then one LDS slot per wave. grid_finish: each workgroup publishes its partial, takes
-// a ticket from a device-scope counter, and the last workgroup to arrive adds all
-// partials and writes the final value, then re-arms the counter at zero so the next
-// launch (or a graph replay) needs no memset.
+// then one LDS slot per wave.
+//
+// Grid reductions come in two forms. Two launches (grid_partials + k_sum_partials): each
+// workgroup writes its partial and ends, and a one-workgroup launch adds them. One launch
+// (grid_finish): each workgroup publishes its partial, takes a ticket from a device-scope
+// counter, and the last workgroup to arrive adds all partials and re-arms the counter. The
+// ticket needs a device-scope release per workgroup, which on this GPU (one L2 per XCD)
+// writes back the XCD's L2; measured on the sumcheck passes, the one-launch form cost
+// 10-25 us per launch more than the second launch, so the standalone kernels use two
+// launches and grid_finish serves only work that rides along another kernel's launch.
 #pragma once
 
 #include "field.h"
@@ -91,4 +97,51 @@ __device__ __forceinline__ bool grid_finish_n(u64 (&v)[M], u64 *lds, u64 *partia
 template <int M>
 __device__ __forceinline__ bool grid_finish(u64 (&v)[M], u64 *lds, u64 *partials, u32 *counter, u64 *result) {
     return grid_finish_n<M>(v, lds, partials, counter, result, blockIdx.x, gridDim.x);
+}
+
+// Two-launch form, first half: a single workgroup writes the total to `result`, otherwise
+// each workgroup writes its M words to partials[blockIdx.x * M ..] (thread 0 holds the
+// workgroup's sums after block_sum). The caller launches k_sum_partials when gridDim.x > 1.
+template <int M>
+__device__ __forceinline__ void grid_partials(const u64 (&v)[M], u64 *partials, u64 *result) {
+    if (threadIdx.x != 0) return;
+    u64 *dst = gridDim.x == 1 ? result : partials + (size_t)blockIdx.x * M;
+#pragma unroll
+    for (int k = 0; k < M; k++) dst[k] = v[k];
+}
+
+// Second half: result[k] = sum of partials[b * M + k] over b < nblocks (one workgroup of
+// blockDim.x >= M threads, L = blockDim.x / M of them per word, eight loads in flight each).
+template <int M>
+__global__ __launch_bounds__(256) void k_sum_partials(const u64 *__restrict__ partials, unsigned nblocks,
+                                                     u64 *__restrict__ result) {
+    static_assert(M <= 256, "one thread per word");
+    __shared__ u64 fin[256];
+    const int L = blockDim.x / M;
+    u64 v = 0;
+    if ((int)threadIdx.x < L * M) {
+        const int k = threadIdx.x % M;
+        unsigned b = threadIdx.x / M;
+        for (; b + 7 * L < nblocks; b += 8 * L) {
+            u64 t[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) t[u] = partials[(size_t)(b + u * L) * M + k];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v = gl_add(v, t[u]);
+        }
+        for (; b < nblocks; b += L) v = gl_add(v, partials[(size_t)b * M + k]);
+    }
+    fin[threadIdx.x] = v;
+    __syncthreads();
+    if ((int)threadIdx.x < M) {
+        u64 sum = 0;
+        for (int j = 0; j < L; j++) sum = gl_add(sum, fin[j * M + threadIdx.x]);
+        result[threadIdx.x] = sum;
+    }
+}
+
+// Host side of the second half (nothing to add for a single workgroup).
+template <int M>
+static inline void launch_sum_partials(const u64 *partials, unsigned nblocks, u64 *result, hipStream_t s) {
+    if (nblocks > 1) k_sum_partials<M><<<1, 256, 0, s>>>(partials, nblocks, result);
 }
