@@ -226,9 +226,16 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
     int rc = 0;
     for (int k = 0; k < P && !rc; k++) {
         const Pass &p = passes[k];
-        rc = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride, p.items, ca,
-                                cb, S.s, S.st);
-        if (!rc) rc = mam_publish_wait(S.s.msg, words(p), nullptr, 0, S.host, S.st);
+        u64 seq = 0;
+        u64 *box = mam_mailbox_arm(&seq);
+        if (!box) rc = -1;
+        const MailJob mail = {nullptr, 0, box, seq};
+        bool posted = false;
+        if (!rc)
+            rc = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride, p.items,
+                                    ca, cb, S.s, S.st, &mail, &posted);
+        if (!rc && !posted) rc = mam_mailbox_post(S.s.msg, words(p), nullptr, 0, box, seq, S.st);
+        if (!rc) rc = mam_mailbox_wait(box, seq, words(p), S.host, S.st);
         if (rc) break;
         u64 *m0 = round_msgs + (size_t)p.r * dp1 * 2;
         memcpy(m0, S.host, dp1 * 2 * sizeof(u64));

@@ -656,7 +656,7 @@ __global__ void k_sc_pass_final(const u64 *__restrict__ in, size_t in_stride, in
 
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t items, ext2 ca,
-                         ext2 cb, const ScScratch &s, hipStream_t st) {
+                         ext2 cb, const ScScratch &s, hipStream_t st, const MailJob *mj, bool *posted) {
     unsigned g = grid_stride(items, MAM_BLOCK, MAM_RED_MAX_BLOCKS);
     ProfScope prof(PROF_SUMCHECK,
                    (double)items * NP * ((LOOK ? 4 : 2) * (FOLDS ? 4 : 1) * (BASE_IN ? 8.0 : 16.0) +
@@ -674,16 +674,18 @@ static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_s
         k_sc_pass<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca, cb,
                                                                      s.partials, s.msg);
     }
-    launch_sum_partials<PassWords<NP, LOOK>::value>(s.partials, g, s.msg, st);
+    const bool p = launch_sum_partials<PassWords<NP, LOOK>::value>(s.partials, g, s.msg, st, mj);
+    if (posted) *posted = p;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 template <int NP>
 static int launch_pass_np(const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
-                          size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st) {
+                          size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
+                          const MailJob *mj, bool *posted) {
 #define MAM_PASS(B, F, L)                                                                            \
     if (base_in == B && folds == F && look == L)                                                     \
-        return launch_pass_t<NP, B, F, L>(in, in_stride, out, out_stride, items, ca, cb, s, st);
+        return launch_pass_t<NP, B, F, L>(in, in_stride, out, out_stride, items, ca, cb, s, st, mj, posted);
     MAM_PASS(true, 0, true)
     MAM_PASS(true, 0, false)
     MAM_PASS(true, 2, true)
@@ -697,11 +699,19 @@ static int launch_pass_np(const u64 *in, size_t in_stride, bool base_in, int fol
 }
 
 int mam_sc_launch_pass(int np, const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
-                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st) {
+                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
+                       const MailJob *mj, bool *posted) {
+    if (posted) *posted = false;
     switch (np) {
-        case 1: return launch_pass_np<1>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st);
-        case 2: return launch_pass_np<2>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st);
-        case 3: return launch_pass_np<3>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st);
+        case 1:
+            return launch_pass_np<1>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, mj,
+                                     posted);
+        case 2:
+            return launch_pass_np<2>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, mj,
+                                     posted);
+        case 3:
+            return launch_pass_np<3>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, mj,
+                                     posted);
         default: return -2;
     }
 }

@@ -13,6 +13,7 @@
 // launches and grid_finish serves only work that rides along another kernel's launch.
 #pragma once
 
+#include "common.h"
 #include "field.h"
 
 static constexpr int WAVE = 64;
@@ -112,9 +113,10 @@ __device__ __forceinline__ void grid_partials(const u64 (&v)[M], u64 *partials, 
 
 // Second half: result[k] = sum of partials[b * M + k] over b < nblocks (one workgroup of
 // blockDim.x >= M threads, L = blockDim.x / M of them per word, eight loads in flight each).
+// With mj.box set, the M words are also posted to that host mailbox (mj.a is not used).
 template <int M>
 __global__ __launch_bounds__(256) void k_sum_partials(const u64 *__restrict__ partials, unsigned nblocks,
-                                                     u64 *__restrict__ result) {
+                                                     u64 *__restrict__ result, MailJob mj) {
     static_assert(M <= 256, "one thread per word");
     __shared__ u64 fin[256];
     const int L = blockDim.x / M;
@@ -137,11 +139,18 @@ __global__ __launch_bounds__(256) void k_sum_partials(const u64 *__restrict__ pa
         u64 sum = 0;
         for (int j = 0; j < L; j++) sum = gl_add(sum, fin[j * M + threadIdx.x]);
         result[threadIdx.x] = sum;
+        if (mj.box) mj.box[threadIdx.x] = sum;
     }
+    if (mj.box) mam_mailbox_raise(mj.box, mj.seq);
 }
 
-// Host side of the second half (nothing to add for a single workgroup).
+// Host side of the second half (nothing to add for a single workgroup). mj (may be null): a
+// mailbox post of the M words made by that launch; returns whether it posted.
 template <int M>
-static inline void launch_sum_partials(const u64 *partials, unsigned nblocks, u64 *result, hipStream_t s) {
-    if (nblocks > 1) k_sum_partials<M><<<1, 256, 0, s>>>(partials, nblocks, result);
+static inline bool launch_sum_partials(const u64 *partials, unsigned nblocks, u64 *result, hipStream_t s,
+                                       const MailJob *mj = nullptr) {
+    if (nblocks <= 1) return false;
+    const MailJob none = {};
+    k_sum_partials<M><<<1, 256, 0, s>>>(partials, nblocks, result, mj ? *mj : none);
+    return mj && mj->box;
 }

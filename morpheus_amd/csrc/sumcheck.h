@@ -5,6 +5,8 @@
 
 #include "field.h"
 
+struct MailJob;
+
 #define MAM_MAX_POLYS 8
 // Upper bound on workgroups of a grid-wide reduction launch (sizes the partials buffer).
 #define MAM_RED_MAX_BLOCKS 2048
@@ -23,7 +25,10 @@ int mam_sc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_
                              hipStream_t st);
 // Two rounds per pass (whole-proof prover, np <= MAM_LOOK_MAX_POLYS): see k_sumcheck.hip.
 #define MAM_LOOK_MAX_POLYS 3
+// mj (may be null): a post of the pass's words to that host mailbox, made by the launch that
+// adds the workgroup partials when there is one; *posted tells whether it did.
 int mam_sc_launch_pass(int np, const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
-                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st);
+                       size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
+                       const MailJob *mj = nullptr, bool *posted = nullptr);
 int mam_sc_launch_pass_final(int np, const u64 *in, size_t in_stride, bool base_in, int folds, ext2 ca, ext2 cb,
                              u64 *finals, hipStream_t st);
