@@ -32,6 +32,8 @@
 #include <string.h>
 
 #include "chacha.h"
+#include <stdlib.h>
+
 #include "common.h"
 #include "profile.h"
 
@@ -352,9 +354,17 @@ static int aj_commit(AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, u64 *
     const int nplanes = 8 * batch + ones;
     const int CTT = (nplanes + 15) / 16;
     const int CT = CTT <= 1 ? 1 : CTT <= 2 ? 2 : CTT <= 4 ? 4 : CTT <= 8 ? 8 : 16;
-    // 2^13 columns per workgroup (512 workgroups at n = 2^22) while the partials stay small,
-    // 2^14 for the widest batches; both keep chunk sums below 2^28.
-    const size_t kchunk = CT <= 4 ? ((size_t)1 << 13) : ((size_t)1 << 14);
+    // 2^13 columns per workgroup (512 workgroups at n = 2^22) for up to 32 planes, 2^14 for
+    // wider batches (fewer partials to write and add: B = 8 0.349 -> 0.337 ms at 2^22; 2^12
+    // was slower at every batch); both keep chunk sums below 2^28.
+    size_t kchunk = CT <= 2 ? ((size_t)1 << 13) : ((size_t)1 << 14);
+    static int kc_log2 = -1;  // MAM_AJ_KCHUNK_LOG2 (12..14) overrides the columns per workgroup
+    if (kc_log2 < 0) {
+        const char *e = getenv("MAM_AJ_KCHUNK_LOG2");
+        const int v = e ? atoi(e) : 0;
+        kc_log2 = (v >= 12 && v <= 14) ? v : 0;
+    }
+    if (kc_log2) kchunk = (size_t)1 << kc_log2;
     const size_t chunks = (n_used + kchunk - 1) / kchunk;
     AjScratch &sc = t_aj_scratch;
     if (aj_reserve((void **)&sc.partial, &sc.bytes, chunks * A->rows * CTT * 16 * sizeof(int), s)) return -1;
