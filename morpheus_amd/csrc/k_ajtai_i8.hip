@@ -358,12 +358,12 @@ static int aj_commit(AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, u64 *
     // wider batches (fewer partials to write and add: B = 8 0.349 -> 0.337 ms at 2^22; 2^12
     // was slower at every batch); both keep chunk sums below 2^28.
     size_t kchunk = CT <= 2 ? ((size_t)1 << 13) : ((size_t)1 << 14);
-    static int kc_log2 = -1;  // MAM_AJ_KCHUNK_LOG2 (12..14) overrides the columns per workgroup
-    if (kc_log2 < 0) {
+    // MAM_AJ_KCHUNK_LOG2 (12..14) overrides the columns per workgroup
+    static const int kc_log2 = [] {
         const char *e = getenv("MAM_AJ_KCHUNK_LOG2");
         const int v = e ? atoi(e) : 0;
-        kc_log2 = (v >= 12 && v <= 14) ? v : 0;
-    }
+        return (v >= 12 && v <= 14) ? v : 0;
+    }();
     if (kc_log2) kchunk = (size_t)1 << kc_log2;
     const size_t chunks = (n_used + kchunk - 1) / kchunk;
     AjScratch &sc = t_aj_scratch;

@@ -201,12 +201,11 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
 }
 
 static size_t narrow_min() {
-    static size_t v = 0;
-    if (!v) {
+    static const size_t v = [] {
         const char *e = getenv("MAM_MERKLE_NARROW_MIN");
-        v = e ? (size_t)strtoull(e, nullptr, 10) : ((size_t)1 << 16);
-        if (v == 0) v = 1;
-    }
+        const size_t x = e ? (size_t)strtoull(e, nullptr, 10) : ((size_t)1 << 16);
+        return x ? x : (size_t)1;
+    }();
     return v;
 }
 

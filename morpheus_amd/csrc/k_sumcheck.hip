@@ -614,24 +614,22 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict_
 // Workgroups of a pass launch: MAM_SC_GRID (default MAM_RED_GRID), at most what the partials
 // scratch holds for W reduced words per workgroup.
 static unsigned pass_grid_cap(int w) {
-    static long v = -1;
-    if (v < 0) {
+    static const long v = [] {
         const char *e = getenv("MAM_SC_GRID");
-        v = e ? strtol(e, nullptr, 10) : (long)MAM_RED_GRID;
-        if (v < 1) v = 1;
-    }
+        const long x = e ? strtol(e, nullptr, 10) : (long)MAM_RED_GRID;
+        return x < 1 ? 1L : x;
+    }();
     const long room = (long)MAM_RED_MAX_BLOCKS * 2 * (MAM_MAX_POLYS + 1) / w;
     return (unsigned)(v < room ? v : room);
 }
 
 // Largest item count that runs on k_sc_pass_lp (MAM_SC_LP_MAX overrides; 0 disables it).
 static size_t lp_max_items() {
-    static long v = -1;
-    if (v < 0) {
+    static const long v = [] {
         const char *e = getenv("MAM_SC_LP_MAX");
-        v = e ? strtol(e, nullptr, 10) : (1L << 14);
-        if (v < 0) v = 0;
-    }
+        const long x = e ? strtol(e, nullptr, 10) : (1L << 14);
+        return x < 0 ? 0L : x;
+    }();
     return (size_t)v;
 }
 

@@ -48,12 +48,11 @@ DevCache g_dev[MAX_DEV];
 std::unordered_map<void *, LiveBlock> g_live;
 
 bool cache_enabled() {
-    static int on = -1;
-    if (on < 0) {
+    static const bool on = [] {
         const char *e = getenv("AGL_ALLOC_CACHE");
-        on = (e && e[0] == '0') ? 0 : 1;
-    }
-    return on == 1;
+        return !(e && e[0] == '0');
+    }();
+    return on;
 }
 
 size_t round_request(size_t n) {
@@ -199,11 +198,10 @@ MAM_API int cuda_memset(void *dst, int value, size_t size) {
 
 // ffi.rs:87: wait for the calling thread's queued work (AGL_DEVICE_SYNC=1: the whole device).
 MAM_API int cuda_sync_mine() {
-    static int dev_sync = -1;
-    if (dev_sync < 0) {
+    static const bool dev_sync = [] {
         const char *e = getenv("AGL_DEVICE_SYNC");
-        dev_sync = (e && e[0] == '1') ? 1 : 0;
-    }
+        return e && e[0] == '1';
+    }();
     const hipError_t r = dev_sync ? hipDeviceSynchronize() : hipStreamSynchronize(mam_stream());
     return r == hipSuccess ? 0 : -1;
 }
