@@ -21,6 +21,7 @@
 // oracle, then per round one fused fold+oracle launch for (eq, bh) and one fused
 // fold+Merkle pass for the codeword; the host reads back 10 words per round (oracle and
 // root) to run the transcript. Query values and paths are gathered by one kernel.
+#include <atomic>
 #include <stdlib.h>
 #include <string.h>
 
@@ -166,18 +167,20 @@ void free_async(void *p, hipStream_t s) {
     if (p) hipFreeAsync(p, s);
 }
 
-// Keep freed stream-ordered allocations cached in the pool across calls.
+// Keep freed stream-ordered allocations cached in the current device's pool across calls
+// (once per device; setting the threshold twice is harmless).
 void pool_setup() {
-    static bool done = false;
-    if (done) return;
+    static std::atomic<unsigned long long> done{0};
     int dev = 0;
-    hipGetDevice(&dev);
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    const unsigned long long bit = 1ull << dev;
+    if (done.load(std::memory_order_acquire) & bit) return;
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
         uint64_t thr = ~0ull;
-        hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
     }
-    done = true;
+    done.fetch_or(bit, std::memory_order_release);
 }
 
 // The opening proper, from the eq table onwards: the sumcheck of eq * bh with its oracles,
