@@ -21,11 +21,11 @@
 // oracle, then per round one fused fold+oracle launch for (eq, bh) and one fused
 // fold+Merkle pass for the codeword; the host reads back 10 words per round (oracle and
 // root) to run the transcript. Query values and paths are gathered by one kernel.
-#include <atomic>
 #include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <vector>
 
 #include "basefold.h"
