@@ -41,9 +41,12 @@ struct Ops<F, false> {
 
 // grid (2^(nv - la - lb), tables), 256 threads; table y reads r_all + y * r_stride and
 // writes out_all + y * out_stride (words).
+// With `scaled`, variable 0 contributes f0 (bit clear) / f1 (bit set) instead of 1 - r_0 / r_0:
+// a scaled eq table, s * eq(r, x) with f0 = s (1 - r_0), f1 = s r_0.
 template <class F, bool EXT>
 __global__ __launch_bounds__(256) void k_eq(const u64 *__restrict__ r_all, size_t r_stride, int nv,
-                                            u64 *__restrict__ out_all, size_t out_stride) {
+                                            u64 *__restrict__ out_all, size_t out_stride, int scaled,
+                                            typename Ops<F, EXT>::V f0, typename Ops<F, EXT>::V f1) {
     using O = Ops<F, EXT>;
     using V = typename O::V;
     constexpr int W = EXT ? 2 : 1;
@@ -58,7 +61,8 @@ __global__ __launch_bounds__(256) void k_eq(const u64 *__restrict__ r_all, size_
         V v = O::one();
         for (int i = 0; i < la; i++) {
             const V ri = O::load(r, i);
-            v = O::mul(v, ((t >> i) & 1) ? ri : O::one_minus(ri));
+            const bool bit = (t >> i) & 1;
+            v = O::mul(v, (i == 0 && scaled) ? (bit ? f1 : f0) : (bit ? ri : O::one_minus(ri)));
         }
         O::put(ta, t, v);
     } else if (t >= 64 && t < 64 + (1 << lb)) {
@@ -83,11 +87,15 @@ __global__ __launch_bounds__(256) void k_eq(const u64 *__restrict__ r_all, size_
 
 template <class F, bool EXT>
 static int launch(const u64 *d_r, size_t r_stride, int nv, u64 *d_out, size_t out_stride, int tables,
-                  hipStream_t s) {
+                  hipStream_t s, const typename Ops<F, EXT>::V *scale01 = nullptr) {
     if (nv < 0 || nv > 40 || tables <= 0 || tables > 65535) return -1;
+    if (scale01 && nv < 1) return -1;
     const int la = nv < 6 ? nv : 6, lb = nv - la < 6 ? nv - la : 6;
     const dim3 grid((unsigned)((size_t)1 << (nv - la - lb)), (unsigned)tables);
-    k_eq<F, EXT><<<grid, 256, 0, s>>>(d_r, r_stride, nv, d_out, out_stride);
+    using V = typename Ops<F, EXT>::V;
+    const V z{};
+    k_eq<F, EXT><<<grid, 256, 0, s>>>(d_r, r_stride, nv, d_out, out_stride, scale01 ? 1 : 0,
+                                      scale01 ? scale01[0] : z, scale01 ? scale01[1] : z);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

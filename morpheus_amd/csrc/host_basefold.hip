@@ -196,7 +196,25 @@ struct OpenSrc {
     bool cw_base;
     const u64 *tree0;  // Merkle tree of cw
     int nv, lr, hash;
+    // Product form (a single opening point, eq == nullptr): the point's Ext2 words on the
+    // device and the host; rounds below K fold bh alone and return the sums U0, U1 of
+    // mam_open_u, round K builds the scaled eq table the later (ride-along) rounds fold.
+    const u64 *pt = nullptr, *pt_host = nullptr;
+    int K = 0;
 };
+
+// The round's oracle (c0, c1, c2) from U0, U1 when eq = sc * eq(r_k, r_k+1, ...) (product form).
+static void oracle_from_u(const u64 *U, ext2 sc, ext2 rk, u64 *o6) {
+    const ext2 u0{gl_canon(U[0]), gl_canon(U[1])}, u1{gl_canon(U[2]), gl_canon(U[3])};
+    const ext2 a = e2_mul(sc, e2_sub(e2_one(), rk)), b = e2_mul(sc, rk);
+    const ext2 c0 = e2_mul(a, u0), c2 = e2_mul(e2_sub(b, a), e2_sub(u1, u0));
+    const ext2 c1 = e2_sub(e2_sub(e2_mul(b, u1), c0), c2);
+    const ext2 c[3] = {c0, c1, c2};
+    for (int k = 0; k < 3; k++) {
+        o6[2 * k] = c[k].c0;
+        o6[2 * k + 1] = c[k].c1;
+    }
+}
 
 int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript *t, int nq, u64 *out) {
     const int nv = S.nv, lr = S.lr;
@@ -243,8 +261,15 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     if (!h) rc = -1;
     RedScratch rs = {red, (u32 *)cnt, red + RED_BLOCKS * 6};
     if (!rc && hipMemsetAsync(cnt, 0, 64, s) != hipSuccess) rc = -1;
-    if (!rc) rc = mam_open_oracle0(S.eq, S.bh, S.bh_base, n / 2, rs, s);
-    if (!rc) rc = read_back(h, rs.result, 6, s);
+    auto point_var = [&](int k) { return ext2{gl_canon(S.pt_host[2 * k]), gl_canon(S.pt_host[2 * k + 1])}; };
+    if (S.eq) {
+        if (!rc) rc = mam_open_oracle0(S.eq, S.bh, S.bh_base, n / 2, rs, s);
+        if (!rc) rc = read_back(h, rs.result, 6, s);
+    } else {
+        if (!rc) rc = mam_open_u(S.bh, S.bh_base, false, e2_zero(), nullptr, n / 2, S.pt + 2, nv - 1, rs, s);
+        if (!rc) rc = read_back(h, rs.result, 4, s);
+        if (!rc) oracle_from_u(h, e2_one(), point_var(0), h);
+    }
     u64 *o_eval = out;
     u64 *o_oracles = out + 2;
     u64 *o_roots = o_oracles + 6L * nv;
@@ -265,12 +290,29 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     size_t flen = cw_len / 2;  // folded codeword length of this round
     const u64 *cw_in = S.cw;
     bool cw_base = S.cw_base;
+    ext2 scale = e2_one();  // product form: eq of this round = scale * eq(r_round, ...)
     for (int round = 0; round < nv - 1 && !rc; round++) {
         ext2 ch = sample_ext2(t);
         const int pp = round & 1;
+        const bool prod = !S.eq && round < S.K;
+        ext2 next_scale = scale;
+        if (!S.eq) {
+            const ext2 rr = point_var(round);
+            next_scale = e2_mul(scale, e2_add(e2_mul(e2_sub(e2_one(), rr), e2_sub(e2_one(), ch)), e2_mul(rr, ch)));
+            if (round == S.K) {
+                // the rounds from here ride along tree launches: materialize the scaled eq table
+                rc = mam_eq_table_scaled(S.pt + 2 * round, nv - round, e2_mul(scale, e2_sub(e2_one(), rr)),
+                                         e2_mul(scale, rr), eqp[pp ^ 1], s);
+                if (rc) break;
+                eq_cur = eqp[pp ^ 1];
+            }
+        }
         // the eq / bh fold with the next oracle, run along with the folded codeword's tree
         const OracleJob job = {eq_cur, bh_cur, ch, eqp[pp], bhp[pp], sc_size / 4, rs.partials,
                                rs.counter, rs.result, 0, bh_base ? 1 : 0};
+        if (prod) rc = mam_open_u(bh_cur, bh_base, true, ch, bhp[pp], sc_size / 4, S.pt + 2 * (round + 2), nv - round - 2,
+                                  rs, s);
+        if (rc) break;
         eq_cur = eqp[pp];
         bh_cur = bhp[pp];
         bh_base = false;
@@ -284,22 +326,25 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
             rc = -1;
             break;
         }
-        const MailJob mail = {rs.result, 6, box, seq};
+        const int nres = prod ? 4 : 6;  // U0, U1 or the oracle
+        const MailJob mail = {rs.result, nres, box, seq};
         bool posted = false;
-        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s, &job, &mail,
-                               &posted);
+        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s, prod ? nullptr : &job,
+                               &mail, &posted);
         if (rc) break;
         const size_t leaves = flen / 2;
-        if ((!posted && mam_mailbox_post(rs.result, 6, ftree[round] + mam_merkle_root_offset(leaves), 4, box, seq,
+        if ((!posted && mam_mailbox_post(rs.result, nres, ftree[round] + mam_merkle_root_offset(leaves), 4, box, seq,
                                          s)) ||
-            mam_mailbox_wait(box, seq, 10, h, s)) {
+            mam_mailbox_wait(box, seq, nres + 4, h, s)) {
             rc = -1;
             break;
         }
         u64 *o = o_oracles + 6L * (round + 1);
-        memcpy(o, h, 6 * sizeof(u64));
+        if (prod) oracle_from_u(h, next_scale, point_var(round + 1), o);
+        else memcpy(o, h, 6 * sizeof(u64));
         for (int k = 0; k < 3; k++) observe_ext2(t, o + 2 * k);
-        memcpy(o_roots + 4L * round, h + 6, 4 * sizeof(u64));
+        memcpy(o_roots + 4L * round, h + nres, 4 * sizeof(u64));
+        scale = next_scale;
         observe_hash(t, o_roots + 4L * round);
         cw_in = fcw[round];
         cw_base = false;
@@ -360,10 +405,27 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     if (!arena) return -1;
     u64 *d_point = arena, *eq = arena + words(2 * nv), *bh = eq + words(2 * n);
     if (hipMemcpyAsync(d_point, point2, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
-    int rc = mam_eq_table(d_point, nv, eq, true, s);
+    // rounds whose eq / bh fold runs on a launch of its own take the product form (no eq
+    // table); from the first round that rides along a lane-parallel tree launch on, the
+    // scaled eq table is built and folded as usual
+    int K = nv - 1;
+    if (C.hash == MAM_HASH_MONOLITH) {
+        for (int r = 0; r < nv - 1; r++) {
+            if (((size_t)1 << (nv + C.lr - r - 2)) < mam_merkle_narrow_min()) {
+                K = r;
+                break;
+            }
+        }
+    }
+    int rc = K == 0 ? mam_eq_table(d_point, nv, eq, true, s) : 0;
     if (!rc) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
     if (rc) return rc;
-    OpenSrc S{eq, bh, true, C.codeword, true, C.tree, nv, C.lr, C.hash};
+    OpenSrc S{K == 0 ? eq : nullptr, bh, true, C.codeword, true, C.tree, nv, C.lr, C.hash};
+    if (K > 0) {
+        S.pt = d_point;
+        S.pt_host = point2;
+        S.K = K;
+    }
     return open_core(S, d_table, table_nv, t, nq, proof);
 }
 

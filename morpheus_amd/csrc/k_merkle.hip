@@ -209,6 +209,8 @@ static size_t narrow_min() {
     return v;
 }
 
+size_t mam_merkle_narrow_min() { return narrow_min(); }
+
 static int ilog2_sz(size_t n) {
     int l = 0;
     while ((size_t(1) << l) < n) l++;
