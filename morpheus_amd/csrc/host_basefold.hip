@@ -399,12 +399,6 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     // transcript preamble
     observe_hash(t, C.root);
     for (int i = 0; i < nv; i++) observe_ext2(t, point2 + 2 * i);
-    // point, eq table and natural-order bh in the thread's arena slot 3
-    auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
-    u64 *arena = mam_dev_arena(words(2 * nv) + words(2 * n) + words(n), 3);
-    if (!arena) return -1;
-    u64 *d_point = arena, *eq = arena + words(2 * nv), *bh = eq + words(2 * n);
-    if (hipMemcpyAsync(d_point, point2, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
     // rounds whose eq / bh fold runs on a launch of its own take the product form (no eq
     // table); from the first round that rides along a lane-parallel tree launch on, the
     // scaled eq table is built and folded as usual
@@ -417,6 +411,13 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
             }
         }
     }
+    // point, natural-order bh and (K == 0 only) the eq table in the thread's arena slot 3
+    auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
+    const size_t eq_words = K == 0 ? words(2 * n) : 0;
+    u64 *arena = mam_dev_arena(words(2 * nv) + words(n) + eq_words, 3);
+    if (!arena) return -1;
+    u64 *d_point = arena, *bh = arena + words(2 * nv), *eq = K == 0 ? bh + words(n) : nullptr;
+    if (hipMemcpyAsync(d_point, point2, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
     int rc = K == 0 ? mam_eq_table(d_point, nv, eq, true, s) : 0;
     if (!rc) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
     if (rc) return rc;
