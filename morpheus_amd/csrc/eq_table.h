@@ -3,9 +3,10 @@
 // else base point and table). Semantics of cuda/eq_lagrange.cuh:57-158 and
 // zk-torch-3/src/poly/mod.rs:136-163; one pass instead of one launch per variable.
 //
-// A workgroup owns 2^(la + lb) consecutive indices (la, lb <= 6). Its first 64 threads build
-// A[j] = eq over the lowest la variables, the next 64 build B[j] = eq over the next lb
-// variables times the workgroup's factor of all higher variables (the bits of blockIdx.x).
+// A workgroup owns 2^(la + lb) consecutive indices (la, lb <= 6; see split). Its first 64
+// threads build A[j] = eq over the lowest la variables, the next 64 build B[j] = eq over the
+// next lb variables times the workgroup's factor of all higher variables (the bits of
+// blockIdx.x).
 // Every output is then one product A[x mod 2^la] * B[x >> la]: about one multiplication and
 // one 16-byte store per entry, so the launch runs at the store stream's rate.
 #pragma once
@@ -13,6 +14,14 @@
 #include "fpolicy.h"
 
 namespace eqt {
+
+// Variables of the A table (la) and of the B table (lb); the lh = nv - la - lb above them
+// number the workgroups. lb shrinks to keep at least 2^8 workgroups where nv allows it.
+__host__ __device__ inline void split(int nv, int &la, int &lb) {
+    la = nv < 6 ? nv : 6;
+    const int rest = nv - la;
+    lb = rest - 8 < 0 ? 0 : (rest - 8 > 6 ? 6 : rest - 8);
+}
 
 template <class F, bool EXT>
 struct Ops;
@@ -53,8 +62,8 @@ __global__ __launch_bounds__(256) void k_eq(const u64 *__restrict__ r_all, size_
     __shared__ u64 ta[64 * W], tb[64 * W];
     const u64 *r = r_all + blockIdx.y * r_stride;
     u64 *out = out_all + blockIdx.y * out_stride;
-    const int la = nv < 6 ? nv : 6;
-    const int lb = nv - la < 6 ? nv - la : 6;
+    int la, lb;
+    split(nv, la, lb);
     const int lh = nv - la - lb;
     const int t = threadIdx.x;
     if (t < (1 << la)) {
@@ -90,7 +99,8 @@ static int launch(const u64 *d_r, size_t r_stride, int nv, u64 *d_out, size_t ou
                   hipStream_t s, const typename Ops<F, EXT>::V *scale01 = nullptr) {
     if (nv < 0 || nv > 40 || tables <= 0 || tables > 65535) return -1;
     if (scale01 && nv < 1) return -1;
-    const int la = nv < 6 ? nv : 6, lb = nv - la < 6 ? nv - la : 6;
+    int la, lb;
+    split(nv, la, lb);
     const dim3 grid((unsigned)((size_t)1 << (nv - la - lb)), (unsigned)tables);
     using V = typename Ops<F, EXT>::V;
     const V z{};
