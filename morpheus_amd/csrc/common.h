@@ -50,6 +50,10 @@ static inline unsigned grid_reduce(size_t n, int block = MAM_BLOCK) {
     return grid_stride(n, block, MAM_RED_GRID);
 }
 
+// Keeps the device's default stream-ordered pool from returning freed hipMallocAsync blocks
+// to the driver at each synchronization (mem_pool.hip; once per device).
+void mam_stream_pool_setup();
+
 static inline int mam_launch_status() { return hipGetLastError() == hipSuccess ? 0 : -1; }
 
 static inline int mam_sync_status() {

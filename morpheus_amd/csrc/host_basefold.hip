@@ -167,22 +167,6 @@ void free_async(void *p, hipStream_t s) {
     if (p) hipFreeAsync(p, s);
 }
 
-// Keep freed stream-ordered allocations cached in the current device's pool across calls
-// (once per device; setting the threshold twice is harmless).
-void pool_setup() {
-    static std::atomic<unsigned long long> done{0};
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
-    const unsigned long long bit = 1ull << dev;
-    if (done.load(std::memory_order_acquire) & bit) return;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-        uint64_t thr = ~0ull;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-    }
-    done.fetch_or(bit, std::memory_order_release);
-}
-
 // The opening proper, from the eq table onwards: the sumcheck of eq * bh with its oracles,
 // the folds of the codeword with their trees, the final codeword and the queries. Writes
 // eval | oracles | folded roots | final cw | queries to `out`. open_ext2 runs it on one
@@ -394,7 +378,7 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     const int nv = C.nv;
     if (nv < 2 || table_nv < nv) return -2;
     hipStream_t s = mam_stream();
-    pool_setup();
+    mam_stream_pool_setup();
     const size_t n = (size_t)1 << nv;
     // transcript preamble
     observe_hash(t, C.root);
@@ -662,7 +646,7 @@ int batch_open_ext2(const std::vector<const Commitment *> &C, const u64 *points,
     for (int e = 0; e < ne; e++)
         if (ev[e].poly < 0 || ev[e].poly >= (int)C.size() || ev[e].point < 0 || ev[e].point >= np) return -2;
     hipStream_t s = mam_stream();
-    pool_setup();
+    mam_stream_pool_setup();
     const size_t n = (size_t)1 << nv, cw_len = (size_t)1 << (nv + lr);
     // transcript preamble: roots, points, claimed values; batching weights
     for (const Commitment *c : C) observe_hash(t, c->root);

@@ -64,6 +64,7 @@ int state_alloc(ScState &S, bool arena) {
         S.finals = S.s.msg + wm;
         return hipMemsetAsync(S.s.counter, 0, 64, st) == hipSuccess ? 0 : -1;
     }
+    mam_stream_pool_setup();
     if (hipMallocAsync((void **)&S.buf[0], S.np * S.stride[0] * sizeof(u64), st) != hipSuccess) return -1;
     if (hipMallocAsync((void **)&S.buf[1], S.np * S.stride[1] * sizeof(u64), st) != hipSuccess) return -1;
     if (hipMallocAsync((void **)&S.s.partials, MAM_RED_MAX_BLOCKS * 2 * (MAM_MAX_POLYS + 1) * sizeof(u64), st) !=

@@ -2,50 +2,51 @@
 // cuda_free (goldilocks-cuda-rs/src/ffi.rs:30-53; almost-goldilocks-cuda-rs/src/ffi.rs:61-100,
 // 342-350, 687). Host code only.
 //
-// Allocator (the behaviour of almost-goldilocks-cuda-rs/cuda/wrapper.cu:118-248, whose callers
-// create and drop DeviceBuffers at a high rate): cuda_free puts a block on a per-device free
-// list instead of returning it to the driver, and cuda_malloc hands out the smallest cached
-// block that is at least the rounded request and at most 1.25 x + 2 MiB of it. Requests round
-// up to 512-byte granules below 1 MiB and to 2 MiB pages above. cuda_free records an event on
-// the freeing thread's per-thread stream and cuda_malloc waits for it before a reuse, so a
-// block is never handed out while work queued before its free can still touch it. When the
-// driver refuses an allocation, the device's cache is released and the request retried.
-// Cached blocks count as used in cuda_mem_get_info; cuda_pool_trim releases them.
-// AGL_ALLOC_CACHE=0 (the reference's switch) selects plain hipMalloc / hipFree.
+// Contract kept from the reference crate (its callers create and drop device buffers at a
+// high rate): cuda_free does not return memory to the driver; a freed block is never handed
+// out while work queued before its free can still touch it; when the driver refuses an
+// allocation the cache is released and the request retried; AGL_ALLOC_CACHE=0 selects plain
+// hipMalloc / hipFree; agl_cached_bytes reports the cache and cuda_pool_trim releases it.
+//
+// Design. Requests map to quarter-octave size classes (512 B, then 5/8, 6/8, 7/8 and 8/8
+// of each power of two: at most 25% slack, no search). Every device keeps one FIFO of freed
+// blocks per class. A block owns one event for its whole life; cuda_free records it on the
+// freeing thread's per-thread stream and queues the block at the back of its class.
+// cuda_malloc looks in its own class and the next one up and takes the first block whose
+// event has already completed (hipEventQuery, no wait); only when every cached block of the
+// class is still in flight does it wait, on the oldest one. A miss allocates a fresh block of
+// the class size.
 #include <stdlib.h>
 
-#include <map>
+#include <atomic>
+#include <deque>
 #include <mutex>
 #include <unordered_map>
-#include <vector>
 
 #include "common.h"
 
 namespace {
 
 constexpr int MAX_DEV = 16;
+constexpr int NCLASS = 4 * 56 + 1;  // class 0 = 512 B; then four per octave up to 2^64
+constexpr int SCAN = 8;             // cached blocks probed per class before waiting
 
-struct CachedBlock {
-    void *p;
-    size_t bytes;
-    hipEvent_t freed;  // recorded on the freeing thread's stream
+struct Block {
+    void *p = nullptr;
+    size_t bytes = 0;  // class size
+    int dev = 0;
+    int cls = 0;
+    hipEvent_t done = nullptr;  // recorded at each free; created at the first one
 };
 
-struct DevCache {
-    std::multimap<size_t, CachedBlock> free_list;  // by size
+struct Device {
+    std::deque<Block *> bins[NCLASS];
     size_t cached = 0;
-    std::vector<hipEvent_t> spare_events;
-    bool pool_threshold_set = false;
-};
-
-struct LiveBlock {
-    int dev;
-    size_t bytes;
 };
 
 std::mutex g_mu;
-DevCache g_dev[MAX_DEV];
-std::unordered_map<void *, LiveBlock> g_live;
+Device g_devs[MAX_DEV];
+std::unordered_map<void *, Block *> g_out;  // blocks handed out, by address
 
 bool cache_enabled() {
     static const bool on = [] {
@@ -55,23 +56,20 @@ bool cache_enabled() {
     return on;
 }
 
-size_t round_request(size_t n) {
-    if (n == 0) n = 1;
-    const size_t small = (size_t)1 << 20, page = (size_t)2 << 20;
-    if (n <= small) return (n + 511) & ~(size_t)511;
-    return (n + page - 1) / page * page;
+// Size class of an n-byte request. For n > 512 with 2^(e-1) < n <= 2^e the class sizes are
+// k * 2^(e-3), k = 5..8.
+int class_of(size_t n) {
+    if (n <= 512) return 0;
+    const int e = 64 - __builtin_clzll((unsigned long long)(n - 1));
+    const size_t step = (size_t)1 << (e - 3);
+    const int k = (int)((n + step - 1) / step);  // 5..8
+    return 1 + (e - 10) * 4 + (k - 5);
 }
 
-// Frees every cached block of `dev`; the caller holds g_mu and has `dev` current.
-void flush_locked(int dev) {
-    DevCache &c = g_dev[dev];
-    for (auto &kv : c.free_list) {
-        (void)hipEventSynchronize(kv.second.freed);
-        (void)hipFree(kv.second.p);
-        c.spare_events.push_back(kv.second.freed);
-    }
-    c.free_list.clear();
-    c.cached = 0;
+size_t class_bytes(int cls) {
+    if (cls == 0) return 512;
+    const int e = 10 + (cls - 1) / 4, k = 5 + (cls - 1) % 4;
+    return (size_t)k << (e - 3);
 }
 
 int current_device() {
@@ -80,110 +78,169 @@ int current_device() {
     return dev;
 }
 
+void destroy(Block *b) {
+    if (b->done) {
+        (void)hipEventSynchronize(b->done);
+        (void)hipEventDestroy(b->done);
+    }
+    (void)hipFree(b->p);
+    delete b;
+}
+
+// Takes every cached block of `dev` out of its bins (the caller holds g_mu) and returns them,
+// so the caller can wait on and free them after unlocking.
+std::deque<Block *> drain_locked(int dev) {
+    std::deque<Block *> out;
+    Device &d = g_devs[dev];
+    for (auto &bin : d.bins) {
+        for (Block *b : bin) out.push_back(b);
+        bin.clear();
+    }
+    d.cached = 0;
+    return out;
+}
+
+void release_cache(int dev) {
+    std::deque<Block *> blocks;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        blocks = drain_locked(dev);
+    }
+    for (Block *b : blocks) destroy(b);
+}
+
+// A cached block of class `cls` on `dev` whose free has completed, or, when `may_wait`, the
+// oldest one (the caller then waits on its event); nullptr when the class has none. The
+// caller holds g_mu.
+Block *take_locked(int dev, int cls, bool may_wait, bool *must_wait) {
+    auto &bin = g_devs[dev].bins[cls];
+    *must_wait = false;
+    int probed = 0;
+    for (auto it = bin.begin(); it != bin.end() && probed < SCAN; ++it, ++probed) {
+        if (hipEventQuery((*it)->done) == hipSuccess) {
+            Block *b = *it;
+            bin.erase(it);
+            g_devs[dev].cached -= b->bytes;
+            return b;
+        }
+    }
+    (void)hipGetLastError();  // hipErrorNotReady from the probes
+    if (!may_wait || bin.empty()) return nullptr;
+    Block *b = bin.front();
+    bin.pop_front();
+    g_devs[dev].cached -= b->bytes;
+    *must_wait = true;
+    return b;
+}
+
 }  // namespace
+
+// The stream-ordered allocations some entry points make for their own scratch
+// (hipMallocAsync on the calling thread's stream) stay cached in the device's default pool
+// between calls instead of going back to the driver at each synchronization. Once per device.
+void mam_stream_pool_setup() {
+    static std::atomic<unsigned long long> done{0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+    const unsigned long long bit = 1ull << dev;
+    if (done.load(std::memory_order_acquire) & bit) return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t thr = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+    }
+    done.fetch_or(bit, std::memory_order_release);
+}
 
 MAM_API int cuda_malloc(void **ptr, size_t size) {
     if (!ptr) return -1;
     const int dev = current_device();
     if (!cache_enabled() || dev < 0 || dev >= MAX_DEV)
         return hipMalloc(ptr, size ? size : 1) == hipSuccess ? 0 : -1;
-    const size_t want = round_request(size);
+    const int cls = class_of(size);
+    Block *b = nullptr;
+    bool wait = false;
     {
-        std::unique_lock<std::mutex> lk(g_mu);
-        DevCache &c = g_dev[dev];
-        if (!c.pool_threshold_set) {
-            // stream-ordered allocations inside the library keep their pool between calls
-            c.pool_threshold_set = true;
-            hipMemPool_t pool;
-            if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-                uint64_t thr = ~0ull;
-                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-            }
-        }
-        auto it = c.free_list.lower_bound(want);
-        if (it != c.free_list.end() && it->first <= want + want / 4 + ((size_t)2 << 20)) {
-            CachedBlock b = it->second;
-            c.free_list.erase(it);
-            c.cached -= b.bytes;
-            g_live[b.p] = LiveBlock{dev, b.bytes};
-            lk.unlock();
-            const bool done = hipEventSynchronize(b.freed) == hipSuccess;
-            lk.lock();
-            c.spare_events.push_back(b.freed);
-            if (!done) {
-                g_live.erase(b.p);
-                (void)hipFree(b.p);
-                return -1;
-            }
-            *ptr = b.p;
-            return 0;
-        }
-    }
-    void *p = nullptr;
-    if (hipMalloc(&p, want) != hipSuccess) {
-        (void)hipGetLastError();
         std::lock_guard<std::mutex> lk(g_mu);
-        flush_locked(dev);
-        if (hipMalloc(&p, want) != hipSuccess) {
+        b = take_locked(dev, cls, false, &wait);
+        if (!b && cls + 1 < NCLASS) b = take_locked(dev, cls + 1, false, &wait);
+        if (!b) b = take_locked(dev, cls, true, &wait);
+        if (b) g_out[b->p] = b;
+    }
+    if (b) {
+        if (wait && hipEventSynchronize(b->done) != hipSuccess) {
+            std::lock_guard<std::mutex> lk(g_mu);
+            g_out.erase(b->p);
+            destroy(b);
+            return -1;
+        }
+        *ptr = b->p;
+        return 0;
+    }
+    b = new Block;
+    b->bytes = class_bytes(cls);
+    b->dev = dev;
+    b->cls = cls;
+    if (hipMalloc(&b->p, b->bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        release_cache(dev);
+        if (hipMalloc(&b->p, b->bytes) != hipSuccess) {
             (void)hipGetLastError();
+            delete b;
             return -1;
         }
     }
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_live[p] = LiveBlock{dev, want};
-    *ptr = p;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_out[b->p] = b;
+    }
+    *ptr = b->p;
     return 0;
 }
 
 MAM_API int cuda_free(void *ptr) {
     if (!ptr) return 0;
     if (!cache_enabled()) return hipFree(ptr) == hipSuccess ? 0 : -1;
-    LiveBlock info;
-    hipEvent_t ev = nullptr;
+    Block *b = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_mu);
-        auto it = g_live.find(ptr);
-        if (it == g_live.end()) return hipFree(ptr) == hipSuccess ? 0 : -1;  // not from cuda_malloc
-        info = it->second;
-        g_live.erase(it);
-        auto &spare = g_dev[info.dev].spare_events;
-        if (!spare.empty()) {
-            ev = spare.back();
-            spare.pop_back();
+        auto it = g_out.find(ptr);
+        if (it != g_out.end()) {
+            b = it->second;
+            g_out.erase(it);
         }
     }
+    if (!b) return hipFree(ptr) == hipSuccess ? 0 : -1;  // not from cuda_malloc
+    // The event and the per-thread stream belong to the block's device.
     const int cur = current_device();
-    if (cur != info.dev) (void)hipSetDevice(info.dev);
-    bool ok = ev || hipEventCreateWithFlags(&ev, hipEventDisableTiming) == hipSuccess;
-    ok = ok && hipEventRecord(ev, mam_stream()) == hipSuccess;
-    if (cur >= 0 && cur != info.dev) (void)hipSetDevice(cur);
+    if (cur != b->dev) (void)hipSetDevice(b->dev);
+    bool ok = b->done || hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventRecord(b->done, mam_stream()) == hipSuccess;
+    if (cur >= 0 && cur != b->dev) (void)hipSetDevice(cur);
     if (!ok) {
-        if (ev) (void)hipEventDestroy(ev);
-        return hipFree(ptr) == hipSuccess ? 0 : -1;
+        destroy(b);
+        return -1;
     }
     std::lock_guard<std::mutex> lk(g_mu);
-    DevCache &c = g_dev[info.dev];
-    c.free_list.emplace(info.bytes, CachedBlock{ptr, info.bytes, ev});
-    c.cached += info.bytes;
+    Device &d = g_devs[b->dev];
+    d.bins[b->cls].push_back(b);
+    d.cached += b->bytes;
     return 0;
 }
 
-// almost-goldilocks-cuda-rs/src/ffi.rs:61: bytes held in the current device's free list.
+// almost-goldilocks-cuda-rs/src/ffi.rs:61: bytes held in the current device's free lists.
 MAM_API size_t agl_cached_bytes() {
     const int dev = current_device();
     if (dev < 0 || dev >= MAX_DEV) return 0;
     std::lock_guard<std::mutex> lk(g_mu);
-    return g_dev[dev].cached;
+    return g_devs[dev].cached;
 }
 
 // ffi.rs:85: release the allocator's cache and trim the default stream-ordered pool.
 MAM_API int cuda_pool_trim(size_t min_bytes_to_keep) {
     const int dev = current_device();
     if (dev < 0) return -1;
-    if (dev < MAX_DEV) {
-        std::lock_guard<std::mutex> lk(g_mu);
-        flush_locked(dev);
-    }
+    if (dev < MAX_DEV) release_cache(dev);
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return -1;
     return hipMemPoolTrimTo(pool, min_bytes_to_keep) == hipSuccess ? 0 : -1;
