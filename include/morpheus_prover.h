@@ -201,6 +201,18 @@ int mam_ajtai_i8_commit(void *handle, const uint64_t *d_z, int batch, uint64_t *
 /* The same against the first n_used columns (witnesses of n_used words; n_used a multiple of
  * 64, <= n; batch <= 16 when n_used < n): equal to a commitment with an n_used-column set-up. */
 int mam_ajtai_i8_commit_prefix(void *handle, const uint64_t *d_z, int batch, uint64_t n_used, uint64_t *d_out);
+/* Allocation-free twin of mam_ajtai_i8_commit_prefix. No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). The chunk partials live in a file-scope device array
+ * of 16 MiB per GPU (held until the process ends; its address is resolved at set-up), so the
+ * call makes no hipMalloc / hipFree and no allocating library call. Returns once the work is
+ * queued on the calling thread's stream: d_out is complete when that stream is synchronized.
+ * Returns -2 for the arguments mam_ajtai_i8_commit_prefix rejects and when the partials would
+ * exceed 16 MiB: (n_used / 2^13) x rows x 16 ceil((8 batch + [n_used < n]) / 16) x 4 bytes for
+ * up to 32 planes (2^14 columns per chunk above); at rows = 256 and n_used = 2^22 that admits
+ * batch <= 8 (a prefix commit, with its all-ones plane, batch <= 7). Every call on one GPU shares that GPU's array: calls that overlap on one GPU race
+ * (different threads or streams); calls queued on one stream do not. */
+int mam_ajtai_i8_commit_noalloc(void *handle, const uint64_t *d_z, int batch, uint64_t n_used, uint64_t *d_out);
 int mam_ajtai_i8_matrix_row(void *handle, int row, uint64_t col0, uint64_t count, int8_t *h_out);
 int mam_ajtai_i8_free(void *handle);
 

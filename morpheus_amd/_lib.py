@@ -162,6 +162,7 @@ _SIGS = {
     "mam_ajtai_i8_setup": (ci, [ctypes.c_char_p, ci, u64, ctypes.POINTER(vp)]),
     "mam_ajtai_i8_commit": (ci, [vp, vp, ci, vp]),
     "mam_ajtai_i8_commit_prefix": (ci, [vp, vp, ci, u64, vp]),
+    "mam_ajtai_i8_commit_noalloc": (ci, [vp, vp, ci, u64, vp]),
     "mam_ajtai_i8_matrix_row": (ci, [vp, ci, u64, u64, ctypes.POINTER(ctypes.c_int8)]),
     "mam_ajtai_i8_free": (ci, [vp]),
     # almost-Goldilocks crate (morpheus_agl.h)

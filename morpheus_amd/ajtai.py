@@ -47,6 +47,15 @@ class AjtaiI8:
         check(lib.mam_ajtai_i8_commit_prefix(self.h, dz.ptr, batch, n_used, do.ptr), "mam_ajtai_i8_commit_prefix")
         return do.to_numpy().reshape(batch, self.rows)
 
+    def commit_noalloc_device(self, d_z, batch, n_used, d_out):
+        """Queue the allocation-free commit (mam_ajtai_i8_commit_noalloc) of batch witnesses of
+        n_used words on the calling thread's stream; d_out is complete after a synchronize."""
+        rc = lib.mam_ajtai_i8_commit_noalloc(self.h, d_z.ptr, batch, n_used, d_out.ptr)
+        if rc == -2:
+            raise ValueError("mam_ajtai_i8_commit_noalloc: invalid arguments, or the chunk partials "
+                             f"of batch {batch} x {n_used} columns exceed its 16 MiB scratch")
+        check(rc, "mam_ajtai_i8_commit_noalloc")
+
     def matrix_row(self, row, col0, count):
         out = np.zeros(count, dtype=np.int8)
         check(lib.mam_ajtai_i8_matrix_row(self.h, row, col0, count, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))),

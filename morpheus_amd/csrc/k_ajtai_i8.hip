@@ -29,11 +29,12 @@
 //   RTW x CTW block of 16x16 accumulator tiles. |chunk sum| <= 2^14 * chunk <= 2^28, exact in
 //   int32; chunk partials are summed in int64 and recombined mod p by the finish kernel.
 //   At B = 1 the kernel streams the 1 GiB matrix once (HBM-bound).
+#include <stdlib.h>
 #include <string.h>
 
-#include "chacha.h"
-#include <stdlib.h>
+#include <mutex>
 
+#include "chacha.h"
 #include "common.h"
 #include "profile.h"
 
@@ -301,12 +302,45 @@ void aj_launch(const AjtaiI8 *A, const u64 *d_z, int batch, int ones, size_t n_u
 }
 }  // namespace
 
+// Chunk partials of the allocation-free commit (mam_ajtai_i8_commit_noalloc): one array per
+// GPU, held until the process ends. Host code never passes the array's name to a kernel (on
+// the host it names a stand-in, not device memory): the device address comes from
+// hipGetSymbolAddress, once per device, with that device current, and is kept in
+// g_scratch_dev[device].
+static constexpr size_t AJ_SCRATCH_BYTES = (size_t)16 << 20;
+__device__ __align__(256) int scratch[AJ_SCRATCH_BYTES / sizeof(int)];
+
+namespace {
+constexpr int AJ_SCRATCH_DEVS = 64;
+std::mutex g_scratch_mu;
+int *g_scratch_dev[AJ_SCRATCH_DEVS] = {};
+
+// The current device's address of `scratch`, resolved on the first call on that device;
+// nullptr when the runtime cannot resolve it.
+int *scratch_on_current_device() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= AJ_SCRATCH_DEVS) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        g_scratch_dev[dev] = (int *)ptr;
+    }
+    return g_scratch_dev[dev];
+}
+}  // namespace
+
 // Expand the matrix for (key, rows, n): rows a multiple of 16 (<= 256), n a multiple of 64
 // (<= 2^30, which keeps every recombined limb sum below p).
 MAM_API int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void **handle) {
     if (!handle || !key32 || rows <= 0 || rows % 16 || rows > 256 || n == 0 || n % 64 || n > ((uint64_t)1 << 30))
         return -2;
     *handle = nullptr;
+    // set-up also resolves this device's address of the allocation-free commit's scratch
+    if (!scratch_on_current_device()) return -1;
     AjtaiI8 *A = new AjtaiI8();
     A->rows = rows;
     A->n = n;
@@ -340,47 +374,72 @@ MAM_API int mam_ajtai_i8_setup(const uint8_t *key32, int rows, uint64_t n, void 
     return 0;
 }
 
+// Launch shape of one commit: plane tiles, columns per workgroup, chunk count and the bytes
+// of chunk partials the GEMM writes (and k_aj_finish reads back).
+struct AjPlan {
+    int ones = 0, CTT = 0, CT = 0;
+    size_t kchunk = 0, chunks = 0, partial_bytes = 0;
+};
+
 // Commit `batch` witnesses of n_used words each (back to back, device) against the first
 // n_used matrix columns; d_out: batch * rows words (device). n_used = n uses the set-up's row
 // sums; a shorter prefix (a multiple of 64, batch <= 16) adds an all-ones plane to the GEMM,
 // whose column is the prefix row sum (the matrix entries depend only on (key, row, column),
 // so a prefix commitment equals a commitment with a set-up of n_used columns).
-static int aj_commit(AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, u64 *d_out) {
+static int aj_plan(const AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, const u64 *d_out, AjPlan &P) {
     if (!A || !d_z || !d_out || batch < 1 || batch > AJ_MAX_BATCH) return -2;
     if (n_used == 0 || n_used % 64 || n_used > A->n) return -2;
-    const int ones = n_used != A->n ? 1 : 0;
-    if (ones && batch > 16) return -2;
-    hipStream_t s = mam_stream();
-    const int nplanes = 8 * batch + ones;
-    const int CTT = (nplanes + 15) / 16;
-    const int CT = CTT <= 1 ? 1 : CTT <= 2 ? 2 : CTT <= 4 ? 4 : CTT <= 8 ? 8 : 16;
+    P.ones = n_used != A->n ? 1 : 0;
+    if (P.ones && batch > 16) return -2;
+    const int nplanes = 8 * batch + P.ones;
+    P.CTT = (nplanes + 15) / 16;
+    P.CT = P.CTT <= 1 ? 1 : P.CTT <= 2 ? 2 : P.CTT <= 4 ? 4 : P.CTT <= 8 ? 8 : 16;
     // 2^13 columns per workgroup (512 workgroups at n = 2^22) for up to 32 planes, 2^14 for
     // wider batches (fewer partials to write and add: B = 8 0.349 -> 0.337 ms at 2^22; 2^12
     // was slower at every batch); both keep chunk sums below 2^28.
-    size_t kchunk = CT <= 2 ? ((size_t)1 << 13) : ((size_t)1 << 14);
+    P.kchunk = P.CT <= 2 ? ((size_t)1 << 13) : ((size_t)1 << 14);
     // MAM_AJ_KCHUNK_LOG2 (12..14) overrides the columns per workgroup
     static const int kc_log2 = [] {
         const char *e = getenv("MAM_AJ_KCHUNK_LOG2");
         const int v = e ? atoi(e) : 0;
         return (v >= 12 && v <= 14) ? v : 0;
     }();
-    if (kc_log2) kchunk = (size_t)1 << kc_log2;
-    const size_t chunks = (n_used + kchunk - 1) / kchunk;
-    AjScratch &sc = t_aj_scratch;
-    if (aj_reserve((void **)&sc.partial, &sc.bytes, chunks * A->rows * CTT * 16 * sizeof(int), s)) return -1;
-    {
-        ProfScope prof(PROF_AJTAI_GEMM, (double)A->rows * n_used + 8.0 * batch * n_used, s);
-        switch (CT) {
-            case 1: aj_launch<2, 1, 1, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
-            case 2: aj_launch<2, 2, 2, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
-            case 4: aj_launch<4, 2, 4, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
-            case 8: aj_launch<4, 4, 8, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
-            default: aj_launch<4, 8, 16, 2>(A, d_z, batch, ones, n_used, kchunk, (unsigned)chunks, sc.partial, s); break;
-        }
+    if (kc_log2) P.kchunk = (size_t)1 << kc_log2;
+    P.chunks = (n_used + P.kchunk - 1) / P.kchunk;
+    P.partial_bytes = P.chunks * A->rows * P.CTT * 16 * sizeof(int);
+    return 0;
+}
+
+// Queues the GEMM and the finish on stream s, with the chunk partials at `partial`
+// (P.partial_bytes, every word written by the GEMM before the finish reads it).
+static int aj_enqueue(const AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, const AjPlan &P, int *partial,
+                      u64 *d_out, hipStream_t s) {
+    const unsigned grid = (unsigned)P.chunks;
+    switch (P.CT) {
+        case 1: aj_launch<2, 1, 1, 2>(A, d_z, batch, P.ones, n_used, P.kchunk, grid, partial, s); break;
+        case 2: aj_launch<2, 2, 2, 2>(A, d_z, batch, P.ones, n_used, P.kchunk, grid, partial, s); break;
+        case 4: aj_launch<4, 2, 4, 2>(A, d_z, batch, P.ones, n_used, P.kchunk, grid, partial, s); break;
+        case 8: aj_launch<4, 4, 8, 2>(A, d_z, batch, P.ones, n_used, P.kchunk, grid, partial, s); break;
+        default: aj_launch<4, 8, 16, 2>(A, d_z, batch, P.ones, n_used, P.kchunk, grid, partial, s); break;
     }
     if (hipGetLastError() != hipSuccess) return -1;
-    k_aj_finish<<<batch * A->rows, 256, 0, s>>>(sc.partial, A->rowsum, A->rows, CTT * 16, (int)chunks,
-                                               ones ? 8 * batch : -1, d_out);
+    k_aj_finish<<<batch * A->rows, 256, 0, s>>>(partial, A->rowsum, A->rows, P.CTT * 16, (int)P.chunks,
+                                               P.ones ? 8 * batch : -1, d_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+static int aj_commit(AjtaiI8 *A, const u64 *d_z, int batch, size_t n_used, u64 *d_out) {
+    AjPlan P;
+    int rc = aj_plan(A, d_z, batch, n_used, d_out, P);
+    if (rc) return rc;
+    hipStream_t s = mam_stream();
+    AjScratch &sc = t_aj_scratch;
+    if (aj_reserve((void **)&sc.partial, &sc.bytes, P.partial_bytes, s)) return -1;
+    {
+        ProfScope prof(PROF_AJTAI_GEMM, (double)A->rows * n_used + 8.0 * batch * n_used, s);
+        rc = aj_enqueue(A, d_z, batch, n_used, P, sc.partial, d_out, s);
+    }
+    if (rc) return rc;
     return mam_sync_status();
 }
 
@@ -392,6 +451,26 @@ MAM_API int mam_ajtai_i8_commit(void *handle, const u64 *d_z, int batch, u64 *d_
 
 MAM_API int mam_ajtai_i8_commit_prefix(void *handle, const u64 *d_z, int batch, uint64_t n_used, u64 *d_out) {
     return aj_commit((AjtaiI8 *)handle, d_z, batch, (size_t)n_used, d_out);
+}
+
+// The commit of mam_ajtai_i8_commit_prefix without any allocation: the chunk partials go to
+// the file-scope `scratch` array of the current device (AJ_SCRATCH_BYTES) instead of a
+// buffer the library allocates. Returns once the work is queued on the calling thread's
+// stream (d_out is complete when that stream is synchronized); -2 for the arguments
+// mam_ajtai_i8_commit_prefix rejects and when the partials need more than AJ_SCRATCH_BYTES
+// (chunks x rows x 16 ceil((8 batch + [n_used < n]) / 16) x 4 bytes, with 2^13 columns per
+// chunk up to 32 planes and 2^14 above: at rows = 256 and n_used = 2^22, batch <= 8). Every
+// call on one GPU shares that GPU's array: calls that overlap on one GPU (other threads'
+// streams, other caller streams) race; calls queued on one stream do not.
+MAM_API int mam_ajtai_i8_commit_noalloc(void *handle, const u64 *d_z, int batch, uint64_t n_used, u64 *d_out) {
+    AjtaiI8 *A = (AjtaiI8 *)handle;
+    AjPlan P;
+    int rc = aj_plan(A, d_z, batch, (size_t)n_used, d_out, P);
+    if (rc) return rc;
+    if (P.partial_bytes > AJ_SCRATCH_BYTES) return -2;
+    int *ptr = scratch_on_current_device();
+    if (!ptr) return -1;
+    return aj_enqueue(A, d_z, batch, (size_t)n_used, P, ptr, d_out, mam_stream());
 }
 
 // Copy `count` matrix entries of one row starting at column col0 to host (int8).
