@@ -2,6 +2,7 @@
 (include/morpheus_prover.h, mam_prove_batch): each instance proves sum f*g with the product
 sumcheck and opens a Basefold commitment of f at the sumcheck challenges, on one transcript."""
 import ctypes
+import time
 
 import numpy as np
 
@@ -47,3 +48,21 @@ def prove_batch(instances, table, log_rate=1, hash_kind=0, num_queries=100, work
     for i, o in enumerate(outs):
         o["start_ms"], o["end_ms"] = arr[i].start_ms, arr[i].end_ms
     return outs
+
+
+def autotune_workers(instances, table, log_rate=1, hash_kind=0, num_queries=100, candidates=(1, 2, 4, 8, 12, 16),
+                     stride=4, repeats=2):
+    """Worker count that proves every `stride`-th instance fastest, after one untimed warm-up
+    at the largest candidate (its threads and buffers). Returns (best, {workers: seconds})."""
+    sub = [instances[i] for i in range(0, len(instances), stride)] or list(instances)
+    prove_batch(sub, table, log_rate, hash_kind, num_queries, workers=max(candidates))
+    times = {}
+    for w in candidates:
+        best = None
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            prove_batch(sub, table, log_rate, hash_kind, num_queries, workers=w)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        times[w] = best
+    return min(times, key=times.get), times

@@ -243,6 +243,7 @@ MAM_API int ext2_to_gl_batch_ffi(const u64 *d_input, u64 *d_output, int n) {
 #include <vector>
 
 #include "profile.h"
+#include "sync.h"
 
 namespace {
 struct ProfRec {
@@ -390,7 +391,7 @@ __global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, 
     else if (t < na + nb)
         box[t] = b[t - na];
     __threadfence_system();
-    __syncthreads();
+    mam_barrier();
     if (t == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 

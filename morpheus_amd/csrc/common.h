@@ -15,6 +15,7 @@
 #include <stdint.h>
 
 #include "field.h"
+#include "sync.h"
 
 #define MAM_API extern "C" __attribute__((visibility("default")))
 
@@ -92,7 +93,7 @@ int mam_mailbox_wait(const u64 *box, u64 seq, int n, u64 *out, hipStream_t s);
 // make them visible to the host, then release the sequence flag.
 __device__ __forceinline__ void mam_mailbox_raise(u64 *box, u64 seq) {
     __threadfence_system();
-    __syncthreads();
+    mam_barrier();
     if (threadIdx.x == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 

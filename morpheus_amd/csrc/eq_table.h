@@ -12,6 +12,7 @@
 #pragma once
 
 #include "fpolicy.h"
+#include "sync.h"
 
 namespace eqt {
 
@@ -88,7 +89,7 @@ __global__ __launch_bounds__(256) void k_eq(const u64 *__restrict__ r_all, size_
         }
         O::put(tb, j, v);
     }
-    __syncthreads();
+    mam_barrier();
     const int per = 1 << (la + lb), amask = (1 << la) - 1;
     const size_t base = (size_t)blockIdx.x << (la + lb);
     for (int j = t; j < per; j += 256) O::put(out, base + j, O::mul(O::get(ta, j & amask), O::get(tb, j >> la)));

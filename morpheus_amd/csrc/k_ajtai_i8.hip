@@ -37,6 +37,7 @@
 #include "chacha.h"
 #include "common.h"
 #include "profile.h"
+#include "sync.h"
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__
 
     stage_load(kt0);
     stage_store(0, kt0);
-    __syncthreads();
+    mam_barrier();
     int cur = 0;
     for (size_t kt = kt0; kt < kt1; kt += DEPTH) {
         const bool more = kt + DEPTH < kt1;
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(AJF_THREADS) void k_aj_gemm(const v4i *__restrict__
             }
         }
         if (more) stage_store(cur ^ 1, kt + DEPTH);
-        __syncthreads();
+        mam_barrier();
         cur ^= 1;
     }
     if (!computes) return;
@@ -237,10 +238,10 @@ __global__ __launch_bounds__(256) void k_aj_finish(const int *__restrict__ parti
     long long s = 0;
     for (int c = sl; c < chunks; c += 32) s += partial[((size_t)c * rows + i) * cols + w * 8 + l];
     red[threadIdx.x] = s;
-    __syncthreads();
+    mam_barrier();
     for (int h = 128; h >= 8; h >>= 1) {
         if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
-        __syncthreads();
+        mam_barrier();
     }
     if (threadIdx.x == 0) {
         long long R = 0;

@@ -22,6 +22,7 @@
 #include "common.h"
 #include "fieldq.h"
 #include "profile.h"
+#include "sync.h"
 
 static constexpr int RING_D = 64;
 static constexpr int RING_KAPPA = 42;
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(64) void k_ring_dense(const u32 *__restrict__ d_key
     const u64 j1 = j0 + cols < N ? j0 + cols : N;
     for (u64 j = j0; j < j1; j++) {
         ring_fill(key, g, j + col_offset, L);
-        __syncthreads();
+        mam_barrier();
 #pragma unroll
         for (int w = 0; w < BW; w++) {
             if (b0 + w >= B) break;
@@ -136,7 +137,7 @@ __global__ __launch_bounds__(64) void k_ring_dense(const u32 *__restrict__ d_key
                 }
             }
         }
-        __syncthreads();
+        mam_barrier();
     }
 #pragma unroll
     for (int w = 0; w < BW; w++) {
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(64) void k_ring_wide(const u32 *__restrict__ d_key,
     for (u64 j = j0; j < j1; j++) {
         ring_fill(key, g, j + col_offset, L);
         zs[r] = fq::canon(z[j * RING_D + r]);
-        __syncthreads();
+        mam_barrier();
         for (int l = 0; l < RING_D; l++) {
             const u64 zl = zs[l];
 #pragma unroll
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(64) void k_ring_wide(const u32 *__restrict__ d_key,
                 a2[ii] += a1[ii] < t;
             }
         }
-        __syncthreads();
+        mam_barrier();
     }
 #pragma unroll
     for (int ii = 0; ii < RING_RG; ii++) {
@@ -212,9 +213,9 @@ __global__ __launch_bounds__(64) void k_ring_sparse(const u32 *__restrict__ d_ke
         const u64 j = p >> 6;
         const int l = (int)(p & 63);
         if (j != cur) {  // consecutive positions in one column share its matrix entries
-            __syncthreads();
+            mam_barrier();
             ring_fill(key, g, j, L);
-            __syncthreads();
+            mam_barrier();
             cur = j;
         }
 #pragma unroll

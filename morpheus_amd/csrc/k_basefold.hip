@@ -27,6 +27,7 @@
 #include "open_fold.h"
 #include "profile.h"
 #include "reduce.h"
+#include "sync.h"
 
 // ---------------------------------------------------------------------------
 // Radix-256 butterfly transform passes
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(256) void k_bfly_pass(const u64 *src, u64 *dst, int
             t[LIDX(row * BF_COLS + col)] = gl_canon(src[base + ((size_t)row << b0) + col]);
         }
     }
-    __syncthreads();
+    mam_barrier();
     for (int j = 0; j < kb; j++) {
         for (int p = threadIdx.x; p < T / 2; p += blockDim.x) {
             int col, rp;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void k_bfly_pass(const u64 *src, u64 *dst, int
             t[LIDX(elo)] = lo;
             t[LIDX(ehi)] = hi;
         }
-        __syncthreads();
+        mam_barrier();
     }
     for (int r = 0; r < reps; r++) {
         u64 *d = dst + r * rep_stride;
@@ -131,7 +132,7 @@ __global__ void k_bfly_small(const u64 *src, u64 *dst, int nbits, int op, int re
     __shared__ u64 t[4096];
     const int n = 1 << nbits;
     for (int e = threadIdx.x; e < n; e += blockDim.x) t[e] = gl_canon(src[e]);
-    __syncthreads();
+    mam_barrier();
     for (int j = 0; j < nbits; j++) {
         for (int p = threadIdx.x; p < n / 2; p += blockDim.x) {
             int lo_i = ((p >> j) << (j + 1)) | (p & ((1 << j) - 1));
@@ -143,7 +144,7 @@ __global__ void k_bfly_small(const u64 *src, u64 *dst, int nbits, int op, int re
             t[lo_i] = lo;
             t[hi_i] = hi;
         }
-        __syncthreads();
+        mam_barrier();
     }
     for (int r = 0; r < reps; r++)
         for (int e = threadIdx.x; e < n; e += blockDim.x) dst[r * rep_stride + e] = t[e];
@@ -194,7 +195,7 @@ __global__ __launch_bounds__(256) void k_bitrev_tiled(const u64 *__restrict__ sr
 #pragma unroll
         for (int w = 0; w < W; w++) tile[a][b * W + w] = gl_canon(src[i * W + w]);
     }
-    __syncthreads();
+    mam_barrier();
     const u32 rm = brev_bits(m, mb);
     for (int e = threadIdx.x; e < 1024; e += 256) {
         u32 b = e >> 5, ap = e & 31;
@@ -688,7 +689,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_u(UJob j) {
         tab[g][m][0] = v.c0;
         tab[g][m][1] = v.c1;
     }
-    __syncthreads();
+    mam_barrier();
     const int ng = j.nb > 0 ? (j.nb + 5) / 6 : 1;
     const ext2 ch = j.ch;
     const u64 ch7 = gl_mul7(ch.c1);

@@ -18,6 +18,7 @@
 // access to in[] / out[] with a runtime count is an unrolled select, so nothing spills.
 #include "common.h"
 #include "poseidon2.h"
+#include "sync.h"
 
 namespace {
 
@@ -196,7 +197,7 @@ __global__ void k_bit_permute(const u64 *__restrict__ in, u64 *__restrict__ out,
                               int n_bits, int total) {
     __shared__ int map[32];
     if (threadIdx.x < 32) map[threadIdx.x] = threadIdx.x < n_bits ? (inv_map[threadIdx.x] & 31) : 0;
-    __syncthreads();
+    mam_barrier();
     const int x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= total) return;
     unsigned y = 0;

@@ -15,6 +15,7 @@
 // slices the Ext2 partials are summed by a second kernel.
 #include "common.h"
 #include "fpolicy.h"
+#include "sync.h"
 
 template <class F>
 __global__ __launch_bounds__(256) void k_fused_peval(const u64 *__restrict__ evals, const u64 *__restrict__ eq,
@@ -25,7 +26,7 @@ __global__ __launch_bounds__(256) void k_fused_peval(const u64 *__restrict__ eva
     const u32 lo_n = 1u << half, hi_n = 1u << (n - half);
     for (u32 i = threadIdx.x; i < lo_n; i += blockDim.x) luts[i] = lo_lut[i];
     for (u32 i = threadIdx.x; i < hi_n; i += blockDim.x) luts[lo_n + i] = hi_lut[i];
-    __syncthreads();
+    mam_barrier();
     const u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x;
     if (t >= out_size * slices) return;
     const u64 j = t % out_size, s = t / out_size;

@@ -21,6 +21,7 @@
 #include "open_fold.h"
 #include "poseidon2.h"
 #include "profile.h"
+#include "sync.h"
 
 struct MonoHash {
     static constexpr bool kWide = true;
@@ -96,13 +97,13 @@ __global__ __launch_bounds__(TREE_BLOCK) void k_tree_levels(const u64 *__restric
     size_t level_n = n_first;
     int block_n = (int)(n_first < TREE_BLOCK ? n_first : TREE_BLOCK);
     for (int l = 1; l < nlev; l++) {
-        __syncthreads();
+        mam_barrier();
         level_off += level_n;
         level_n >>= 1;
         block_n >>= 1;
         u64 r[4];
         if (t < block_n) H::node(sh + 8 * t, sh + 8 * t + 4, r);
-        __syncthreads();
+        mam_barrier();
         if (t < block_n) {
             u64 *o = tree + 4 * (level_off + (size_t)blockIdx.x * (TREE_BLOCK >> l) + t);
 #pragma unroll
@@ -179,12 +180,12 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
     size_t level_off = first_off, level_n = n_first;
     int block_n = (int)(n_first < WIDE_GROUPS ? n_first : WIDE_GROUPS);
     for (int l = 1; l < nlev; l++) {
-        __syncthreads();
+        mam_barrier();
         level_off += level_n;
         level_n >>= 1;
         block_n >>= 1;
         u64 y = (g < block_n && j < 8) ? sh[8 * g + j] : 0;
-        __syncthreads();
+        mam_barrier();
         // wave-uniform: a wave (4 groups) runs the permutation if its first group is active
         if ((g & ~3) < block_n) y = mono_wide::permute(y, j, wst[g], lc);
         if (g < block_n && j < 4) {
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
         }
     }
     if (mj.box) {
-        __syncthreads();  // the root is in sh[0..3]
+        mam_barrier();  // the root is in sh[0..3]
         const int t = threadIdx.x;
         if (t < mj.na + 4) mj.box[t] = t < mj.na ? mj.a[t] : sh[t - mj.na];
         mam_mailbox_raise(mj.box, mj.seq);

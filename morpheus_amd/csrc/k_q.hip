@@ -10,6 +10,7 @@
 #include "eq_table.h"
 #include "fpolicy.h"
 #include "sumcheck.h"
+#include "sync.h"
 
 // almost-goldilocks-cuda-rs/src/ffi.rs:13 (the q crate's init; same device as the p crate)
 MAM_API int almost_goldilocks_cuda_init() {
@@ -344,7 +345,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_round_msg_f(const u64 *__restrict
             lds[(c * (MAM_BLOCK / 64) + wave) * 2 + 1] = acc[c].c1;
         }
     }
-    __syncthreads();
+    mam_barrier();
     if (threadIdx.x == 0) {
         const int nw = blockDim.x / 64;
         for (int c = 0; c <= d; c++) {
@@ -443,7 +444,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_batched_msg_f(const u64 *__restri
             lds[(c * (MAM_BLOCK / 64) + wave) * 2 + 1] = acc[c].c1;
         }
     }
-    __syncthreads();
+    mam_barrier();
     if (threadIdx.x == 0) {
         const int nw = blockDim.x / 64;
         for (int c = 0; c < 3; c++) {
@@ -516,7 +517,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_batched_r0bin_msg_f(const u64 *__
             lds[(c * (MAM_BLOCK / 64) + wave) * 2 + 1] = acc[c].c1;
         }
     }
-    __syncthreads();
+    mam_barrier();
     if (threadIdx.x == 0) {
         const int nw = blockDim.x / 64;
         for (int c = 0; c < 3; c++) {

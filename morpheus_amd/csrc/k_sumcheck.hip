@@ -18,6 +18,7 @@
 #include "profile.h"
 #include "reduce.h"
 #include "sumcheck.h"
+#include "sync.h"
 
 // ---------------------------------------------------------------------------
 // Reference-named kernels (per-workgroup partial sums; caller reduces)
@@ -533,7 +534,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict_
             ev[slot][sub][0] = e.c0;
             ev[slot][sub][1] = e.c1;
         }
-        __syncthreads();
+        mam_barrier();
         if (live && sub < T::ALL) {
             auto E = [&](int i, int t) { return ext2{ev[slot][i * G + t][0], ev[slot][i * G + t][1]}; };
             if (sub < T::MSG) {
@@ -580,7 +581,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict_
                 }
             }
         }
-        __syncthreads();
+        mam_barrier();
     }
     // lanes of one task: across the wave (lane offsets 16, 32), then across the waves
 #pragma unroll
@@ -593,7 +594,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict_
 #pragma unroll
         for (int k = 0; k < AW; k++) red[wave][lane][k] = acc[k];
     }
-    __syncthreads();
+    mam_barrier();
     // thread t < W forms word t of the workgroup's sums: msg[c] (t = 2 c + h), then
     // sym[c][k] (t = 2 (NP + 1) + 2 (c (NP + 1) + k) + h)
     u64 word = 0;

@@ -15,6 +15,7 @@
 
 #include "common.h"
 #include "field.h"
+#include "sync.h"
 
 static constexpr int WAVE = 64;
 
@@ -37,12 +38,12 @@ __device__ __forceinline__ void block_sum(u64 (&v)[M], u64 *lds) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
     const int nwaves = (blockDim.x + WAVE - 1) / WAVE;
-    __syncthreads();
+    mam_barrier();
     if (lane == 0) {
 #pragma unroll
         for (int k = 0; k < M; k++) lds[wave * M + k] = v[k];
     }
-    __syncthreads();
+    mam_barrier();
     if (threadIdx.x == 0) {
         for (int w = 1; w < nwaves; w++) {
 #pragma unroll
@@ -75,7 +76,7 @@ __device__ __forceinline__ bool grid_finish_n(u64 (&v)[M], u64 *lds, u64 *partia
         u32 ticket = atomicAdd(counter, 1u);
         s_last = (ticket == nblocks - 1) ? 1u : 0u;
     }
-    __syncthreads();
+    mam_barrier();
     if (!s_last) return false;
     __threadfence();
 #pragma unroll
@@ -134,7 +135,7 @@ __global__ __launch_bounds__(256) void k_sum_partials(const u64 *__restrict__ pa
         for (; b < nblocks; b += L) v = gl_add(v, partials[(size_t)b * M + k]);
     }
     fin[threadIdx.x] = v;
-    __syncthreads();
+    mam_barrier();
     if ((int)threadIdx.x < M) {
         u64 sum = 0;
         for (int j = 0; j < L; j++) sum = gl_add(sum, fin[j * M + threadIdx.x]);

@@ -328,8 +328,11 @@ def test_sumcheck_reference_ffi_loop_and_state():
     st.free()
 
 
+# Poseidon2 trees run every opening round in product form (k_open_u): nv 16 and 17 give the
+# eq factor 3 sub-table groups; Monolith at nv 20 runs its product-form rounds with 4 groups.
 @pytest.mark.parametrize("nv,lr,nq,hash_kind", [(2, 1, 3, 0), (3, 1, 10, 0), (6, 3, 34, 0), (9, 1, 20, 1),
-                                                 (12, 1, 100, 0), (14, 2, 30, 0)])
+                                                 (12, 1, 100, 0), (14, 2, 30, 0), (16, 1, 20, 1), (17, 1, 16, 1),
+                                                 (20, 1, 8, 0)])
 def test_basefold_open_matches_oracle(nv, lr, nq, hash_kind):
     say("open", nv, lr, nq, hash_kind)
     rng = np.random.default_rng(1000 + nv)

@@ -304,3 +304,19 @@ def test_ajtai_i8_is_exact_matvec():
         for w_ in range(batch):
             want = sum(row[j] * (int(z[w_ * n + j]) % P) for j in range(n)) % P
             assert int(got[w_ * rows + i]) == want
+
+
+def test_numpy_goldilocks_helpers_against_python_ints():
+    from refmath import gl_dot_np, gl_mul_np
+    rng = np.random.default_rng(99)
+    edge = [0, 1, 2, P - 1, P, P + 1, (1 << 64) - 1, (1 << 32) - 1, 1 << 32, (1 << 63), P - (1 << 32)]
+    a = np.concatenate([np.array(edge, dtype=np.uint64), rng.integers(0, 1 << 64, 20000, dtype=np.uint64)])
+    b = np.concatenate([np.array(edge[::-1], dtype=np.uint64), rng.integers(0, 1 << 64, 20000, dtype=np.uint64)])
+    got = gl_mul_np(a, b)
+    for x, y, z in zip(a.tolist(), b.tolist(), got.tolist()):
+        assert z == x * y % P, (x, y)
+    ea = np.array(edge, dtype=np.uint64)
+    for x in edge:
+        col = gl_mul_np(np.full(len(edge), x, dtype=np.uint64), ea)
+        assert col.tolist() == [x * y % P for y in edge]
+    assert gl_dot_np(a, b) == sum(x * y for x, y in zip(a.tolist(), b.tolist())) % P
