@@ -6,9 +6,12 @@ Monolith-w12 Merkle trees, 22 fold rounds with Ext2 challenges from the Poseidon
 transcript, 100 query openings. A step is one full commit + open with the evaluations
 already resident in HBM; the folding table (setup) is built once before timing.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload basefold|sumcheck]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|basefold|sumcheck|ajtai|batch|serve|...]
 
-Prints one JSON line. `value` is proved elements per second (2^22 per step).
+Prints one JSON line. `value` is proved elements per second (2^22 per step). The default
+workload `all` adds, in the same line, a record per other benchmark configuration (the 2^20
+product sumcheck, the int8 Ajtai commit, the 64-instance batch, the matmul proving service with
+its proofs/s and p50/p99 latency), each with its roofline, CPU baseline and verification.
 """
 import argparse
 import ctypes
@@ -25,15 +28,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "proved Goldilocks elems/s (sumcheck+Basefold, 2^22); proofs/s + p99 latency"
 HBM_PEAK_GBS = 8000.0
-# Nominal 32-bit VALU lane-operations per second: 256 CUs x 64 lanes x 2.4 GHz (AMD's
-# 157.3 TFLOP/s FP32 vector figure counts an FMA as 2 flops on 2 packed lanes: 157.3 / 4).
-# The roofline uses the rate measured on the box by the VALU probe (valu_peak()): chains of
-# one instruction form, which issue at one wave instruction per ~4.1 cycles per SIMD. The
-# hash kernels' mix (about a third moves, selects and compares) issues at 3.3-3.7 cycles
-# per instruction (rocprofv3 SQ_INSTS_VALU and GRBM_GUI_ACTIVE), so a hash launch at the
-# VALU issue limit reads as frac slightly above 1.
+# VALU peak: 32-bit lane-operations per second, 256 CUs x 64 lanes x 2.4 GHz (AMD's MI355X FP32
+# vector figure, 157.3 TFLOP/s, counts an FMA as 2 flops on 2 packed lanes: 157.3 / 4). Hash
+# rooflines divide the algorithmic operation count of the work (counted from the definition,
+# ALG_OPS_PER_PERM) by the measured time and by this peak. The VALU probe (valu_peak()) is
+# reported beside it: chains of one instruction form issue at ~4.1 cycles per wave instruction.
 VALU_NOMINAL_TOPS = 39.3
-VALU_PEAK_TOPS = VALU_NOMINAL_TOPS  # replaced by the measured rate in main()
 
 
 VALU_PROBE = {}
@@ -51,9 +51,21 @@ def valu_peak(M):
         waves = blocks * 256 // 64
         VALU_PROBE[name] = waves * iters * 8 * 64 / (ms.value / 1e3) / 1e12
     return max(VALU_PROBE.values())
-# VALU lane-operations of one Monolith permutation in the leaf kernel: SQ_INSTS_VALU x 64
+# Algorithmic 32-bit VALU operations of one Monolith-w12 permutation, counted from its
+# definition (goldilocks-cuda-rs/src/cpu_monolith.rs:65-171) by a fixed rule:
+#   7 concrete layers x 144 multiply-adds of a 64-bit word by a small constant, 2 ops each
+#     (one 32 x 32 + 64 multiply-add per 32-bit half), plus 12 row reductions of 3 ops;
+#   5 x 12 round-constant additions, 2 ops each;
+#   6 x 11 bricks x_j += x_(j-1)^2 mod p, 12 ops each (three 32 x 32 products, carries and a
+#     128 -> 64 bit reduction);
+#   6 x 4 bars, 24 ops each (per 32-bit half: three byte rotations of 3 ops, the chi step
+#     and the final rotation).
+MONO_ALG_OPS = {"concrete": 7 * (144 * 2 + 12 * 3), "round_constants": 5 * 12 * 2, "bricks": 6 * 11 * 12,
+                "bars": 6 * 4 * 24}
+ALG_OPS_PER_PERM = float(sum(MONO_ALG_OPS.values()))
+# VALU lane-operations the leaf kernel executes per Monolith permutation: SQ_INSTS_VALU x 64
 # over the permutations of one k_tree_levels<MonoHash, SRC_GL> dispatch (rocprofv3 --pmc,
-# profiles/merkle_leaf_kernel_2p22.md).
+# profiles/merkle_leaf_kernel_2p22.md). Reported beside the algorithmic count.
 VALU_OPS_PER_PERM = 4982.0
 # VALU instructions one lane executes for one full Monolith permutation with every lane
 # active: SQ_INSTS_VALU / SQ_WAVES of k_mono_permute_n (rocprofv3 --pmc,
@@ -76,9 +88,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["basefold", "sumcheck", "ajtai", "ring_ajtai", "batch", "hash", "serve",
-                                                 "batch_open"],
-                    default="basefold")
+    ap.add_argument("--workload", choices=["all", "basefold", "sumcheck", "ajtai", "ring_ajtai", "batch", "hash",
+                                           "serve", "batch_open"],
+                    default="all")
     ap.add_argument("--instances", type=int, default=64, help="instances per batch (batch workload)")
     ap.add_argument("--workers", type=int, default=0, help="batch workers (0: autotune)")
     ap.add_argument("--batch", type=int, default=1, help="witnesses per Ajtai commit")
@@ -113,6 +125,126 @@ def profile_collect(lib):
     work = (ctypes.c_double * 7)()
     assert lib.mam_profile_collect(ms, cnt, work) == 0
     return [(ms[i], cnt[i], work[i]) for i in range(7)]
+
+
+def cpu_facts():
+    """CPU model, visible cores and the thread count of the all-core runs (the job's CPU share:
+    OMP_NUM_THREADS when set, else every visible core)."""
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    try:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or nproc
+    except ValueError:
+        threads = nproc
+    return {"cpu_model": model, "nproc": nproc, "threads_all": min(threads, nproc)}
+
+
+def oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+    return O
+
+
+def timed_threads(O, threads, fn):
+    O.set_threads(threads)
+    t0 = time.perf_counter()
+    r = fn()
+    return time.perf_counter() - t0, r
+
+
+def cpu_basefold_all(nv_all, nv_single, lr, nq):
+    """The oracle's commit + open_ext2 (the opening takes the commitment's tree, as the
+    reference's CPU opening takes it from the commitment cache) on every core at nv_all and on
+    one core at nv_single; value is the all-core rate."""
+    O = oracle()
+    f = cpu_facts()
+
+    def run(nv):
+        ev = gen_inputs(nv, seed=3)
+        pt = np.resize(gen_inputs(6, seed=5), 2 * nv)
+        table = O.basefold_table(nv, lr, nv, 42)
+
+        def go():
+            cw, bh, tree = O.basefold_commit(ev, nv, lr, O.MONOLITH)
+            O.basefold_open_ext2(cw, bh, tree[-4:], nv, lr, O.MONOLITH, pt, table, nv, O.Transcript(b"bench"), nq,
+                                 tree=tree)
+        return go
+    da, _ = timed_threads(O, f["threads_all"], run(nv_all))
+    ds, _ = timed_threads(O, 1, run(nv_single))
+    return {"value": (1 << nv_all) / da, "unit": "elems/s", "cores": f["threads_all"], "kind": "port",
+            "sample": f"oracle commit + open_ext2 of 2^{nv_all} evaluations (rate 1/2, Monolith, {nq} queries) on "
+                      f"{f['threads_all']} threads, {da:.2f} s",
+            "single_thread": {"value": (1 << nv_single) / ds, "unit": "elems/s", "cores": 1,
+                              "sample": f"the same at 2^{nv_single} on one thread, {ds:.2f} s"},
+            "cpu_model": f["cpu_model"], "nproc": f["nproc"]}
+
+
+def cpu_sumcheck_all(nv):
+    O = oracle()
+    f = cpu_facts()
+    fg = gen_inputs(nv + 1, seed=6)
+    ext = np.zeros((2, 1 << nv, 2), dtype=np.uint64)
+    ext[:, :, 0] = fg.reshape(2, -1)
+    out = {}
+    for name, thr in (("all", f["threads_all"]), ("single", 1)):
+        dt, _ = timed_threads(O, thr, lambda: O.sumcheck_prove_ext2(O.Transcript(b"bench"), ext.copy(), 2, nv))
+        out[name] = dt
+    return {"value": (2 << nv) / out["all"], "unit": "elems/s", "cores": f["threads_all"], "kind": "port",
+            "sample": f"oracle Ext2 sumcheck of f*g, 2 x 2^{nv} evaluations, {f['threads_all']} threads, "
+                      f"{out['all']:.2f} s",
+            "single_thread": {"value": (2 << nv) / out["single"], "unit": "elems/s", "cores": 1,
+                              "sample": f"the same on one thread, {out['single']:.2f} s"},
+            "cpu_model": f["cpu_model"], "nproc": f["nproc"]}
+
+
+def cpu_ajtai_all(nv):
+    O = oracle()
+    f = cpu_facts()
+    from morpheus_amd.ajtai import key_from_seed
+    z = gen_inputs(nv, seed=11)
+    out = {}
+    for name, thr in (("all", f["threads_all"]), ("single", 1)):
+        out[name], _ = timed_threads(O, thr, lambda: O.ajtai_i8_commit(key_from_seed(2024), 256, 1 << nv, z, 1))
+    return {"value": (1 << nv) / out["all"], "unit": "elems/s", "cores": f["threads_all"], "kind": "port",
+            "sample": f"oracle int8 Ajtai (ChaCha8 matrix rows + exact matvec), 256 x 2^{nv}, {f['threads_all']} "
+                      f"threads, {out['all']:.2f} s",
+            "single_thread": {"value": (1 << nv) / out["single"], "unit": "elems/s", "cores": 1,
+                              "sample": f"the same on one thread, {out['single']:.2f} s"},
+            "cpu_model": f["cpu_model"], "nproc": f["nproc"]}
+
+
+def cpu_batch_sample(data, stride, lr, nq):
+    """The oracle proving every stride-th batched instance (product sumcheck, commit, open on
+    one transcript) on every core; value is proved elements per second over the sample."""
+    O = oracle()
+    f = cpu_facts()
+    sample = data[::stride]
+    tnv = max(int(fv.size).bit_length() - 1 for fv, _, _ in sample)
+    table = O.basefold_table(tnv, lr, tnv, 42)
+
+    def go():
+        for fv, gv, label in sample:
+            nv = int(fv.size).bit_length() - 1
+            ext = np.zeros((2, 1 << nv, 2), dtype=np.uint64)
+            ext[0, :, 0], ext[1, :, 0] = fv, gv
+            t = O.Transcript(label)
+            sc = O.sumcheck_prove_ext2(t, ext, 2, nv)
+            cw, bh, tree = O.basefold_commit(fv, nv, lr, O.MONOLITH)
+            O.basefold_open_ext2(cw, bh, tree[-4:], nv, lr, O.MONOLITH, sc["challenges"].ravel(), table, tnv, t, nq,
+                                 tree=tree)
+    dt, _ = timed_threads(O, f["threads_all"], go)
+    elems = sum(int(fv.size) for fv, _, _ in sample)
+    return {"value": elems / dt, "unit": "elems/s", "cores": f["threads_all"], "kind": "port",
+            "sample": f"oracle sumcheck + commit + open of every {stride}th instance ({len(sample)} instances, "
+                      f"{elems} padded elements) on {f['threads_all']} threads, {dt:.2f} s",
+            "cpu_model": f["cpu_model"], "nproc": f["nproc"]}
 
 
 def cpu_baseline_basefold(nv, lr, nq):
@@ -239,15 +371,26 @@ def bench_basefold(args, M):
     lk_avg_ms = lk_ms / max(lk_n, 1)
     perms_per_launch = lk_perm / max(lk_n, 1)
     perm_rate = perms_per_launch / (lk_avg_ms / 1e3) if lk_ms > 0 else 0.0
-    ach = perm_rate * VALU_OPS_PER_PERM / 1e12
+    ach = perm_rate * ALG_OPS_PER_PERM / 1e12
     roof = {"kernel": "k_tree_levels<MonoHash, SRC_GL> (2^22 leaves + 2 in-LDS levels)",
-            "bound": "valu", "unit": "Tops/s", "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
-            "peak_source": "VALU probe (mam_valu_probe)", "probe_tops": dict(VALU_PROBE), "achieved": ach,
-            "frac": ach / VALU_PEAK_TOPS, "traffic": LEAF_LAUNCH_TRAFFIC_BYTES if nv == 22 else None,
+            "bound": "valu", "unit": "Tops/s", "achieved": ach, "peak": VALU_NOMINAL_TOPS,
+            "frac": ach / VALU_NOMINAL_TOPS,
+            "peak_source": "MI355X FP32 vector 157.3 TFLOP/s / 4 (no packing, no FMA): one 32-bit lane-op per "
+                           "lane per clock, 256 CUs x 64 lanes x 2.4 GHz",
+            "achieved_from": "permutations per second of the launch x algorithmic 32-bit ops per permutation "
+                             "(counted from the definition)",
+            "alg_ops_per_perm": ALG_OPS_PER_PERM, "alg_ops_breakdown": dict(MONO_ALG_OPS),
+            "traffic": LEAF_LAUNCH_TRAFFIC_BYTES if nv == 22 else None,
             "avg_launch_ms": lk_avg_ms, "launches": lk_n, "perms_per_launch": perms_per_launch,
-            "perms_per_s": perm_rate, "valu_ops_per_perm": VALU_OPS_PER_PERM}
+            "perms_per_s": perm_rate,
+            "executed": {"valu_ops_per_perm": VALU_OPS_PER_PERM, "tops": perm_rate * VALU_OPS_PER_PERM / 1e12,
+                         "note": "SQ_INSTS_VALU x 64 per permutation (rocprofv3 --pmc); the kernel's instruction "
+                                 "mix issues faster than one wave instruction per 4 cycles per SIMD"},
+            "valu_probe_tops": dict(VALU_PROBE)}
     mk_ms, mk_n, mk_perm = prof_all[0]
     roof["all_merkle_builds"] = {"perms_per_s": mk_perm / (mk_ms / 1e3) if mk_ms > 0 else 0.0,
+                                 "frac": (mk_perm / (mk_ms / 1e3) * ALG_OPS_PER_PERM / 1e12 / VALU_NOMINAL_TOPS)
+                                 if mk_ms > 0 else 0.0,
                                  "avg_build_ms": mk_ms / max(mk_n, 1), "builds_per_step": mk_n / 3,
                                  "from": "3 profiled steps after the timed region"}
     hbm = {}
@@ -270,6 +413,19 @@ def bench_basefold(args, M):
     return out
 
 
+def verify_sumcheck_proof(M, got, fv, gv, nv, label=b"bench"):
+    """The product's host checks of one f*g sumcheck proof: the verifier accepts it for the
+    claim sum f g (numpy, mod p) and f(r), g(r) (host MLE evaluation at the challenges) are the
+    final evaluations whose product ends the proof."""
+    from morpheus_amd.hostfield import e2mul, gl_dot_np, mle_eval
+    claim = (gl_dot_np(fv, gv), 0)
+    ok, chs = M.sumcheck_verify(M.Transcript(label), claim, 2, nv, got["round_msgs"], got["final_eval"])
+    r = np.array(chs, dtype=np.uint64).ravel()
+    fr, gr = mle_eval(fv, r), mle_eval(gv, r)
+    fe = [tuple(int(x) for x in got["final_evals"][i]) for i in range(2)]
+    return bool(ok and fe == [fr, gr] and e2mul(fr, gr) == tuple(int(x) for x in got["final_eval"]))
+
+
 def bench_sumcheck(args, M):
     nv = args.nv or 20
     f = gen_inputs(nv + 1)
@@ -279,8 +435,9 @@ def bench_sumcheck(args, M):
     def step():
         return M.sumcheck_prove(M.Transcript(b"bench"), d, 2, nv, base=True)
 
+    got = None
     for _ in range(args.warmup):
-        step()
+        got = step()
     M.synchronize()
     lib.mam_profile_enable_mask(1 << 3)  # the pass kernels only
     lat = []
@@ -296,6 +453,7 @@ def bench_sumcheck(args, M):
     ms, n, by = prof[3]
     gbs = by / (ms / 1e3) / 1e9 if ms > 0 else 0.0
     lat_ms = np.array(lat) * 1e3
+    verified = verify_sumcheck_proof(M, step(), f[:1 << nv], f[1 << nv:], nv)
     return {"metric": METRIC, "value": args.steps * (2 << nv) / elapsed, "unit": "elems/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
@@ -303,7 +461,7 @@ def bench_sumcheck(args, M):
             "config": {"workload": "sumcheck_product_f_g", "num_vars": nv, "num_polys": 2,
                        "transcript": "poseidon2-w8", "parallelism": "single-gpu"},
             "proofs_per_s": args.steps / elapsed, "p50_ms": float(np.percentile(lat_ms, 50)),
-            "p99_ms": float(np.percentile(lat_ms, 99)),
+            "p99_ms": float(np.percentile(lat_ms, 99)), "verified": verified,
             "roofline": {"kernel": "k_sc_pass (two rounds per pass: folds on read + message + next message "
                                    "as a polynomial in the challenge); 2^20 is latency-bound, the host transcript "
                                    "(20 rounds x ~20 us) dominates", "bound": "hbm", "achieved": gbs,
@@ -341,6 +499,17 @@ def bench_ajtai(args, M):
     gbs = by / max(cnt, 1) / (avg / 1e3) / 1e9 if ms > 0 else 0.0
     mfma_tops = 2.0 * rows * n * 8 * batch / (avg / 1e3) / 1e12 if ms > 0 else 0.0
     lat_ms = np.array(lat) * 1e3
+    # sampled rows of the commitment against exact integer products with the set-up's matrix
+    out_words = dout.to_numpy().reshape(batch, rows)
+    verified = True
+    m32 = np.uint64(0xFFFFFFFF)
+    for r in (0, 1, rows // 2, rows - 1):
+        mrow = A.matrix_row(r, 0, n).astype(np.int64)
+        for w in range(batch):
+            zw = z[w * n:(w + 1) * n] % np.uint64(M.P)
+            lo = int(np.dot(mrow, (zw & m32).astype(np.int64)))   # |term| < 2^39, sum < 2^61
+            hi = int(np.dot(mrow, (zw >> np.uint64(32)).astype(np.int64)))
+            verified &= (lo + (hi << 32)) % M.P == int(out_words[w, r])
     return {"metric": METRIC, "value": args.steps * batch * n / elapsed, "unit": "elems/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": elapsed / args.steps * 1e3,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8->int32 (u64 out)",
@@ -348,7 +517,7 @@ def bench_ajtai(args, M):
             "config": {"workload": "ajtai_int8_commit", "witness_len": n, "rows": rows, "batch": batch,
                        "limbs": 8, "matrix": "ChaCha8 int8, expanded once", "parallelism": "single-gpu"},
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
-            "setup_s": setup_s,
+            "setup_s": setup_s, "verified": bool(verified),
             "roofline": {"kernel": "k_aj_gemm (v_mfma_i32_16x16x64_i8, LDS-staged, fused limb split)",
                          "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS,
@@ -378,7 +547,7 @@ def bench_hash(args, M):
             M.check(call(), name)
         M.synchronize()
         res[name] = args.steps * n / (time.perf_counter() - t0)
-    ach = res["monolith_w12"] * MONO_PERM_VALU / 1e12
+    ach = res["monolith_w12"] * ALG_OPS_PER_PERM / 1e12
     return {"metric": METRIC, "value": res["monolith_w12"], "unit": "perms/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": n / res["monolith_w12"] * 1e3, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "u64 (Goldilocks)", "data": "synthetic",
@@ -386,18 +555,9 @@ def bench_hash(args, M):
                        "parallelism": "single-gpu"},
             "perms_per_s": res,
             "roofline": {"kernel": "k_mono_permute_n (one Monolith permutation per lane)", "bound": "valu",
-                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
-                         "probe_tops": dict(VALU_PROBE), "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
-                         "traffic": None, "valu_per_perm": MONO_PERM_VALU}}
-
-
-def batch_sizes(count, seed=4):
-    """log2 size ~ U[12, 22] (continuous, seeded); each instance is padded with zeros to the
-    next power of two. Returns (real sizes, num_vars)."""
-    u = np.random.default_rng(seed).uniform(12.0, 22.0, count)
-    n = np.floor(2.0 ** u).astype(np.int64)
-    nv = np.ceil(np.log2(n)).astype(np.int64)
-    return n, nv
+                         "achieved": ach, "peak": VALU_NOMINAL_TOPS, "unit": "Tops/s", "frac": ach / VALU_NOMINAL_TOPS,
+                         "traffic": None, "alg_ops_per_perm": ALG_OPS_PER_PERM,
+                         "executed_valu_per_perm": MONO_PERM_VALU, "valu_probe_tops": dict(VALU_PROBE)}}
 
 
 def merkle_perms(nv, lr):
@@ -408,53 +568,65 @@ def merkle_perms(nv, lr):
     return total
 
 
+def verify_batch_instance(M, o, fv, gv, label, table, lr, nq):
+    """Host checks of one batched instance: the sumcheck verifier accepts the proof for the
+    claim sum f g, f(r) and g(r) are its final evaluations, and the Basefold verifier accepts the
+    opening of f at r (evaluation f(r)) on the transcript the sumcheck leaves."""
+    from morpheus_amd.hostfield import e2mul, gl_dot_np, mle_eval
+    nv = int(fv.size).bit_length() - 1
+    t = M.Transcript(label)
+    fe = [tuple(int(x) for x in o["final_evals"][i]) for i in range(2)]
+    ok, chs = M.sumcheck_verify(t, (gl_dot_np(fv, gv), 0), 2, nv, o["round_msgs"], e2mul(fe[0], fe[1]))
+    r = np.array(chs, dtype=np.uint64).ravel()
+    ok = ok and fe == [mle_eval(fv, r), mle_eval(gv, r)] and tuple(int(x) for x in o["proof"][:2]) == fe[0]
+    return bool(ok and M.verify_ext2(o["root"], r, nv, lr, 0, o["proof"], nq, table, t))
+
+
 def bench_batch(args, M):
     """64 sumcheck + Basefold instances of log-uniform size 2^12..2^22, proved concurrently by
     persistent worker streams (count autotuned) against one-at-a-time proving."""
-    from morpheus_amd.batch import Instance, prove_batch
+    from morpheus_amd import workloads as W
+    from morpheus_amd.batch import Instance, autotune_workers, prove_batch
     lr, nq = 1, args.queries
-    sizes, nvs = batch_sizes(args.instances)
-    rng = np.random.default_rng(5)
-    insts = []
-    for i, (n, nv) in enumerate(zip(sizes, nvs)):
-        fg = np.zeros(2 << int(nv), dtype=np.uint64)
-        fg[:n] = rng.integers(0, M.P, int(n), dtype=np.uint64)
-        fg[(1 << int(nv)):(1 << int(nv)) + n] = rng.integers(0, M.P, int(n), dtype=np.uint64)
-        insts.append(Instance(fg[:1 << int(nv)], fg[1 << int(nv):], b"c4-%d" % i))
+    data = W.batch_instance_data(args.instances)
+    sizes, nvs = W.batch_sizes(args.instances)
+    insts = [Instance(f, g, label) for f, g, label in data]
     tnv = int(nvs.max())
     table = M.BasefoldTable.generate(tnv, lr, tnv, 42)
     table.upload()
     total = int(sizes.sum())
     perms = sum(merkle_perms(int(v), lr) for v in nvs)
 
-    def run(w, sub=None):
-        batch = insts if sub is None else [insts[i] for i in sub]
+    def run(w):
         t0 = time.perf_counter()
-        out = prove_batch(batch, table, lr, 0, nq, workers=w)
+        out = prove_batch(insts, table, lr, 0, nq, workers=w)
         return time.perf_counter() - t0, out
 
-    # autotune the worker count on a quarter of the instances (every fourth), untimed
-    cands = [args.workers] if args.workers else [1, 2, 4, 8, 12, 16]
-    sub = list(range(0, len(insts), 4))
-    run(cands[-1], sub)  # warm every candidate's threads and buffers
-    tune = {}
-    for w in cands:
-        run(w, sub)
-        tune[w] = min(run(w, sub)[0] for _ in range(2))
-    best = min(tune, key=tune.get)
+    # worker count autotuned on every fourth instance (untimed)
+    cands = (args.workers,) if args.workers else (1, 2, 4, 8, 12, 16)
+    best, tune = autotune_workers(insts, table, lr, 0, nq, candidates=cands)
     for _ in range(max(args.warmup, 1)):
         run(best)
         run(1)
     seq_t, seq_lat, bat_t, bat_lat = [], [], [], []
+    seq_out = bat_out = None
     for _ in range(args.steps):
-        dt, out = run(1)
+        dt, seq_out = run(1)
         seq_t.append(dt)
-        seq_lat += [o["end_ms"] - o["start_ms"] for o in out]
-        dt, out = run(best)
+        seq_lat += [o["end_ms"] - o["start_ms"] for o in seq_out]
+        dt, bat_out = run(best)
         bat_t.append(dt)
-        bat_lat += [o["end_ms"] - o["start_ms"] for o in out]
+        bat_lat += [o["end_ms"] - o["start_ms"] for o in bat_out]
     seq_s, bat_s = float(np.mean(seq_t)), float(np.mean(bat_t))
-    ach = perms * VALU_OPS_PER_PERM / bat_s / 1e12
+    # verification: batched proofs equal one-at-a-time proofs word for word, and the host
+    # verifiers accept the largest, the smallest and two middle instances
+    same = all(np.array_equal(a[k], b[k]) for a, b in zip(seq_out, bat_out)
+               for k in ("round_msgs", "challenges", "final_evals", "root", "proof"))
+    order = np.argsort(sizes)
+    sample = sorted({int(order[-1]), int(order[0]), int(order[len(order) // 2]), int(order[len(order) // 4])})
+    host_ok = all(verify_batch_instance(M, bat_out[i], data[i][0], data[i][1], data[i][2], table, lr, nq)
+                  for i in sample)
+    ach = perms * ALG_OPS_PER_PERM / bat_s / 1e12
     return {"metric": METRIC, "value": total / bat_s, "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": bat_s * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64 (Goldilocks, Ext2 challenges)", "data": "synthetic",
@@ -463,14 +635,16 @@ def bench_batch(args, M):
                        "log_rate": lr, "num_queries": nq, "merkle_hash": "monolith-w12", "workers": best,
                        "autotune_s": {str(k): v for k, v in tune.items()}, "parallelism": "single-gpu"},
             "proofs_per_s": len(insts) / bat_s,
+            "verified": bool(same and host_ok),
+            "verification": {"batched_equals_one_at_a_time": bool(same), "host_verifiers_accept": bool(host_ok),
+                             "host_checked_instances": sample},
             "sequential": {"value": total / seq_s, "ms_per_batch": seq_s * 1e3, "proofs_per_s": len(insts) / seq_s,
                            "p50_ms": float(np.percentile(seq_lat, 50)), "p99_ms": float(np.percentile(seq_lat, 99))},
             "speedup_vs_sequential": seq_s / bat_s,
             "p50_ms": float(np.percentile(bat_lat, 50)), "p99_ms": float(np.percentile(bat_lat, 99)),
             "roofline": {"kernel": "all Monolith Merkle work of the batch over its wall time", "bound": "valu",
-                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
-                         "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
-                         "traffic": None, "perms": perms, "valu_ops_per_perm": VALU_OPS_PER_PERM}}
+                         "achieved": ach, "peak": VALU_NOMINAL_TOPS, "unit": "Tops/s", "frac": ach / VALU_NOMINAL_TOPS,
+                         "traffic": None, "perms": perms, "alg_ops_per_perm": ALG_OPS_PER_PERM}}
 
 
 def batch_open_claims(polys):
@@ -524,7 +698,7 @@ def bench_batch_open(args, M):
     b_s, s_s = float(np.mean(bt)), float(np.mean(st))
     opened = k * (1 << nv)
     perms = merkle_perms(nv, lr)  # the combined codeword's tree and the folded trees
-    ach = perms * VALU_OPS_PER_PERM / b_s / 1e12
+    ach = perms * ALG_OPS_PER_PERM / b_s / 1e12
     return {"metric": METRIC, "value": opened / b_s, "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": b_s * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64 (Goldilocks, Ext2 challenges)", "data": "synthetic",
@@ -535,8 +709,9 @@ def bench_batch_open(args, M):
             "one_open_per_claim": {"ms": s_s * 1e3, "value": opened / s_s},
             "speedup_vs_one_open_per_claim": s_s / b_s,
             "roofline": {"kernel": "Monolith Merkle work of the batch opening over its wall time", "bound": "valu",
-                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS, "unit": "Tops/s",
-                         "frac": ach / VALU_PEAK_TOPS, "traffic": None, "perms": perms}}
+                         "achieved": ach, "peak": VALU_NOMINAL_TOPS, "unit": "Tops/s",
+                         "frac": ach / VALU_NOMINAL_TOPS, "traffic": None, "perms": perms,
+                         "alg_ops_per_perm": ALG_OPS_PER_PERM}}
 
 
 def cpu_baseline_batch_open(nv, polys, nq):
@@ -563,29 +738,19 @@ def cpu_baseline_batch_open(nv, polys, nq):
                       f"rate 1/2, Monolith, {nq} queries, single thread, {dt:.2f} s"}
 
 
-SERVE_DIMS = (256, 512, 1024, 2048, 4096)
-
-
-def serve_trace(count, seed=8):
-    """m, k, n of each request, uniform over SERVE_DIMS (seeded)."""
-    return np.random.default_rng(seed).choice(SERVE_DIMS, (count, 3))
-
-
 def bench_serve(args, M):
     """Matmul proving service: requests C = A B with m, k, n uniform over {256, ..., 4096},
     int8 A, B lifted into Goldilocks, C resident with them in HBM; each proof is the Basefold
     commits of A, B, C, an int8 Ajtai commit of C (256 rows), the degree-3 sumcheck over k and
     three openings. Poisson arrivals at a fixed mean rate; continuous batching on W worker
     streams with HBM-footprint admission, against the same trace proved one at a time."""
+    from morpheus_amd import workloads as W
     from morpheus_amd.ajtai import AjtaiI8, key_from_seed
-    from morpheus_amd.matmul import MatmulJob, serve
+    from morpheus_amd.matmul import MatmulJob, serve, verify_matmul
     lr, nq = 1, args.queries
-    dims = serve_trace(args.instances)
-    rng = np.random.default_rng(9)
+    dims = W.serve_trace(args.instances)
     jobs, elems, perms = [], 0, 0
-    for i, (m, k, n) in enumerate(dims):
-        a = rng.integers(-128, 128, (m, k), dtype=np.int8)
-        b = rng.integers(-128, 128, (k, n), dtype=np.int8)
+    for i, ((m, k, n), (a, b)) in enumerate(zip(dims, W.matmul_operands(dims, seed=9))):
         jobs.append(MatmulJob.from_int8(a, b, b"c5-%d" % i))
         lm, lk, ln = (int(x).bit_length() - 1 for x in (m, k, n))
         elems += int(m * k + k * n + m * n)
@@ -602,8 +767,8 @@ def bench_serve(args, M):
         js = jobs if sub is None else [jobs[i] for i in sub]
         arr = arrivals if sub is None else arrivals[sub]
         t0 = time.perf_counter()
-        _, admit, done = serve(js, arr, table, lr, 0, nq, aj, w, budget)
-        return time.perf_counter() - t0, admit, done
+        outs, admit, done = serve(js, arr, table, lr, 0, nq, aj, w, budget)
+        return time.perf_counter() - t0, admit, done, outs
 
     zero = np.zeros(len(jobs))
     # worker count autotuned on every fourth request, all arriving at once (untimed)
@@ -621,18 +786,25 @@ def bench_serve(args, M):
     rate = args.rate or 1.25 * len(jobs) / sat_seq
     arrivals = np.cumsum(np.random.default_rng(10).exponential(1e3 / rate, len(jobs)))
     arrivals -= arrivals[0]
-    res = {}
+    res, last = {}, {}
     for name, w in (("sequential", 1), ("served", best)):
         spans, lat = [], []
         for _ in range(args.steps):
-            dt, admit, done = run(w, arrivals)
+            dt, admit, done, outs = run(w, arrivals)
             spans.append(float(done.max()))
             lat += list(done - arrivals)
+        last[name] = outs
         span = float(np.mean(spans)) / 1e3
         res[name] = {"value": elems / span, "proofs_per_s": len(jobs) / span, "ms_per_trace": span * 1e3,
                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99))}
+    # verification: each served result equals the one-at-a-time result, and the host verifier
+    # accepts the largest request's proof
+    same = all(np.array_equal(a[k], b[k]) for a, b in zip(last["sequential"], last["served"]) for k in a)
+    big = int(np.argmax([int(m) * int(k) * int(n) for m, k, n in dims]))
+    jb = jobs[big]
+    host_ok = verify_matmul(last["served"][big], jb.lm, jb.lk, jb.ln, jb.label, table, lr, 0, nq)
     srv = res["served"]
-    ach = perms * VALU_OPS_PER_PERM / (srv["ms_per_trace"] / 1e3) / 1e12
+    ach = perms * ALG_OPS_PER_PERM / (srv["ms_per_trace"] / 1e3) / 1e12
     return {"metric": METRIC, "value": srv["value"], "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": srv["ms_per_trace"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64 (Goldilocks, Ext2 challenges)", "data": "synthetic",
@@ -646,31 +818,39 @@ def bench_serve(args, M):
                        "hbm_budget_gib": args.hbm_budget_gib, "autotune_s": {str(k): v for k, v in tune.items()},
                        "parallelism": "single-gpu"},
             "proofs_per_s": srv["proofs_per_s"], "p50_ms": srv["p50_ms"], "p99_ms": srv["p99_ms"],
+            "verified": bool(same and host_ok),
+            "verification": {"served_equals_one_at_a_time": bool(same), "host_verifier_accepts_largest": bool(host_ok),
+                             "largest_request": [int(x) for x in dims[big]]},
             "sequential": res["sequential"],
             "saturated": {"sequential_proofs_per_s": len(jobs) / sat_seq, "served_proofs_per_s": len(jobs) / sat_srv,
                           "speedup": sat_seq / sat_srv},
             "roofline": {"kernel": "all Monolith Merkle work of the trace over its span", "bound": "valu",
-                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
-                         "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS, "traffic": None, "perms": perms,
-                         "valu_ops_per_perm": VALU_OPS_PER_PERM}}
+                         "achieved": ach, "peak": VALU_NOMINAL_TOPS, "unit": "Tops/s", "frac": ach / VALU_NOMINAL_TOPS,
+                         "traffic": None, "perms": perms, "alg_ops_per_perm": ALG_OPS_PER_PERM}}
 
 
 def cpu_baseline_serve(lg):
-    """The oracle proving one matmul request (m = k = n = 2^lg) end to end."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle as O
+    """The oracle proving one matmul request (m = k = n = 2^lg) end to end, on every core and on
+    one."""
+    O = oracle()
+    f = cpu_facts()
     from morpheus_amd.ajtai import key_from_seed
     rng = np.random.default_rng(9)
     n = 1 << lg
     a = rng.integers(-128, 128, (n, n))
     b = rng.integers(-128, 128, (n, n))
     table = O.basefold_table(2 * lg, 1, 2 * lg, 42)
-    t0 = time.perf_counter()
-    O.matmul_prove(a, b, b"c5-cpu", table, 2 * lg, 1, 100, key_from_seed(2024), 256)
-    dt = time.perf_counter() - t0
-    return {"value": 3 * n * n / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+    out = {}
+    for name, thr in (("all", f["threads_all"]), ("single", 1)):
+        out[name], _ = timed_threads(O, thr, lambda: O.matmul_prove(a, b, b"c5-cpu", table, 2 * lg, 1, 100,
+                                                                    key_from_seed(2024), 256))
+    return {"value": 3 * n * n / out["all"], "unit": "elems/s", "cores": f["threads_all"], "kind": "port",
             "sample": f"oracle matmul proof, m = k = n = {n} (A, B, C Basefold commit + open, int8 Ajtai of C, "
-                      f"sumcheck), single thread, {dt:.2f} s ({1 / dt:.3f} proofs/s at this size)"}
+                      f"sumcheck) on {f['threads_all']} threads, {out['all']:.2f} s ({1 / out['all']:.3f} proofs/s "
+                      f"at this size)",
+            "single_thread": {"value": 3 * n * n / out["single"], "unit": "elems/s", "cores": 1,
+                              "sample": f"the same on one thread, {out['single']:.2f} s"},
+            "cpu_model": f["cpu_model"], "nproc": f["nproc"]}
 
 
 RING_KEY = bytes((7 * i + 3) % 256 for i in range(32))
@@ -718,10 +898,43 @@ def bench_ring_ajtai(args, M):
                        "parallelism": "single-gpu"},
             "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
             "roofline": {"kernel": "k_ring_dense (ChaCha8 + LDS negacyclic rows)", "bound": "valu",
-                         "achieved": ach, "peak": VALU_PEAK_TOPS, "peak_nominal": VALU_NOMINAL_TOPS,
-                         "unit": "Tops/s", "frac": ach / VALU_PEAK_TOPS,
+                         "achieved": ach, "peak": VALU_NOMINAL_TOPS, "unit": "Tops/s",
+                         "frac": ach / VALU_NOMINAL_TOPS, "achieved_from": "wave instructions of its gfx950 ISA x 64",
                          "traffic": None, "avg_launch_ms": avg, "chacha_blocks_per_s": blocks / (avg / 1e3),
                          "set_bits": set_bits}}
+
+
+SHARED_KEYS = ("metric", "n_gpus", "higher_is_better", "scaling", "vs_baseline", "data")
+
+
+def bench_all(args, M):
+    """The headline (Basefold commit + open at 2^22) and, in the same line, one record per other
+    benchmark configuration: the 2^20 product sumcheck, the int8 Ajtai commit at 2^22 x 256, the
+    64-instance batch and the matmul proving service."""
+    from morpheus_amd import workloads as W
+
+    def sub(**kw):
+        return argparse.Namespace(**{**vars(args), **kw})
+    out = bench_basefold(sub(nv=22), M)
+    c1 = bench_sumcheck(sub(nv=20, steps=max(args.steps, 20), warmup=max(args.warmup, 2)), M)
+    c3 = bench_ajtai(sub(nv=22, batch=1, steps=max(args.steps, 20), warmup=max(args.warmup, 2)), M)
+    c4 = bench_batch(sub(instances=64, steps=min(max(args.steps, 3), 5), warmup=1, workers=0), M)
+    c5 = bench_serve(sub(instances=64, steps=min(max(args.steps, 2), 3), warmup=1, workers=0, rate=0.0), M)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_basefold_all(22, 22, 1, args.queries)
+        c1["cpu_baseline"] = cpu_sumcheck_all(20)
+        c3["cpu_baseline"] = cpu_ajtai_all(22)
+        c4["cpu_baseline"] = cpu_batch_sample(W.batch_instance_data(64), 8, 1, args.queries)
+        c5["cpu_baseline"] = cpu_baseline_serve(8)
+    configs = {}
+    for name, rec in (("sumcheck_2p20", c1), ("ajtai_int8_2p22", c3), ("batched_64", c4), ("serve_matmul", c5)):
+        configs[name] = {k: v for k, v in rec.items() if k not in SHARED_KEYS}
+    out["configs"] = configs
+    out["served"] = {"proofs_per_s": c5["proofs_per_s"], "p50_ms": c5["p50_ms"], "p99_ms": c5["p99_ms"],
+                     "one_at_a_time": {k: c5["sequential"][k] for k in ("proofs_per_s", "p50_ms", "p99_ms")},
+                     "verified": c5["verified"]}
+    out["verified_all"] = bool(out["verified"] and all(r.get("verified") for r in (c1, c3, c4, c5)))
+    return out
 
 
 def main():
@@ -735,8 +948,7 @@ def main():
     if M.lib.goldilocks_cuda_init() != 0:
         print("no usable GPU", file=sys.stderr)
         sys.exit(1)
-    global VALU_PEAK_TOPS
-    VALU_PEAK_TOPS = valu_peak(M)
+    valu_peak(M)
     if args.calibrate_copy:
         n = 1 << 25
         a, b = M.DeviceBuffer(n), M.DeviceBuffer(n)
@@ -748,7 +960,9 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8, "wall_ms": dt * 1e3,
                           "valu_probe_tops": VALU_PROBE}), flush=True)
         return
-    if args.workload == "basefold":
+    if args.workload == "all":
+        out = bench_all(args, M)
+    elif args.workload == "basefold":
         out = bench_basefold(args, M)
     elif args.workload == "sumcheck":
         out = bench_sumcheck(args, M)
@@ -764,9 +978,9 @@ def main():
         out = bench_batch_open(args, M)
     else:
         out = bench_ring_ajtai(args, M)
-    if not args.no_cpu:
+    if not args.no_cpu and args.workload != "all":
         if args.workload == "basefold":
-            out["cpu_baseline"] = cpu_baseline_basefold(args.cpu_sample_nv or 18, 1, args.queries)
+            out["cpu_baseline"] = cpu_basefold_all(args.cpu_sample_nv or 22, 20, 1, args.queries)
         elif args.workload == "sumcheck":
             out["cpu_baseline"] = cpu_baseline_sumcheck(args.cpu_sample_nv or 20)
         elif args.workload == "ajtai":
@@ -781,8 +995,8 @@ def main():
             out["cpu_baseline"] = cpu_baseline_batch_open(args.cpu_sample_nv or 16, args.polys, args.queries)
         else:
             out["cpu_baseline"] = cpu_baseline_ring_ajtai(args.cpu_sample_nv or 13)
-        out["cpu_baseline"]["host"] = platform.processor() or platform.machine()
-        out["cpu_baseline"]["nproc"] = os.cpu_count()
+        out["cpu_baseline"].setdefault("cpu_model", cpu_facts()["cpu_model"])
+        out["cpu_baseline"].setdefault("nproc", os.cpu_count())
     print(json.dumps(out), flush=True)
 
 

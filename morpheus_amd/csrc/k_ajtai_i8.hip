@@ -33,6 +33,7 @@
 #include <string.h>
 
 #include <mutex>
+#include <vector>
 
 #include "chacha.h"
 #include "common.h"
@@ -475,21 +476,29 @@ MAM_API int mam_ajtai_i8_commit_noalloc(void *handle, const u64 *d_z, int batch,
 }
 
 // Copy `count` matrix entries of one row starting at column col0 to host (int8).
+// Gathers 8-entry groups g0 .. g0 + groups - 1 of one row from the tiled matrix.
+__global__ void k_aj_row(const uint8_t *__restrict__ mt, int row, int RT, uint64_t g0, uint64_t groups,
+                         u64 *__restrict__ out) {
+    const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (g < groups) out[g] = *(const u64 *)(mt + aj_offset(row, (g0 + g) * 8, RT));
+}
+
 MAM_API int mam_ajtai_i8_matrix_row(void *handle, int row, uint64_t col0, uint64_t count, int8_t *h_out) {
     AjtaiI8 *A = (AjtaiI8 *)handle;
-    if (!A || row < 0 || row >= A->rows || col0 + count > A->n) return -2;
-    if (hipStreamSynchronize(mam_stream()) != hipSuccess) return -1;
-    const int RT = A->rows / 16;
-    for (uint64_t j = col0; j < col0 + count; j += 8) {
-        u64 v;
-        if (hipMemcpy(&v, (const uint8_t *)A->mt + aj_offset(row, j & ~7ull, RT), 8, hipMemcpyDeviceToHost) !=
-            hipSuccess)
-            return -1;
-        for (uint64_t k = 0; k < 8; k++) {
-            uint64_t jj = (j & ~7ull) + k;
-            if (jj >= col0 && jj < col0 + count) h_out[jj - col0] = (int8_t)(v >> (8 * k));
-        }
-    }
+    if (!A || row < 0 || row >= A->rows || col0 + count > A->n || (count && !h_out)) return -2;
+    if (count == 0) return 0;
+    hipStream_t s = mam_stream();
+    const uint64_t g0 = col0 / 8, g1 = (col0 + count + 7) / 8, groups = g1 - g0;
+    u64 *d = nullptr;
+    if (hipMallocAsync((void **)&d, groups * sizeof(u64), s) != hipSuccess) return -1;
+    std::vector<u64> h(groups);
+    k_aj_row<<<grid_1d(groups), MAM_BLOCK, 0, s>>>((const uint8_t *)A->mt, row, A->rows / 16, g0, groups, d);
+    int rc = hipGetLastError() == hipSuccess ? 0 : -1;
+    if (!rc && hipMemcpyAsync(h.data(), d, groups * sizeof(u64), hipMemcpyDeviceToHost, s) != hipSuccess) rc = -1;
+    hipFreeAsync(d, s);
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = -1;
+    if (rc) return rc;
+    memcpy(h_out, (const uint8_t *)h.data() + (col0 - g0 * 8), count);
     return 0;
 }
 

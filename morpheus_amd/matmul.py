@@ -96,3 +96,42 @@ def serve(jobs, arrivals_ms, table, log_rate=1, hash_kind=0, num_queries=32, ajt
     admit = np.array([reqs[i].admit_ms for i in range(n)])
     done = np.array([reqs[i].done_ms for i in range(n)])
     return outs, admit, done
+
+
+def verify_matmul(o, lm, lk, ln, label, table, log_rate=1, hash_kind=0, num_queries=32):
+    """Host verifier of a matmul proof (the fields prove_matmul returns): replays the transcript
+    (roots of A, B, C, the Ajtai commitment when present, the point, the claim), checks the
+    degree-3 sumcheck over the inner index against the claim C(r) ending at A(r_l, r_i) B(r_j, r_l),
+    and checks the Basefold openings of A at (r_l, r_i), B at (r_j, r_l) and C at r, each
+    opening's evaluation being the value the sumcheck relies on."""
+    from .hostfield import e2mul
+    from .prover import Transcript, sumcheck_verify, verify_ext2
+
+    def pair(v):
+        return (int(v[0]), int(v[1]))
+    t = Transcript(label)
+    for root in o["roots"]:
+        t.append_scalars(b"matmul-root", root)
+    if "ajtai" in o:
+        t.append_scalars(b"matmul-ajtai", o["ajtai"])
+    point = np.array([t.challenge_ext2(b"matmul-point") for _ in range(ln + lm)], dtype=np.uint64)
+    if not np.array_equal(point, o["point"]):
+        return False
+    claim = np.asarray(o["claim"], dtype=np.uint64)
+    t.append_ext2(b"matmul-claim", claim)
+    fe = np.asarray(o["final_evals"], dtype=np.uint64)
+    if pair(fe[2]) != (1, 0):
+        return False
+    ok, chs = sumcheck_verify(t, claim, 3, lk, o["round_msgs"], e2mul(pair(fe[0]), pair(fe[1])))
+    if not ok or not np.array_equal(np.array(chs, dtype=np.uint64).reshape(-1, 2), o["challenges"]):
+        return False
+    r_j, r_i, r_l = point[:ln], point[ln:], np.asarray(o["challenges"], dtype=np.uint64)
+    opens = (("proof_a", lm + lk, np.concatenate([r_l, r_i]), fe[0]),
+             ("proof_b", lk + ln, np.concatenate([r_j, r_l]), fe[1]),
+             ("proof_c", lm + ln, point, claim))
+    for (name, nv, pt, ev), root in zip(opens, o["roots"]):
+        if pair(o[name][:2]) != pair(ev):
+            return False
+        if not verify_ext2(root, pt.ravel(), nv, log_rate, hash_kind, o[name], num_queries, table, t):
+            return False
+    return True

@@ -5,6 +5,7 @@
   GpuSumcheckStateExt2    goldilocks-cuda-rs/src/sumcheck_prover.rs:205-423
   BasefoldTable           goldilocks-cuda-rs/src/basefold.rs:281-360
   BasefoldCommitment      goldilocks-cuda-rs/src/basefold.rs:366-863
+  sumcheck_verify         zk-torch-3/src/sumcheck/verifier.rs:17-67 (SumcheckVerifier::verify)
   verify_ext2             goldilocks-cuda-rs/src/basefold.rs:1759-1925
   batch_open_ext2         goldilocks-cuda-rs/src/basefold.rs:1202-1525
   batch_verify_ext2       goldilocks-cuda-rs/src/basefold.rs:1931-2221
@@ -16,6 +17,7 @@ import numpy as np
 from ._lib import Transcript_t, lib
 from .device import DeviceBuffer, check
 
+P = 0xFFFFFFFF00000001
 HASH_MONOLITH = 0
 HASH_POSEIDON2 = 1
 
@@ -86,6 +88,31 @@ def sumcheck_prove(transcript, d_polys, num_polys, num_vars, base=True):
     check(lib.mam_sumcheck_prove(ctypes.byref(transcript.t), d_polys.ptr, num_polys, num_vars, 1 if base else 0,
                                  _ptr(msgs), _ptr(chs), _ptr(fin), _ptr(fe)), "mam_sumcheck_prove")
     return {"round_msgs": msgs, "challenges": chs, "final_evals": fin, "final_eval": fe}
+
+
+def sumcheck_verify(transcript, claimed_sum, num_polys, num_vars, round_msgs, final_eval):
+    """SumcheckVerifier::verify: absorbs num_var and num_poly, then per round checks
+    s(0) + s(1) against the running claim, absorbs the num_polys + 1 evaluations and draws the
+    challenge, and moves the claim to s(challenge); finally compares with final_eval.
+    round_msgs: [num_vars][num_polys + 1][2] words. Returns (accepted, challenges)."""
+    from .hostfield import e2add, interp_eval
+    transcript.append_u64(b"num_var", num_vars)
+    transcript.append_u64(b"num_poly", num_polys)
+    msgs = np.asarray(round_msgs, dtype=np.uint64).reshape(-1, num_polys + 1, 2)
+    if msgs.shape[0] != num_vars:
+        return False, []
+    cur = (int(claimed_sum[0]) % P, int(claimed_sum[1]) % P)
+    chs = []
+    for r in range(num_vars):
+        ev = [(int(x) % P, int(y) % P) for x, y in msgs[r]]
+        if e2add(ev[0], ev[1]) != cur:
+            return False, chs
+        for e in msgs[r]:
+            transcript.append_ext2(b"round_message", e)
+        ch = transcript.challenge_ext2(b"challenge")
+        chs.append(ch)
+        cur = interp_eval(ev, (int(ch[0]), int(ch[1])))
+    return cur == (int(final_eval[0]) % P, int(final_eval[1]) % P), chs
 
 
 class GpuSumcheckStateExt2:

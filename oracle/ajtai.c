@@ -53,7 +53,10 @@ void or_ajtai_i8_row(const uint8_t *key32, uint32_t row, uint64_t n, int8_t *out
 
 /* out[w * rows + i] = sum_j M_ij z_w[j] mod p for batch witnesses of n words */
 void or_ajtai_i8_commit(const uint8_t *key32, int rows, uint64_t n, const uint64_t *z, int batch, uint64_t *out) {
+#pragma omp parallel
+    {
     int8_t *m = (int8_t *)malloc(n);
+#pragma omp for schedule(dynamic)
     for (int i = 0; i < rows; i++) {
         or_ajtai_i8_row(key32, (uint32_t)i, n, m);
         for (int w = 0; w < batch; w++) {
@@ -65,4 +68,5 @@ void or_ajtai_i8_commit(const uint8_t *key32, int rows, uint64_t n, const uint64
         }
     }
     free(m);
+    }
 }

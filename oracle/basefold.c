@@ -92,6 +92,7 @@ void or_basefold_commit(const u64 *evals, int nv, int lr, int hash, u64 *codewor
     }
     for (int k = 1; k < nv; k++) {
         long hc = 1L << k;
+#pragma omp parallel for schedule(static) if (n > 16384)
         for (long idx = 0; idx < n / 2; idx++) {
             long base = (idx / hc) * 2 * hc, pos = idx % hc;
             coeffs[base + hc + pos] = gl_sub(coeffs[base + hc + pos], coeffs[base + pos]);
@@ -103,6 +104,7 @@ void or_basefold_commit(const u64 *evals, int nv, int lr, int hash, u64 *codewor
     for (long i = 0; i < cwn; i++) codeword[i] = coeffs[i >> lr];
     for (int i = 0; i < nv; i++) {
         long hc = 1L << (i + lr);
+#pragma omp parallel for schedule(static) if (cwn > 16384)
         for (long idx = 0; idx < cwn / 2; idx++) {
             long lo = (idx / hc) * 2 * hc + idx % hc, hi = lo + hc;
             u64 a = codeword[lo], b = codeword[hi];
@@ -131,14 +133,35 @@ static void oracle_coeffs(ext2 p0, ext2 p1, ext2 p2, ext2 *c) {
     c[1] = e2_sub(e2_sub(p1, p0), c[2]);
 }
 
+/* a[j] = a[2j] + ch (a[2j+1] - a[2j]) for j < half, through a temporary so that the
+ * loop can run in parallel */
+static void fold_pairs_par(ext2 *a, long half, ext2 ch) {
+    ext2 *t = (ext2 *)malloc((half > 0 ? half : 1) * sizeof(ext2));
+#pragma omp parallel for schedule(static) if (half > 16384)
+    for (long j = 0; j < half; j++) t[j] = e2_add(a[2 * j], e2_mul(ch, e2_sub(a[2 * j + 1], a[2 * j])));
+    memcpy(a, t, (half > 0 ? half : 1) * sizeof(ext2));
+    free(t);
+}
+
 static void compute_oracle(const ext2 *bh, const ext2 *eq, long size, ext2 *c) {
     ext2 two = e2_base(2), p0 = e2_zero(), p1 = e2_zero(), p2 = e2_zero();
-    for (long j = 0; j < size / 2; j++) {
-        p0 = e2_add(p0, e2_mul(eq[2 * j], bh[2 * j]));
-        p1 = e2_add(p1, e2_mul(eq[2 * j + 1], bh[2 * j + 1]));
-        ext2 ei = e2_sub(e2_mul(two, eq[2 * j + 1]), eq[2 * j]);
-        ext2 bi = e2_sub(e2_mul(two, bh[2 * j + 1]), bh[2 * j]);
-        p2 = e2_add(p2, e2_mul(ei, bi));
+#pragma omp parallel if (size > 16384)
+    {
+        ext2 q0 = e2_zero(), q1 = e2_zero(), q2 = e2_zero();
+#pragma omp for schedule(static)
+        for (long j = 0; j < size / 2; j++) {
+            q0 = e2_add(q0, e2_mul(eq[2 * j], bh[2 * j]));
+            q1 = e2_add(q1, e2_mul(eq[2 * j + 1], bh[2 * j + 1]));
+            ext2 ei = e2_sub(e2_mul(two, eq[2 * j + 1]), eq[2 * j]);
+            ext2 bi = e2_sub(e2_mul(two, bh[2 * j + 1]), bh[2 * j]);
+            q2 = e2_add(q2, e2_mul(ei, bi));
+        }
+#pragma omp critical
+        {
+            p0 = e2_add(p0, q0);
+            p1 = e2_add(p1, q1);
+            p2 = e2_add(p2, q2);
+        }
     }
     oracle_coeffs(p0, p1, p2, c);
 }
@@ -174,9 +197,11 @@ static void tree_path(const u64 *tree, long num_leaves, long leaf, u64 *out) {
     }
 }
 
-void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *root4, int nv, int lr,
-                           int hash, const u64 *point2, const u64 *table, int table_nv, or_transcript *t,
-                           int nq, u64 *proof) {
+/* tree0: the commitment's Merkle tree over the codeword (the reference's CPU opening takes it
+ * from the commitment's host cache, cpu_basefold.rs:235-260); NULL rebuilds it here. */
+void or_basefold_open_ext2_tree(const u64 *codeword, const u64 *bh_evals, const u64 *root4, const u64 *tree0,
+                                int nv, int lr, int hash, const u64 *point2, const u64 *table, int table_nv,
+                                or_transcript *t, int nq, u64 *proof) {
     long n = 1L << nv, cwn = 1L << (nv + lr);
     u64 *pc = proof;
     /* transcript preamble */
@@ -186,9 +211,17 @@ void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *
     ext2 *eq = (ext2 *)malloc(n * sizeof(ext2));
     or_eq_table_ext2(point2, nv, (u64 *)eq);
     ext2 *bh = (ext2 *)malloc(n * sizeof(ext2));
+#pragma omp parallel for schedule(static) if (n > 16384)
     for (long i = 0; i < n; i++) bh[bitrev(i, nv)] = e2_base(bh_evals[i]);
     ext2 eval = e2_zero();
-    for (long i = 0; i < n; i++) eval = e2_add(eval, e2_mul(bh[i], eq[i]));
+#pragma omp parallel if (n > 16384)
+    {
+        ext2 part = e2_zero();
+#pragma omp for schedule(static)
+        for (long i = 0; i < n; i++) part = e2_add(part, e2_mul(bh[i], eq[i]));
+#pragma omp critical
+        eval = e2_add(eval, part);
+    }
     *pc++ = eval.c0;
     *pc++ = eval.c1;
     u64 *oracles = pc;
@@ -203,10 +236,8 @@ void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *
     for (int k = 0; k < 3; k++) { oracles[2 * k] = c[k].c0; oracles[2 * k + 1] = c[k].c1; observe_ext2(t, c[k]); }
     ext2 ch = sample_ext2(t);
     long sc = n / 2;
-    for (long j = 0; j < sc; j++) {
-        bh[j] = e2_add(bh[2 * j], e2_mul(ch, e2_sub(bh[2 * j + 1], bh[2 * j])));
-        eq[j] = e2_add(eq[2 * j], e2_mul(ch, e2_sub(eq[2 * j + 1], eq[2 * j])));
-    }
+    fold_pairs_par(bh, sc, ch);
+    fold_pairs_par(eq, sc, ch);
     compute_oracle(bh, eq, sc, c);
     for (int k = 0; k < 3; k++) { oracles[6 + 2 * k] = c[k].c0; oracles[6 + 2 * k + 1] = c[k].c1; observe_ext2(t, c[k]); }
 
@@ -216,6 +247,7 @@ void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *
     long fsize = cwn / 2;
     const u64 *lvl = table + 2 * table_level_offset(table_nv, lr, 0);
     fcw[0] = (ext2 *)malloc(fsize * sizeof(ext2));
+#pragma omp parallel for schedule(static) if (fsize > 16384)
     for (long i = 0; i < fsize; i++)
         fcw[0][i] = fold_one(e2_base(codeword[2 * i]), e2_base(codeword[2 * i + 1]), lvl[2 * i], lvl[2 * i + 1], ch);
     ftree[0] = (u64 *)malloc((2 * (fsize / 2) - 1) * 4 * sizeof(u64));
@@ -226,10 +258,8 @@ void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *
     for (int round = 1; round < nv - 1; round++) {
         ch = sample_ext2(t);
         long half = sc / 2;
-        for (long j = 0; j < half; j++) {
-            eq[j] = e2_add(eq[2 * j], e2_mul(ch, e2_sub(eq[2 * j + 1], eq[2 * j])));
-            bh[j] = e2_add(bh[2 * j], e2_mul(ch, e2_sub(bh[2 * j + 1], bh[2 * j])));
-        }
+        fold_pairs_par(eq, half, ch);
+        fold_pairs_par(bh, half, ch);
         sc = half;
         compute_oracle(bh, eq, sc, c);
         for (int k = 0; k < 3; k++) {
@@ -240,6 +270,7 @@ void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *
         lvl = table + 2 * table_level_offset(table_nv, lr, round);
         long nsize = fsize / 2;
         fcw[round] = (ext2 *)malloc(nsize * sizeof(ext2));
+#pragma omp parallel for schedule(static) if (nsize > 16384)
         for (long i = 0; i < nsize; i++)
             fcw[round][i] = fold_one(fcw[round - 1][2 * i], fcw[round - 1][2 * i + 1], lvl[2 * i], lvl[2 * i + 1], ch);
         ftree[round] = (u64 *)malloc((2 * (nsize / 2) - 1) * 4 * sizeof(u64));
@@ -257,8 +288,13 @@ void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *
         final_cw[2 * i + 1] = v.c1;
     }
     /* queries */
-    u64 *itree = (u64 *)malloc((2 * (cwn / 2) - 1) * 4 * sizeof(u64));
-    or_merkle_tree_gl(codeword, cwn / 2, hash, itree);
+    u64 *own = NULL;
+    const u64 *itree = tree0;
+    if (!itree) {
+        own = (u64 *)malloc((2 * (cwn / 2) - 1) * 4 * sizeof(u64));
+        or_merkle_tree_gl(codeword, cwn / 2, hash, own);
+        itree = own;
+    }
     long *qidx = (long *)malloc((nq > 0 ? nq : 1) * sizeof(long));
     for (int q = 0; q < nq; q++) qidx[q] = (long)(or_tr_challenge_scalar(t, NULL, 0) % (u64)(cwn / 2));
     for (int q = 0; q < nq; q++) {
@@ -289,7 +325,13 @@ void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *
         pc = qp;
     }
     for (int i = 0; i < nv; i++) { free(fcw[i]); free(ftree[i]); }
-    free(fcw); free(ftree); free(itree); free(qidx); free(eq); free(bh);
+    free(fcw); free(ftree); free(own); free(qidx); free(eq); free(bh);
+}
+
+void or_basefold_open_ext2(const u64 *codeword, const u64 *bh_evals, const u64 *root4, int nv, int lr,
+                           int hash, const u64 *point2, const u64 *table, int table_nv, or_transcript *t,
+                           int nq, u64 *proof) {
+    or_basefold_open_ext2_tree(codeword, bh_evals, root4, NULL, nv, lr, hash, point2, table, table_nv, t, nq, proof);
 }
 
 static ext2 ld2(const u64 *p) { return e2(p[0], p[1]); }

@@ -13,6 +13,24 @@
 #include <string.h>
 #include "gl.h"
 #include "oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the parallel loops (OpenMP); every result is independent of the count:
+ * parallel sums add exact residues mod p, whose order does not matter. */
+void or_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#endif
+}
+int or_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 u64 or_gl_add(u64 a, u64 b) { return gl_add(a, b); }
 u64 or_gl_sub(u64 a, u64 b) { return gl_sub(a, b); }
@@ -101,6 +119,7 @@ void or_eq_table(const u64 *r, int nv, u64 *out) {
     for (int i = 0; i < nv; i++) {
         long half = 1L << i;
         u64 ri = gl_canon(r[i]), omr = gl_sub(1, ri);
+#pragma omp parallel for schedule(static) if (half > 8192)
         for (long j = half - 1; j >= 0; j--) {
             out[j | half] = gl_mul(out[j], ri);
             out[j] = gl_mul(out[j], omr);
@@ -116,6 +135,7 @@ void or_eq_table_ext2(const u64 *r2, int nv, u64 *out2) {
     for (int i = 0; i < nv; i++) {
         long half = 1L << i;
         ext2 ri = e2(r2[2 * i], r2[2 * i + 1]), omr = e2_sub(e2_one(), ri);
+#pragma omp parallel for schedule(static) if (half > 8192)
         for (long j = half - 1; j >= 0; j--) {
             o[j | half] = e2_mul(o[j], ri);
             o[j] = e2_mul(o[j], omr);
@@ -131,7 +151,13 @@ void or_partial_eval_ext2(const u64 *in2, int nv, const u64 *r2, int m, u64 *out
     for (int k = 0; k < m; k++) {
         ext2 r = e2(r2[2 * k], r2[2 * k + 1]);
         n /= 2;
-        for (long j = 0; j < n; j++) buf[j] = e2_add(buf[2 * j], e2_mul(r, e2_sub(buf[2 * j + 1], buf[2 * j])));
+        /* in place: chunks of j only read entries 2j, 2j + 1 >= j, so a sequential pass is
+         * required; a second buffer makes the fold parallel */
+        ext2 *nb = (ext2 *)malloc((n > 0 ? n : 1) * sizeof(ext2));
+#pragma omp parallel for schedule(static) if (n > 8192)
+        for (long j = 0; j < n; j++) nb[j] = e2_add(buf[2 * j], e2_mul(r, e2_sub(buf[2 * j + 1], buf[2 * j])));
+        memcpy(buf, nb, (n > 0 ? n : 1) * sizeof(ext2));
+        free(nb);
     }
     memcpy(out2, buf, n * sizeof(ext2));
     free(buf);
@@ -159,13 +185,20 @@ void or_sumcheck_prove_ext2(or_transcript *t, u64 *polys, int np, int nv, u64 *r
         ext2 *msg = (ext2 *)(round_msgs + (long)round * (np + 1) * 2);
         for (int e = 0; e <= np; e++) {
             ext2 c = e2_base((u64)e), sum = e2_zero();
-            for (long j = 0; j < half; j++) {
-                ext2 prod = e2_one();
-                for (int i = 0; i < np; i++) {
-                    ext2 a = p[i * size + 2 * j], b = p[i * size + 2 * j + 1];
-                    prod = e2_mul(prod, e2_add(a, e2_mul(c, e2_sub(b, a))));
+#pragma omp parallel if (half > 8192)
+            {
+                ext2 part = e2_zero();
+#pragma omp for schedule(static)
+                for (long j = 0; j < half; j++) {
+                    ext2 prod = e2_one();
+                    for (int i = 0; i < np; i++) {
+                        ext2 a = p[i * size + 2 * j], b = p[i * size + 2 * j + 1];
+                        prod = e2_mul(prod, e2_add(a, e2_mul(c, e2_sub(b, a))));
+                    }
+                    part = e2_add(part, prod);
                 }
-                sum = e2_add(sum, prod);
+#pragma omp critical
+                sum = e2_add(sum, part);
             }
             msg[e] = sum;
         }
@@ -175,10 +208,14 @@ void or_sumcheck_prove_ext2(or_transcript *t, u64 *polys, int np, int nv, u64 *r
         challenges[2 * round] = ch[0];
         challenges[2 * round + 1] = ch[1];
         ext2 r = e2(ch[0], ch[1]);
+        ext2 *tmp = (ext2 *)malloc((half > 0 ? half : 1) * sizeof(ext2));
         for (int i = 0; i < np; i++) {
             ext2 *q = p + (long)i * size;
-            for (long j = 0; j < half; j++) q[j] = e2_add(q[2 * j], e2_mul(r, e2_sub(q[2 * j + 1], q[2 * j])));
+#pragma omp parallel for schedule(static) if (half > 8192)
+            for (long j = 0; j < half; j++) tmp[j] = e2_add(q[2 * j], e2_mul(r, e2_sub(q[2 * j + 1], q[2 * j])));
+            memcpy(q, tmp, (half > 0 ? half : 1) * sizeof(ext2));
         }
+        free(tmp);
         cur = half;
     }
     ext2 fe = e2_one();

@@ -207,6 +207,7 @@ static void tree_upper(u64 *tree, long num_leaves, int hash) {
     long off = 0, n = num_leaves;
     while (n > 1) {
         u64 *prev = tree + off * 4, *next = tree + (off + n) * 4;
+#pragma omp parallel for schedule(static) if (n > 8192)
         for (long i = 0; i < n / 2; i++) {
             if (hash == OR_HASH_MONOLITH)
                 or_monolith_compress(prev + 8 * i, prev + 8 * i + 4, next + 4 * i);
@@ -219,6 +220,7 @@ static void tree_upper(u64 *tree, long num_leaves, int hash) {
 }
 
 void or_merkle_tree_gl(const u64 *cw, long num_leaves, int hash, u64 *tree) {
+#pragma omp parallel for schedule(static) if (num_leaves > 4096)
     for (long i = 0; i < num_leaves; i++) {
         if (hash == OR_HASH_MONOLITH) or_monolith_hash_gl_leaf(cw[2 * i], cw[2 * i + 1], tree + 4 * i);
         else or_poseidon2_hash_gl_leaf(cw[2 * i], cw[2 * i + 1], tree + 4 * i);
@@ -227,6 +229,7 @@ void or_merkle_tree_gl(const u64 *cw, long num_leaves, int hash, u64 *tree) {
 }
 
 void or_merkle_tree_ext2(const u64 *cw, long num_leaves, int hash, u64 *tree) {
+#pragma omp parallel for schedule(static) if (num_leaves > 4096)
     for (long i = 0; i < num_leaves; i++) {
         if (hash == OR_HASH_MONOLITH) or_monolith_hash_ext2_leaf(cw + 4 * i, cw + 4 * i + 2, tree + 4 * i);
         else or_poseidon2_hash_ext2_leaf(cw + 4 * i, cw + 4 * i + 2, tree + 4 * i);

@@ -103,6 +103,11 @@ void or_basefold_open_ext2(const uint64_t *codeword, const uint64_t *bh_evals, c
                            int nv, int lr, int hash, const uint64_t *point2,
                            const uint64_t *table, int table_nv, or_transcript *t, int nq,
                            uint64_t *proof);
+void or_basefold_open_ext2_tree(const uint64_t *codeword, const uint64_t *bh_evals, const uint64_t *root4,
+                                const uint64_t *tree0, int nv, int lr, int hash, const uint64_t *point2,
+                                const uint64_t *table, int table_nv, or_transcript *t, int nq, uint64_t *proof);
+void or_set_threads(int n);
+int or_get_threads(void);
 /* batch_open_ext2 / batch_verify_ext2 (basefold.rs:1202-1525, 1931-2221): num_comms
  * commitments (codewords, bh_evals, roots[4]) of the same nv, lr opened at num_points Ext2
  * points (points[num_points][nv][2]) for num_evals claims (poly, point, value[2]). */

@@ -48,6 +48,9 @@ _sigs = {
     "or_basefold_commit": (None, [u64p, ci, ci, ci, u64p, u64p, u64p]),
     "or_basefold_proof_len": (cl, [ci, ci, ci]),
     "or_basefold_open_ext2": (None, [u64p, u64p, u64p, ci, ci, ci, u64p, u64p, ci, trp, ci, u64p]),
+    "or_basefold_open_ext2_tree": (None, [u64p, u64p, u64p, u64p, ci, ci, ci, u64p, u64p, ci, trp, ci, u64p]),
+    "or_set_threads": (None, [ci]),
+    "or_get_threads": (ci, []),
     "or_basefold_verify_ext2": (ci, [u64p, u64p, ci, ci, ci, u64p, ci, u64p, ci, trp]),
     "or_basefold_batch_proof_len": (cl, [ci, ci, ci, ci]),
     "or_basefold_batch_open_ext2": (None, [ci, ctypes.POINTER(u64p), ctypes.POINTER(u64p), u64p, ci, ci, ci, ci, u64p,
@@ -191,11 +194,25 @@ def basefold_proof_len(nv, lr, nq):
     return int(L.or_basefold_proof_len(nv, lr, nq))
 
 
-def basefold_open_ext2(cw, bh, root, nv, lr, hash_kind, point2, table, table_nv, t, nq):
+def basefold_open_ext2(cw, bh, root, nv, lr, hash_kind, point2, table, table_nv, t, nq, tree=None):
+    """tree: the commitment's Merkle tree (as basefold_commit returns it); None rebuilds it."""
     proof = np.zeros(basefold_proof_len(nv, lr, nq), dtype=np.uint64)
-    L.or_basefold_open_ext2(_p(arr(cw)), _p(arr(bh)), _p(arr(root)), nv, lr, hash_kind, _p(arr(point2)),
-                            _p(arr(table)), table_nv, ctypes.byref(t.t), nq, _p(proof))
+    if tree is None:
+        L.or_basefold_open_ext2(_p(arr(cw)), _p(arr(bh)), _p(arr(root)), nv, lr, hash_kind, _p(arr(point2)),
+                                _p(arr(table)), table_nv, ctypes.byref(t.t), nq, _p(proof))
+    else:
+        L.or_basefold_open_ext2_tree(_p(arr(cw)), _p(arr(bh)), _p(arr(root)), _p(arr(tree)), nv, lr, hash_kind,
+                                     _p(arr(point2)), _p(arr(table)), table_nv, ctypes.byref(t.t), nq, _p(proof))
     return proof
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel loops (results do not depend on the count)."""
+    L.or_set_threads(int(n))
+
+
+def get_threads():
+    return int(L.or_get_threads())
 
 
 def basefold_verify_ext2(root, point2, nv, lr, hash_kind, proof, nq, table, table_nv, t):

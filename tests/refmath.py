@@ -88,43 +88,6 @@ def poseidon2_matrix_form(state):
     return s
 
 
-# ---- vectorised Goldilocks arithmetic on uint64 arrays (schoolbook 32-bit limbs) ----
-_M32 = np.uint64(0xFFFFFFFF)
-_P64 = np.uint64(P)
-
-
-def gl_mul_np(a, b):
-    """Elementwise a * b mod p of uint64 arrays (any residues in [0, 2^64)): the 128-bit
-    product from four 32 x 32 partial products, then 2^64 = 2^32 - 1 and 2^96 = -1 mod p."""
-    a = np.asarray(a, dtype=np.uint64)
-    b = np.asarray(b, dtype=np.uint64)
-    with np.errstate(over="ignore"):
-        a0, a1, b0, b1 = a & _M32, a >> np.uint64(32), b & _M32, b >> np.uint64(32)
-        ll, m1, m2, hh = a0 * b0, a0 * b1, a1 * b0, a1 * b1
-        mid = m1 + m2
-        cmid = (mid < m1).astype(np.uint64)             # a carry out of mid is worth 2^96
-        lo = ll + (mid << np.uint64(32))
-        clo = (lo < ll).astype(np.uint64)
-        hi = hh + (mid >> np.uint64(32)) + clo + (cmid << np.uint64(32))
-        hi0, hi1 = hi & _M32, hi >> np.uint64(32)
-        t = lo - hi1
-        t = np.where(lo < hi1, t - _M32, t)             # borrow: + p = - (2^32 - 1) mod 2^64
-        u = (hi0 << np.uint64(32)) - hi0                # hi0 (2^32 - 1)
-        r = t + u
-        r = np.where(r < u, r + _M32, r)                # carry: + 2^64 = + (2^32 - 1)
-        return np.where(r >= _P64, r - _P64, r)
-
-
-def gl_sum_np(a):
-    """(sum of a uint64 array) mod p, exact."""
-    a = np.asarray(a, dtype=np.uint64).ravel()
-    total = 0
-    for s in range(0, a.size, 1 << 24):
-        c = a[s:s + (1 << 24)]
-        total += int((c & _M32).sum(dtype=np.uint64)) + (int((c >> np.uint64(32)).sum(dtype=np.uint64)) << 32)
-    return total % P
-
-
-def gl_dot_np(a, b):
-    """sum_i a_i b_i mod p."""
-    return gl_sum_np(gl_mul_np(a, b))
+# vectorised Goldilocks arithmetic on uint64 arrays (the library's host-side helpers; pinned
+# against Python integers in test_oracle.py)
+from morpheus_amd.hostfield import gl_dot_np, gl_mul_np, gl_sum_np  # noqa: E402,F401

@@ -27,8 +27,10 @@ def test_ajtai_i8_matches_oracle(rows, n, batch):
     got = A.commit(z, batch)
     want = O.ajtai_i8_commit(key, rows, n, z, batch).reshape(batch, rows)
     assert np.array_equal(got, want)
-    for r in (0, rows - 1):
-        assert np.array_equal(A.matrix_row(r, 0, min(n, 256)), O.ajtai_i8_row(key, r, n)[:min(n, 256)])
+    for r in (0, rows // 2 + 1, rows - 1):
+        want_row = O.ajtai_i8_row(key, r, n)
+        assert np.array_equal(A.matrix_row(r, 0, n), want_row)
+        assert np.array_equal(A.matrix_row(r, 13, min(n - 13, 77)), want_row[13:13 + min(n - 13, 77)])
     A.free()
 
 

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import pyoracle as O
+from refmath import mle_eval as mle_eval_direct
 from refmath import (GOLDEN, P, e2add, e2inv, e2mul, e2sub, eq_direct, mle_eval, pairs, poseidon2_matrix_form,
                      rand_field)
 
@@ -307,7 +308,7 @@ def test_ajtai_i8_is_exact_matvec():
 
 
 def test_numpy_goldilocks_helpers_against_python_ints():
-    from refmath import gl_dot_np, gl_mul_np
+    from morpheus_amd.hostfield import gl_add_np, gl_dot_np, gl_mul_np, gl_sub_np, mle_eval
     rng = np.random.default_rng(99)
     edge = [0, 1, 2, P - 1, P, P + 1, (1 << 64) - 1, (1 << 32) - 1, 1 << 32, (1 << 63), P - (1 << 32)]
     a = np.concatenate([np.array(edge, dtype=np.uint64), rng.integers(0, 1 << 64, 20000, dtype=np.uint64)])
@@ -320,3 +321,88 @@ def test_numpy_goldilocks_helpers_against_python_ints():
         col = gl_mul_np(np.full(len(edge), x, dtype=np.uint64), ea)
         assert col.tolist() == [x * y % P for y in edge]
     assert gl_dot_np(a, b) == sum(x * y for x, y in zip(a.tolist(), b.tolist())) % P
+    ca, cb = a % np.uint64(P), b % np.uint64(P)
+    assert gl_add_np(ca, cb).tolist() == [(x + y) % P for x, y in zip(ca.tolist(), cb.tolist())]
+    assert gl_sub_np(ca, cb).tolist() == [(x - y) % P for x, y in zip(ca.tolist(), cb.tolist())]
+    # MLE at an Ext2 point: the direct sum over the hypercube and the oracle's partial evaluation
+    for nv in (1, 3, 6):
+        ev = rand_field(rng, 1 << nv)
+        pt = rand_field(rng, 2 * nv)
+        want = mle_eval_direct([(int(v), 0) for v in ev], [(int(pt[2 * i]), int(pt[2 * i + 1])) for i in range(nv)])
+        assert mle_eval(ev, pt) == want
+    from morpheus_amd.hostfield import e2inv as h_inv
+    from morpheus_amd.hostfield import interp_eval
+    for _ in range(20):
+        x = (int(rng.integers(0, P, dtype=np.uint64)), int(rng.integers(0, P, dtype=np.uint64)))
+        assert e2mul(h_inv(x), x) == (1, 0)
+    coef = [(int(rng.integers(0, P, dtype=np.uint64)), int(rng.integers(0, P, dtype=np.uint64))) for _ in range(4)]
+
+    def poly(x):
+        acc = (0, 0)
+        for c in reversed(coef):
+            acc = e2add(e2mul(acc, x), c)
+        return acc
+    x = (int(rng.integers(0, P, dtype=np.uint64)), int(rng.integers(0, P, dtype=np.uint64)))
+    assert interp_eval([poly((i, 0)) for i in range(4)], x) == poly(x)
+    ev = rand_field(rng, 1 << 12)
+    pt = rand_field(rng, 24)
+    lifted = np.zeros((ev.size, 2), dtype=np.uint64)
+    lifted[:, 0] = ev
+    assert list(mle_eval(ev, pt)) == [int(x) for x in O.partial_eval_ext2(lifted, 12, pt, 12).ravel()[:2]]
+
+
+def test_host_sumcheck_verifier_accepts_oracle_proofs():
+    """morpheus_amd.sumcheck_verify (the product's SumcheckVerifier) on oracle proofs: accepts
+    them, leaves the transcript where the oracle's verifier does, and rejects a changed word."""
+    import morpheus_amd as M
+    rng = np.random.default_rng(31)
+    for np_, nv in ((2, 6), (3, 4), (1, 5)):
+        vals = rand_field(rng, np_ << nv)
+        ext = np.zeros((np_, 1 << nv, 2), dtype=np.uint64)
+        ext[:, :, 0] = vals.reshape(np_, -1)
+        sc = O.sumcheck_prove_ext2(O.Transcript(b"hv"), ext, np_, nv)
+        m0 = sc["round_msgs"][0]
+        claim = [(int(m0[0][k]) + int(m0[1][k])) % P for k in range(2)]
+        t = M.Transcript(b"hv")
+        ok, chs = M.sumcheck_verify(t, claim, np_, nv, sc["round_msgs"], sc["final_eval"])
+        assert ok and np.array_equal(np.array(chs), sc["challenges"])
+        ot = O.Transcript(b"hv")
+        assert O.sumcheck_verify_ext2(ot, np.array(claim, dtype=np.uint64), np_, nv, sc["round_msgs"], sc["final_eval"])
+        assert np.array_equal(t.state(), ot.state())
+        bad = sc["round_msgs"].copy()
+        bad[nv - 1][np_][0] ^= np.uint64(1)
+        assert not M.sumcheck_verify(M.Transcript(b"hv"), claim, np_, nv, bad, sc["final_eval"])[0]
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's OpenMP loops (sums of exact residues, folds into temporaries) give the same
+    words at one thread and at eight, and the opening with the commitment's tree equals the
+    opening that rebuilds it."""
+    rng = np.random.default_rng(77)
+    nv, lr, nq = 16, 1, 12
+    ev = rand_field(rng, 1 << nv)
+    pt = rand_field(rng, 2 * nv)
+    table = O.basefold_table(nv, lr, nv, 42)
+    fg = rand_field(rng, 2 << nv)
+    ext = np.zeros((2, 1 << nv, 2), dtype=np.uint64)
+    ext[:, :, 0] = fg.reshape(2, -1)
+    key = bytes(range(32))
+    z = rand_field(rng, 1 << 13)
+    out = []
+    before = O.get_threads()
+    try:
+        for threads in (1, 8):
+            O.set_threads(threads)
+            cw, bh, tree = O.basefold_commit(ev, nv, lr, O.MONOLITH)
+            p_rebuild = O.basefold_open_ext2(cw, bh, tree[-4:], nv, lr, O.MONOLITH, pt, table, nv,
+                                             O.Transcript(b"thr"), nq)
+            p_tree = O.basefold_open_ext2(cw, bh, tree[-4:], nv, lr, O.MONOLITH, pt, table, nv,
+                                          O.Transcript(b"thr"), nq, tree=tree)
+            assert np.array_equal(p_rebuild, p_tree)
+            sc = O.sumcheck_prove_ext2(O.Transcript(b"thr-sc"), ext, 2, nv)
+            aj = O.ajtai_i8_commit(key, 64, 1 << 13, z, 1)
+            out.append((cw, tree, p_tree, sc["round_msgs"], sc["final_evals"], aj))
+    finally:
+        O.set_threads(before)
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)

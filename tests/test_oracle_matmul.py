@@ -98,3 +98,22 @@ def test_matmul_proof_composition(dims, ajtai):
     bad = pr["proof_b"].copy()
     bad[-3] ^= np.uint64(1)
     assert not verify(bad)
+
+
+def test_host_matmul_verifier_on_oracle_proofs():
+    """morpheus_amd.matmul.verify_matmul (host code of the library) accepts the oracle's matmul
+    proofs and rejects changed claim, round-message and opening words."""
+    import morpheus_amd as M
+    from morpheus_amd.matmul import verify_matmul
+    rng = np.random.default_rng(44)
+    for (lm, lk, ln), key in (((3, 4, 3), bytes(range(5, 37))), ((2, 5, 4), None)):
+        a = rng.integers(-128, 128, (1 << lm, 1 << lk))
+        b = rng.integers(-128, 128, (1 << lk, 1 << ln))
+        tnv = max(lm + lk, lk + ln, lm + ln)
+        table = M.BasefoldTable.generate(tnv, 1, tnv, 42)
+        o = O.matmul_prove(a, b, b"hv-mm", table.entries, tnv, 1, 9, key, 64)
+        assert verify_matmul(o, lm, lk, ln, b"hv-mm", table, 1, 0, 9)
+        for field, idx in (("claim", (0,)), ("round_msgs", (lk - 1, 3, 1)), ("proof_b", (40,)), ("proof_c", (0,))):
+            bad = {k: np.array(v, copy=True) for k, v in o.items()}
+            bad[field][idx] ^= np.uint64(1)
+            assert not verify_matmul(bad, lm, lk, ln, b"hv-mm", table, 1, 0, 9), field
