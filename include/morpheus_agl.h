@@ -123,6 +123,12 @@ int aext2_batched_lift_binary_ffi(const uint64_t *d_packed, uint64_t *d_polys, s
                                   int num_leaves, size_t packed_size_u64);
 int aext2_batched_lift_ternary_single_ffi(const uint64_t *d_pos, const uint64_t *d_neg, uint64_t *d_polys,
                                           size_t original_size, int num_leaves, size_t packed_size_u64);
+/* Selective add over a shared eq table (ffi.rs:283-291): for each of n_planes packed binary
+ * planes, d_partial[(bx * n_planes + p) * 2 ..] = the Ext2 sum of eq[i] over the set bits i that
+ * workgroup bx of num_blocks_x (256 threads, grid-stride) visits; callers add the partials. */
+int aext2_selective_add_batched_planes_ffi(const uint64_t *d_eq, const uint64_t *d_packed_planes,
+                                           uint64_t *d_partial, size_t total, int n_planes, size_t packed_size_u64,
+                                           int num_blocks_x);
 
 /* ring Ajtai over F_q[X]/(X^64 + 1), KAPPA = 42 (ajtai.rs:138-386); d_key: 8 u32 on the
  * device; outputs [B][42][64] canonical coefficients. */
@@ -134,6 +140,45 @@ int ajtai_commit_wide_ffi(const uint32_t *d_key, const uint64_t *d_z_wide, uint6
                           int chunk, uint64_t *d_out);                                 /* ffi.rs:595-602 */
 int ajtai_commit_sparse_ffi(const uint32_t *d_key, const uint64_t *d_positions, uint64_t K, int chunk,
                             uint64_t *d_out);                                          /* ffi.rs:649-655 */
+/* Ternary commits of the 13 split chunks z_b = pos_b - neg_b (pos, neg: [13][N_ring] bit words;
+ * out [13][42][64]); from ChaCha8, or from a matrix materialized as M[j][row][64]
+ * (ajtai_materialize_m_ffi, N * 42 * 64 words). Queued on the calling thread's stream; chunk as
+ * above (else -3). */
+int ajtai_commit_ternary_ffi(const uint32_t *d_key, const uint64_t *d_pos, const uint64_t *d_neg, uint64_t N_ring,
+                             int chunk, uint64_t *d_out);                              /* ffi.rs:603-610 */
+int ajtai_materialize_m_ffi(const uint32_t *d_chacha_key, uint64_t *d_M, uint64_t N);  /* ffi.rs:612-616 */
+int ajtai_commit_ternary_premat_ffi(const uint64_t *d_M, const uint64_t *d_pos, const uint64_t *d_neg,
+                                    uint64_t N_ring, int chunk, uint64_t *d_out);      /* ffi.rs:618-625 */
+/* Folds with challenges of 64 coefficients in {-1, 0, 1, 2}: out_j = z1_j + z2_j r (binary
+ * witnesses [N_ring], out [N_ring][64] canonical; r_coeffs host memory) and out_i = c1_i + r c2_i
+ * (commitments [42][64]). */
+int ajtai_fold_witness_ffi(const uint64_t *d_z1, const uint64_t *d_z2, const int8_t *r_coeffs, uint64_t N_ring,
+                           int chunk, uint64_t *d_out);                                /* ffi.rs:435-442 */
+int ajtai_fold_commitment_ffi(const uint64_t *d_c1, const uint64_t *d_c2, const int8_t *r_coeffs,
+                              uint64_t *d_out);                                        /* ffi.rs:642-647 */
+/* Multifolds: out_j = sum_m z_m,j r_m as int16 [N_ring][64] (binary instances [M][N_ring];
+ * the mixed forms add ternary instances pos - neg after the binary ones; challenges
+ * [(K + T)][64] on the device), and out_i = sum_m r_m c_m,i (commitments [M][42][64]). The _tc
+ * forms compute the same result. */
+int ajtai_multifold_witness_ffi(const uint64_t *d_z_packed, const int8_t *d_r_all, int16_t *d_out,
+                                int num_instances, uint64_t N_ring, uint64_t chunk_size); /* ffi.rs:444-451 */
+int ajtai_multifold_mixed_witness_ffi(const uint64_t *d_z_bin_packed, const uint64_t *d_pos_packed,
+                                      const uint64_t *d_neg_packed, const int8_t *d_r_all, int16_t *d_out,
+                                      int num_binary, int num_ternary, uint64_t N_ring,
+                                      uint64_t chunk_size);                            /* ffi.rs:453-463 */
+int ajtai_multifold_mixed_witness_tc_ffi(const uint64_t *d_z_bin_packed, const uint64_t *d_pos_packed,
+                                         const uint64_t *d_neg_packed, const int8_t *d_r_all, int16_t *d_out,
+                                         int num_binary, int num_ternary, uint64_t N_ring); /* ffi.rs:465-474 */
+int ajtai_multifold_mixed_witness_tc_fused_ffi(const uint64_t *d_z_bin_packed, const uint64_t *d_pos_packed,
+                                               const uint64_t *d_neg_packed, const int8_t *d_r_all,
+                                               int16_t *d_out, int num_binary, int num_ternary,
+                                               uint64_t N_ring);                       /* ffi.rs:476-485 */
+int ajtai_multifold_commitment_ffi(const uint64_t *d_c_packed, const int8_t *d_r_all, int num_instances,
+                                   uint64_t *d_out);                                   /* ffi.rs:635-640 */
+/* Split of a wide int16 witness [N_ring][64] into 13 ternary chunks: v = sum_i 2^i (pos_i - neg_i),
+ * pos, neg [13][N_ring] bit words with no common bit. */
+int ajtai_split_witness_ffi(const int16_t *d_z_wide, uint64_t *d_pos_chunks, uint64_t *d_neg_chunks,
+                            uint64_t N_ring);                                          /* ffi.rs:487-492 */
 /* No reference counterpart: host copy of M_{row, col} (64 coefficients) for checks. */
 int mam_ajtai_ring_matrix_entry(const uint8_t *key32, uint32_t row, uint64_t col, uint64_t *out64);
 

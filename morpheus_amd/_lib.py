@@ -215,6 +215,18 @@ _SIGS = {
     "ajtai_commit_dense_batched_at_ffi": (ci, [vp, vp, u64, ci, ci, u64, vp]),
     "ajtai_commit_wide_ffi": (ci, [vp, vp, u64, u64, ci, vp]),
     "ajtai_commit_sparse_ffi": (ci, [vp, vp, u64, ci, vp]),
+    "ajtai_commit_ternary_ffi": (ci, [vp, vp, vp, u64, ci, vp]),
+    "ajtai_materialize_m_ffi": (ci, [vp, vp, u64]),
+    "ajtai_commit_ternary_premat_ffi": (ci, [vp, vp, vp, u64, ci, vp]),
+    "ajtai_fold_witness_ffi": (ci, [vp, vp, vp, u64, ci, vp]),
+    "ajtai_fold_commitment_ffi": (ci, [vp, vp, vp, vp]),
+    "ajtai_multifold_witness_ffi": (ci, [vp, vp, vp, ci, u64, u64]),
+    "ajtai_multifold_mixed_witness_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, u64, u64]),
+    "ajtai_multifold_mixed_witness_tc_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, u64]),
+    "ajtai_multifold_mixed_witness_tc_fused_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, u64]),
+    "ajtai_multifold_commitment_ffi": (ci, [vp, vp, ci, vp]),
+    "ajtai_split_witness_ffi": (ci, [vp, vp, vp, u64]),
+    "aext2_selective_add_batched_planes_ffi": (ci, [vp, vp, vp, csz, ci, csz, ci]),
     "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
     "mam_prove_batch": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci]),
     "mam_matmul_prove": (ci, [ctypes.POINTER(MatmulInstance_t), vp, ci, ci, ci, ci, vp, ci]),

@@ -22,50 +22,8 @@
 #include "common.h"
 #include "fieldq.h"
 #include "profile.h"
+#include "ring_ajtai.h"
 #include "sync.h"
-
-static constexpr int RING_D = 64;
-static constexpr int RING_KAPPA = 42;
-static constexpr int RING_RG = 7;  // rows per group
-static constexpr int RING_GROUPS = RING_KAPPA / RING_RG;
-
-// Coefficients 8 blk .. 8 blk + 7 of M_{row, j}: the accepted 64-bit words (< q, in order) of
-// ChaCha8 blocks with counters blk, blk + 8, blk + 16, ... (16 tries; unfilled slots 0).
-__host__ __device__ __forceinline__ void ring_prg_block(const u32 key[8], u32 row, u64 j, u32 blk, u64 out[8]) {
-    const u32 nonce[3] = {row, (u32)j, (u32)(j >> 32)};
-    u32 w[16];
-    chacha::block<8>(key, blk, nonce, w);
-    bool ok = true;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        out[k] = ((u64)w[2 * k + 1] << 32) | w[2 * k];
-        ok = ok && out[k] < fq::Q;
-    }
-    if (ok) return;
-    // rare path: collect accepted words into a shift register (oldest in out[0])
-    int written = 0;
-#pragma unroll
-    for (int k = 0; k < 8; k++) out[k] = 0;
-    for (u32 retry = 0; retry < 16 && written < 8; retry++) {
-        chacha::block<8>(key, blk + 8 * retry, nonce, w);
-        for (int k = 0; k < 8 && written < 8; k++) {
-            const u64 s = ((u64)w[2 * k + 1] << 32) | w[2 * k];
-            if (s < fq::Q) {
-#pragma unroll
-                for (int t = 0; t < 7; t++) out[t] = out[t + 1];
-                out[7] = s;
-                written++;
-            }
-        }
-    }
-    // fewer than 8 accepted (never observed): move them to the front, zeros after
-    for (int pad = 8 - written; pad > 0; pad--) {
-        const u64 first = out[0];
-#pragma unroll
-        for (int t = 0; t < 7; t++) out[t] = out[t + 1];
-        out[7] = first;
-    }
-}
 
 namespace {
 
@@ -230,6 +188,99 @@ __global__ __launch_bounds__(64) void k_ring_sparse(const u32 *__restrict__ d_ke
         partial[((u64)blockIdx.x * RING_KAPPA + g * RING_RG + ii) * RING_D + r] = ring_finish(lo[ii], hi[ii]);
 }
 
+// Ternary witnesses of the split (SPLIT_K = 13 chunks): chunk b is z_b = pos_b - neg_b, bit l of
+// pos[b * N + j] / neg[b * N + j] being coefficient l of column j. A pos bit adds
+// (X^l M)[r] = L[64 + r - l], a neg bit its negation L[(r - l) & 127] (the doubled row holds
+// -M then M). The matrix column comes from ChaCha8 (m == nullptr) or from a materialized
+// M[j][row][64] (m != nullptr). grid (chunks, RING_GROUPS, chunk groups of BW);
+// partial[((chunk * 13 + b) * KAPPA + i) * 64 + r].
+static constexpr int RING_SPLIT_K = 13;
+
+template <int BW>
+__global__ __launch_bounds__(64) void k_ring_ternary(const u32 *__restrict__ d_key, const u64 *__restrict__ m,
+                                                     const u64 *__restrict__ pos, const u64 *__restrict__ neg, u64 N,
+                                                     u64 cols, u32 one, u64 *__restrict__ partial) {
+    __shared__ u64 L[RING_RG][2 * RING_D];
+    const int r = threadIdx.x, g = blockIdx.y;
+    const int b0 = blockIdx.z * BW;
+    u32 key[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (!m) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) key[k] = d_key[k];
+    }
+    u64 lo[RING_RG][BW], hi[RING_RG][BW];
+#pragma unroll
+    for (int ii = 0; ii < RING_RG; ii++)
+#pragma unroll
+        for (int w = 0; w < BW; w++) lo[ii][w] = hi[ii][w] = 0;
+    const u64 j0 = (u64)blockIdx.x * cols;
+    const u64 j1 = j0 + cols < N ? j0 + cols : N;
+    for (u64 j = j0; j < j1; j++) {
+        if (m) {
+#pragma unroll
+            for (int ii = 0; ii < RING_RG; ii++) {
+                const u64 v = m[(j * RING_KAPPA + g * RING_RG + ii) * RING_D + r];
+                L[ii][RING_D + r] = v;
+                L[ii][r] = fq::neg(v);
+            }
+        } else {
+            ring_fill(key, g, j, L);
+        }
+        mam_barrier();
+#pragma unroll
+        for (int w = 0; w < BW; w++) {
+            if (b0 + w >= RING_SPLIT_K) break;
+            u64 mp = uniform_u64(pos[(u64)(b0 + w) * N + j]);
+            while (mp) {
+                const int l = __builtin_ctzll(mp);
+                mp &= mp - 1;
+#pragma unroll
+                for (int ii = 0; ii < RING_RG; ii++) {
+                    const u64 v = L[ii][RING_D + r - l];
+                    lo[ii][w] = (u64)(u32)v * one + lo[ii][w];
+                    hi[ii][w] = (u64)(u32)(v >> 32) * one + hi[ii][w];
+                }
+            }
+            u64 mn = uniform_u64(neg[(u64)(b0 + w) * N + j]);
+            while (mn) {
+                const int l = __builtin_ctzll(mn);
+                mn &= mn - 1;
+#pragma unroll
+                for (int ii = 0; ii < RING_RG; ii++) {
+                    const u64 v = L[ii][(r - l) & (2 * RING_D - 1)];
+                    lo[ii][w] = (u64)(u32)v * one + lo[ii][w];
+                    hi[ii][w] = (u64)(u32)(v >> 32) * one + hi[ii][w];
+                }
+            }
+        }
+        mam_barrier();
+    }
+#pragma unroll
+    for (int w = 0; w < BW; w++) {
+        if (b0 + w >= RING_SPLIT_K) break;
+#pragma unroll
+        for (int ii = 0; ii < RING_RG; ii++)
+            partial[(((u64)blockIdx.x * RING_SPLIT_K + b0 + w) * RING_KAPPA + g * RING_RG + ii) * RING_D + r] =
+                ring_finish(lo[ii][w], hi[ii][w]);
+    }
+}
+
+// M[j][row][64] for columns j < N: one thread per (column, row, ChaCha8 block of 8 coefficients)
+__global__ void k_ring_materialize(const u32 *__restrict__ d_key, u64 N, u64 *__restrict__ m) {
+    const u64 t = blockIdx.x * (u64)blockDim.x + threadIdx.x;
+    if (t >= N * RING_KAPPA * 8) return;
+    const u64 j = t / (RING_KAPPA * 8);
+    const u32 rem = (u32)(t % (RING_KAPPA * 8)), row = rem >> 3, blk = rem & 7;
+    u32 key[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) key[k] = d_key[k];
+    u64 c[8];
+    ring_prg_block(key, row, j, blk, c);
+    u64 *o = m + (j * RING_KAPPA + row) * RING_D + 8 * blk;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = c[k];
+}
+
 // out[s * total + t] = sum of partial[c * total + t] over chunks c in slice s (mod q):
 // grid (ceil(total / 256), slices); consecutive threads read consecutive words.
 __global__ void k_ring_reduce(const u64 *__restrict__ partial, u64 chunks, u64 total, u64 per,
@@ -361,6 +412,57 @@ MAM_API int ajtai_commit_sparse_ffi(const u32 *d_key, const u64 *d_positions, u6
     if (!rc) rc = ring_reduce_to(partial, chunks, total, d_out, s);
     (void)hipFreeAsync(partial, s);
     return rc ? rc : mam_sync_status();
+}
+
+namespace {
+// Shared body of the two ternary commits: out[b][i][r] for the 13 chunks.
+int ring_ternary(const u32 *d_key, const u64 *d_m, const u64 *d_pos, const u64 *d_neg, u64 N, int chunk,
+                 u64 *d_out) {
+    if (!ring_chunk_ok(chunk)) return -3;
+    if (N == 0 || (!d_key && !d_m) || !d_pos || !d_neg || !d_out) return -1;
+    hipStream_t s = mam_stream();
+    constexpr int BW = 4;
+    const u64 wgroups = (RING_SPLIT_K + BW - 1) / BW;
+    const u64 cols = ring_cols(N, chunk, RING_GROUPS * wgroups);
+    const u64 chunks = (N + cols - 1) / cols;
+    const u64 total = (u64)RING_SPLIT_K * RING_KAPPA * RING_D;
+    if (chunks > 0x7FFFFFFFull) return -1;
+    u64 *partial = nullptr;
+    if (hipMallocAsync((void **)&partial, ring_partial_words(chunks, total) * sizeof(u64), s) != hipSuccess)
+        return -1;
+    k_ring_ternary<BW><<<dim3((unsigned)chunks, RING_GROUPS, (unsigned)wgroups), 64, 0, s>>>(d_key, d_m, d_pos, d_neg,
+                                                                                           N, cols, 1u, partial);
+    int rc = hipGetLastError() == hipSuccess ? 0 : -1;
+    if (!rc) rc = ring_reduce_to(partial, chunks, total, d_out, s);
+    (void)hipFreeAsync(partial, s);
+    return rc;
+}
+}  // namespace
+
+// ffi.rs:603-610 / wrapper.cu:1642-1661: 13 commitments [13][KAPPA][64] of the ternary chunks
+// z_b = pos_b - neg_b (pos, neg: [13][N_ring] bit words); chunk as for the dense commit (else -3).
+// Queued on the calling thread's stream.
+MAM_API int ajtai_commit_ternary_ffi(const u32 *d_key, const u64 *d_pos, const u64 *d_neg, u64 N_ring, int chunk,
+                                     u64 *d_out) {
+    if (!d_key) return -1;
+    return ring_ternary(d_key, nullptr, d_pos, d_neg, N_ring, chunk, d_out);
+}
+
+// ffi.rs:612-616 / wrapper.cu:1663-1671: M[j][row][64] for j < N (N * 42 * 64 words).
+MAM_API int ajtai_materialize_m_ffi(const u32 *d_chacha_key, u64 *d_M, u64 N) {
+    if (!d_chacha_key || !d_M) return -1;
+    if (N == 0) return 0;
+    const u64 threads = N * RING_KAPPA * 8;
+    k_ring_materialize<<<(unsigned)((threads + MAM_BLOCK - 1) / MAM_BLOCK), MAM_BLOCK, 0, mam_stream()>>>(d_chacha_key,
+                                                                                                  N, d_M);
+    return mam_launch_status();
+}
+
+// ffi.rs:618-625 / wrapper.cu:1682-1700: the ternary commit against a materialized matrix.
+MAM_API int ajtai_commit_ternary_premat_ffi(const u64 *d_M, const u64 *d_pos, const u64 *d_neg, u64 N_ring, int chunk,
+                                            u64 *d_out) {
+    if (!d_M) return -1;
+    return ring_ternary(nullptr, d_M, d_pos, d_neg, N_ring, chunk, d_out);
 }
 
 // Host copy of matrix entry M_{row, col} (64 coefficients), for checks.

@@ -617,6 +617,51 @@ MAM_API int aext2_batched_lift_ternary_single_ffi(const u64 *d_pos, const u64 *d
     return mam_sync_status();
 }
 
+// Selective add over a shared eq table (almost_sumcheck_prover.cuh:924-966): for each packed
+// binary plane p, partial[(bx * n_planes + p) * 2 ..] = sum of the Ext2 eq[i] over the set bits i
+// that workgroup bx visits in the grid-stride order of the reference (so every per-block partial,
+// which callers add up themselves, holds the same sum). grid (num_blocks_x, n_planes).
+__global__ __launch_bounds__(MAM_BLOCK) void k_selective_add_planes(const u64 *__restrict__ eq,
+                                                                    const u64 *__restrict__ planes,
+                                                                    u64 *__restrict__ partial, size_t total,
+                                                                    int n_planes, size_t packed) {
+    using F = FieldQ;
+    __shared__ u64 lds[2 * MAM_BLOCK / 64];
+    const int plane = blockIdx.y;
+    const u64 *bits = planes + (size_t)plane * packed;
+    F::E acc = F::e_zero();
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x)
+        if ((bits[i >> 6] >> (i & 63)) & 1) acc = F::e_add(acc, F::e_load(eq, i));
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int off = 32; off > 0; off >>= 1) {
+        const u64 o0 = ((u64)__shfl_xor((u32)(acc.c0 >> 32), off) << 32) | (u32)__shfl_xor((u32)acc.c0, off);
+        const u64 o1 = ((u64)__shfl_xor((u32)(acc.c1 >> 32), off) << 32) | (u32)__shfl_xor((u32)acc.c1, off);
+        acc = F::e_add(acc, F::e(o0, o1));
+    }
+    if (lane == 0) {
+        lds[2 * wave] = acc.c0;
+        lds[2 * wave + 1] = acc.c1;
+    }
+    mam_barrier();
+    if (threadIdx.x == 0) {
+        F::E sum = F::e_zero();
+        for (int w = 0; w < (int)blockDim.x / 64; w++) sum = F::e_add(sum, F::e(lds[2 * w], lds[2 * w + 1]));
+        F::e_store(partial, (size_t)blockIdx.x * n_planes + plane, sum);
+    }
+}
+
+// ffi.rs:283-291 / almost wrapper.cu:875-885
+MAM_API int aext2_selective_add_batched_planes_ffi(const u64 *d_eq, const u64 *d_packed_planes, u64 *d_partial,
+                                                   size_t total, int n_planes, size_t packed_size_u64,
+                                                   int num_blocks_x) {
+    if (n_planes <= 0) return 0;
+    if (num_blocks_x <= 0 || n_planes > 65535 || !d_eq || !d_packed_planes || !d_partial) return -1;
+    if (packed_size_u64 * 64 < total) return -1;
+    k_selective_add_planes<<<dim3(num_blocks_x, n_planes), MAM_BLOCK, 0, mam_stream()>>>(
+        d_eq, d_packed_planes, d_partial, total, n_planes, packed_size_u64);
+    return mam_launch_status();
+}
+
 // ffi.rs:265-272, :320-328
 MAM_API int aext2_sumcheck_batched_round_message_ffi(const u64 *d_polys, u64 *d_partial, size_t original_size,
                                                      size_t half, int num_leaves, int num_blocks_x) {

@@ -157,6 +157,16 @@ void or_ring_commit_binary(const uint32_t *key8, const uint64_t *z, uint64_t N, 
 void or_ring_commit_wide(const uint32_t *key8, const uint64_t *z_wide, uint64_t N, uint64_t col_offset,
                          uint64_t *out);
 void or_ring_commit_sparse(const uint32_t *key8, const uint64_t *pos, uint64_t K, uint64_t *out);
+void or_ring_fold_witness(const uint64_t *z1, const uint64_t *z2, const int8_t *r, uint64_t N, uint64_t *out);
+void or_ring_fold_commitment(const uint64_t *c1, const uint64_t *c2, const int8_t *r, uint64_t *out);
+void or_ring_multifold_mixed_witness(const uint64_t *bin, const uint64_t *pos, const uint64_t *neg, const int8_t *r_all,
+                                     int K, int T, uint64_t N, int16_t *out);
+void or_ring_multifold_commitment(const uint64_t *c_all, const int8_t *r_all, int M, uint64_t *out);
+void or_ring_split_witness(const int16_t *z_wide, uint64_t N, uint64_t *pos, uint64_t *neg);
+void or_ring_commit_ternary(const uint32_t *key8, const uint64_t *pos, const uint64_t *neg, uint64_t N, uint64_t *out);
+void or_ring_materialize(const uint32_t *key8, uint64_t N, uint64_t *m);
+void or_q_selective_add_planes(const uint64_t *eq2, const uint64_t *planes, uint64_t total, int n_planes,
+                               uint64_t packed, uint64_t *out2);
 /* fused permute + partial eval over modulus p or q (pinned by: identity LUTs reduce it to
  * the partial evaluation, which is pinned by direct MLE sums) */
 void or_fused_permute_peval(uint64_t mod, const uint64_t *evals, const uint64_t *eq, const uint32_t *lo,

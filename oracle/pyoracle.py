@@ -441,3 +441,82 @@ def sumcheck_proof_bincode(final_eval, round_msgs):
         for e in range(msgs.shape[1]):
             out += struct.pack("<QQ", int(msgs[r, e, 0]) % P, int(msgs[r, e, 1]) % P)
     return out
+
+
+# ---- folding, splitting and ternary commits of ring witnesses (q crate) ----
+L.or_ring_fold_witness.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+L.or_ring_fold_commitment.argtypes = [ctypes.c_void_p] * 4
+L.or_ring_multifold_mixed_witness.argtypes = [ctypes.c_void_p] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                                                       ctypes.c_void_p]
+L.or_ring_multifold_commitment.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+L.or_ring_split_witness.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p, ctypes.c_void_p]
+L.or_ring_commit_ternary.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint64, ctypes.c_void_p]
+L.or_ring_materialize.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_void_p]
+L.or_q_selective_add_planes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                        ctypes.c_uint64, ctypes.c_void_p]
+
+
+def _c(a, dtype):
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+def ring_fold_witness(z1, z2, r64):
+    z1, z2, r = _c(z1, np.uint64), _c(z2, np.uint64), _c(r64, np.int8)
+    out = np.zeros(z1.size * 64, dtype=np.uint64)
+    L.or_ring_fold_witness(z1.ctypes.data, z2.ctypes.data, r.ctypes.data, z1.size, out.ctypes.data)
+    return out
+
+
+def ring_fold_commitment(c1, c2, r64):
+    c1, c2, r = _c(c1, np.uint64), _c(c2, np.uint64), _c(r64, np.int8)
+    out = np.zeros(KAPPA * 64, dtype=np.uint64)
+    L.or_ring_fold_commitment(c1.ctypes.data, c2.ctypes.data, r.ctypes.data, out.ctypes.data)
+    return out
+
+
+def ring_multifold_mixed_witness(bin_packed, pos, neg, r_all, K, T, N):
+    b = _c(bin_packed if K else np.zeros(1), np.uint64)
+    p = _c(pos if T else np.zeros(1), np.uint64)
+    n = _c(neg if T else np.zeros(1), np.uint64)
+    r = _c(r_all, np.int8)
+    out = np.zeros(N * 64, dtype=np.int16)
+    L.or_ring_multifold_mixed_witness(b.ctypes.data, p.ctypes.data, n.ctypes.data, r.ctypes.data, K, T, N,
+                                      out.ctypes.data)
+    return out
+
+
+def ring_multifold_commitment(c_all, r_all, M):
+    c, r = _c(c_all, np.uint64), _c(r_all, np.int8)
+    out = np.zeros(KAPPA * 64, dtype=np.uint64)
+    L.or_ring_multifold_commitment(c.ctypes.data, r.ctypes.data, M, out.ctypes.data)
+    return out
+
+
+def ring_split_witness(z_wide_i16, N):
+    z = _c(z_wide_i16, np.int16)
+    pos = np.zeros(13 * N, dtype=np.uint64)
+    neg = np.zeros(13 * N, dtype=np.uint64)
+    L.or_ring_split_witness(z.ctypes.data, N, pos.ctypes.data, neg.ctypes.data)
+    return pos, neg
+
+
+def ring_commit_ternary(key32, pos, neg, N):
+    kw = ring_key_words(key32)
+    p, n = _c(pos, np.uint64), _c(neg, np.uint64)
+    out = np.zeros(13 * KAPPA * 64, dtype=np.uint64)
+    L.or_ring_commit_ternary(kw.ctypes.data, p.ctypes.data, n.ctypes.data, N, out.ctypes.data)
+    return out.reshape(13, KAPPA, 64)
+
+
+def ring_materialize(key32, N):
+    kw = ring_key_words(key32)
+    out = np.zeros(N * KAPPA * 64, dtype=np.uint64)
+    L.or_ring_materialize(kw.ctypes.data, N, out.ctypes.data)
+    return out
+
+
+def q_selective_add_planes(eq2, planes, total, n_planes, packed):
+    e, pl = _c(eq2, np.uint64), _c(planes, np.uint64)
+    out = np.zeros(2 * n_planes, dtype=np.uint64)
+    L.or_q_selective_add_planes(e.ctypes.data, pl.ctypes.data, total, n_planes, packed, out.ctypes.data)
+    return out.reshape(n_planes, 2)

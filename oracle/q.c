@@ -244,6 +244,138 @@ void or_ring_commit_sparse(const uint32_t *key8, const uint64_t *pos, uint64_t K
     }
 }
 
+/* ---- folding, splitting and ternary commits of ring witnesses
+ * (cuda_almost_goldilocks/ajtai.cuh:700-1280, ajtai_premat.cuh:35-200, restated with a generic
+ * negacyclic product instead of the kernels' rotated look-ups) ---- */
+
+/* out += a * b in Z[X]/(X^64 + 1) on signed integers */
+static void ring_mul_int_acc(const int64_t *a, const int64_t *b, int64_t *out) {
+    for (int i = 0; i < RD; i++) {
+        if (!a[i]) continue;
+        for (int k = 0; k < RD; k++) {
+            int r = i + k;
+            if (r < RD) out[r] += a[i] * b[k];
+            else out[r - RD] -= a[i] * b[k];
+        }
+    }
+}
+
+static uint64_t lift_int(int64_t v) {
+    if (v >= 0) return (uint64_t)v % QP;
+    const uint64_t m = (uint64_t)(-v) % QP;
+    return m ? QP - m : 0;
+}
+
+static void bits_to_int(uint64_t bits, int64_t *z) {
+    for (int l = 0; l < RD; l++) z[l] = (int64_t)((bits >> l) & 1);
+}
+
+/* fold_witness: out[j * 64 + k] = (z1_j + z2_j * r)[k] mod q (canonical), z1, z2 binary */
+void or_ring_fold_witness(const uint64_t *z1, const uint64_t *z2, const int8_t *r, uint64_t N, uint64_t *out) {
+    int64_t rr[RD], a[RD], acc[RD];
+    for (int l = 0; l < RD; l++) rr[l] = r[l];
+    for (uint64_t j = 0; j < N; j++) {
+        bits_to_int(z1[j], acc);
+        bits_to_int(z2[j], a);
+        ring_mul_int_acc(a, rr, acc);
+        for (int k = 0; k < RD; k++) out[j * RD + k] = lift_int(acc[k]);
+    }
+}
+
+/* fold_commitment: out = c1 + r * c2 in R^KAPPA (r's small coefficients lifted into F_q) */
+/* The commitment folds take a challenge coefficient only when it lies in [-2, 2]; others add
+ * nothing (cuda_almost_goldilocks/ajtai.cuh:808-826 and 1120-1140 branch on +-1 and +-2 only). */
+static uint64_t commit_coeff(int8_t r) {
+    return (r >= -2 && r <= 2) ? lift_int(r) : 0;
+}
+
+void or_ring_fold_commitment(const uint64_t *c1, const uint64_t *c2, const int8_t *r, uint64_t *out) {
+    uint64_t rf[RD];
+    for (int l = 0; l < RD; l++) rf[l] = commit_coeff(r[l]);
+    for (int i = 0; i < RKAPPA; i++) {
+        for (int k = 0; k < RD; k++) out[i * RD + k] = cq(c1[i * RD + k]);
+        ring_mul_acc(rf, c2 + (uint64_t)i * RD, out + (uint64_t)i * RD);
+    }
+}
+
+/* multifold_mixed_witness: out[j * 64 + k] = sum_b (bin_b,j * r_b)[k] + sum_t ((pos_t,j - neg_t,j) * r_(K+t))[k]
+ * as 16-bit integers; multifold_witness is the case T = 0 */
+void or_ring_multifold_mixed_witness(const uint64_t *bin, const uint64_t *pos, const uint64_t *neg, const int8_t *r_all,
+                                     int K, int T, uint64_t N, int16_t *out) {
+    int64_t z[RD], rr[RD], acc[RD];
+    for (uint64_t j = 0; j < N; j++) {
+        memset(acc, 0, sizeof(acc));
+        for (int m = 0; m < K + T; m++) {
+            for (int l = 0; l < RD; l++) rr[l] = r_all[m * RD + l];
+            if (m < K) {
+                bits_to_int(bin[(uint64_t)m * N + j], z);
+            } else {
+                uint64_t p = pos[(uint64_t)(m - K) * N + j], n = neg[(uint64_t)(m - K) * N + j];
+                for (int l = 0; l < RD; l++) z[l] = (int64_t)((p >> l) & 1) - (int64_t)((n >> l) & 1);
+            }
+            ring_mul_int_acc(z, rr, acc);
+        }
+        for (int k = 0; k < RD; k++) out[j * RD + k] = (int16_t)acc[k];
+    }
+}
+
+/* multifold_commitment: out = sum_m r_m * c_m in R^KAPPA */
+void or_ring_multifold_commitment(const uint64_t *c_all, const int8_t *r_all, int M, uint64_t *out) {
+    uint64_t rf[RD];
+    memset(out, 0, (size_t)RKAPPA * RD * sizeof(uint64_t));
+    for (int m = 0; m < M; m++) {
+        for (int l = 0; l < RD; l++) rf[l] = commit_coeff(r_all[m * RD + l]);
+        for (int i = 0; i < RKAPPA; i++)
+            ring_mul_acc(rf, c_all + ((uint64_t)m * RKAPPA + i) * RD, out + (uint64_t)i * RD);
+    }
+}
+
+/* split_witness (SPLIT_K = 13 chunks): v = sum_i 2^i (pos_i - neg_i) with the bits of |v| on the
+ * side of v's sign; pos[i * N + j] bit k, neg likewise */
+void or_ring_split_witness(const int16_t *z_wide, uint64_t N, uint64_t *pos, uint64_t *neg) {
+    memset(pos, 0, (size_t)13 * N * sizeof(uint64_t));
+    memset(neg, 0, (size_t)13 * N * sizeof(uint64_t));
+    for (uint64_t j = 0; j < N; j++)
+        for (int k = 0; k < RD; k++) {
+            int v = z_wide[j * RD + k], a = v < 0 ? -v : v;
+            for (int i = 0; i < 13; i++)
+                if ((a >> i) & 1) (v < 0 ? neg : pos)[(uint64_t)i * N + j] |= 1ull << k;
+        }
+}
+
+/* commit_ternary: out[b][i][r] for the 13 ternary chunks z_b = pos_b - neg_b (wide commits) */
+void or_ring_commit_ternary(const uint32_t *key8, const uint64_t *pos, const uint64_t *neg, uint64_t N, uint64_t *out) {
+    uint64_t *zw = (uint64_t *)malloc((N ? N : 1) * RD * sizeof(uint64_t));
+    for (int b = 0; b < 13; b++) {
+        for (uint64_t j = 0; j < N; j++) {
+            uint64_t p = pos[(uint64_t)b * N + j], n = neg[(uint64_t)b * N + j];
+            for (int l = 0; l < RD; l++)
+                zw[j * RD + l] = lift_int((int64_t)((p >> l) & 1) - (int64_t)((n >> l) & 1));
+        }
+        or_ring_commit_wide(key8, zw, N, 0, out + (uint64_t)b * RKAPPA * RD);
+    }
+    free(zw);
+}
+
+/* materialized matrix M[j][row][k] (ajtai_premat.cuh:41-70) */
+void or_ring_materialize(const uint32_t *key8, uint64_t N, uint64_t *m) {
+    for (uint64_t j = 0; j < N; j++)
+        for (int i = 0; i < RKAPPA; i++) or_ring_matrix_entry(key8, (uint32_t)i, j, m + (j * RKAPPA + i) * RD);
+}
+
+/* selective add over shared eq (almost_sumcheck_prover.cuh:924-966), totals per plane:
+ * out[p] = sum of the Ext2 eq[i] over set bits i of plane p */
+void or_q_selective_add_planes(const uint64_t *eq2, const uint64_t *planes, uint64_t total, int n_planes,
+                               uint64_t packed, uint64_t *out2) {
+    for (int p = 0; p < n_planes; p++) {
+        uint64_t s[2] = {0, 0};
+        for (uint64_t i = 0; i < total; i++)
+            if ((planes[(uint64_t)p * packed + (i >> 6)] >> (i & 63)) & 1) e_add(s, s, eq2 + 2 * i);
+        out2[2 * p] = s[0];
+        out2[2 * p + 1] = s[1];
+    }
+}
+
 /* fused permute + partial evaluation (cuda/fused_permute_peval.cuh:31-120) over the prime
  * `mod` (p or q): out[j] = sum_b evals[lo[x & (2^half-1)] | hi[x >> half]] * eq[b], x = j 2^m + b */
 void or_fused_permute_peval(uint64_t mod, const uint64_t *evals, const uint64_t *eq, const uint32_t *lo,

@@ -215,3 +215,123 @@ def test_fused_permute_peval_reduces_to_partial_eval():
                         dtype=np.uint64)
     got = O.fused_permute_peval(Q, f, eq, lo, hi, n, m, 3, 1 << (n - m))
     assert np.array_equal(got, O.q_partial_eval(permuted, 0, n, r, 1, m))
+
+
+# ---- folding, splitting and ternary commits (ajtai.cuh:700-1280), pinned by the identities the
+# reference's own integration tests check (almost-goldilocks-cuda-rs/tests/ajtai_integration.rs) ----
+RKEY = bytes((11 * i + 5) % 256 for i in range(32))
+
+
+def rand_challenge(rng):
+    return rng.integers(-1, 3, 64, dtype=np.int8)
+
+
+def test_fold_homomorphism_and_trivial_challenges():
+    """c1 + r c2 = M (z1 + r z2) word for word (ajtai_integration.rs:551-582); r = 0 keeps z1
+    and c1; r = 1 adds z2 and c2."""
+    rng = np.random.default_rng(61)
+    N = 3
+    z1 = rng.integers(0, 1 << 64, N, dtype=np.uint64)
+    z2 = rng.integers(0, 1 << 64, N, dtype=np.uint64)
+    r = rand_challenge(rng)
+    c1, c2 = O.ring_commit_binary(RKEY, z1, N, 1)[0], O.ring_commit_binary(RKEY, z2, N, 1)[0]
+    w = O.ring_fold_witness(z1, z2, r)
+    assert np.array_equal(O.ring_commit_wide(RKEY, w, N).ravel(), O.ring_fold_commitment(c1, c2, r))
+    zero, one = np.zeros(64, dtype=np.int8), np.zeros(64, dtype=np.int8)
+    one[0] = 1
+    bits = lambda z: np.array([(int(z[j]) >> k) & 1 for j in range(N) for k in range(64)], dtype=np.uint64)
+    assert np.array_equal(O.ring_fold_witness(z1, z2, zero), bits(z1))
+    assert np.array_equal(O.ring_fold_witness(z1, z2, one), bits(z1) + bits(z2))
+    assert np.array_equal(O.ring_fold_commitment(c1, c2, zero), c1.ravel())
+    assert [int(x) for x in O.ring_fold_commitment(c1, c2, one)] == \
+        [(int(a) + int(b)) % Q for a, b in zip(c1.ravel(), c2.ravel())]
+    # the commitment fold takes challenge coefficients in [-2, 2] only (ajtai.cuh:808-826);
+    # the witness fold takes any int8 (ajtai.cuh:738-748), so -128 on a wrap adds +128
+    big = r.copy()
+    big[5], big[9] = 3, -128
+    cut = big.copy()
+    cut[5] = cut[9] = 0
+    assert np.array_equal(O.ring_fold_commitment(c1, c2, big), O.ring_fold_commitment(c1, c2, cut))
+    e = np.zeros(64, dtype=np.int8)
+    e[0] = -128
+    top = np.array([1 << 63], dtype=np.uint64)                   # X^63 * (-128) = 128 X^63
+    one_bit = np.array([2], dtype=np.uint64)                     # X * (-128 X^63) = +128
+    e63 = np.zeros(64, dtype=np.int8)
+    e63[63] = -128
+    assert int(O.ring_fold_witness(np.zeros(1, np.uint64), top, e)[63]) == Q - 128
+    assert int(O.ring_fold_witness(np.zeros(1, np.uint64), one_bit, e63)[0]) == 128
+    # X * z: challenge X shifts negacyclically (top coefficient wraps with a sign change)
+    x = np.zeros(64, dtype=np.int8)
+    x[1] = 1
+    shifted = O.ring_fold_witness(np.zeros(N, dtype=np.uint64), z2, x).reshape(N, 64)
+    for j in range(N):
+        b = [(int(z2[j]) >> k) & 1 for k in range(64)]
+        assert [int(v) for v in shifted[j]] == [(Q - b[63]) % Q] + b[:63]
+
+
+def test_multifold_equals_repeated_folds_and_commitment_homomorphism():
+    """multifold of [z1, z2] with [1, r] is fold_witness (as integers); the multifold of
+    commitments is the commitment of the multifold witness (ajtai_integration.rs:779-822)."""
+    rng = np.random.default_rng(62)
+    N, M = 2, 5
+    z = rng.integers(0, 1 << 64, M * N, dtype=np.uint64)
+    r_all = np.concatenate([np.eye(1, 64, 0, dtype=np.int8)[0]] + [rand_challenge(rng) for _ in range(M - 1)])
+    w = O.ring_multifold_mixed_witness(z, None, None, r_all, M, 0, N)
+    f = O.ring_fold_witness(z[:N], z[N:2 * N], r_all[64:128])
+    w2 = O.ring_multifold_mixed_witness(z[:2 * N], None, None, r_all[:128], 2, 0, N)
+    assert [int(v) % Q for v in w2] == [int(v) for v in f]
+    cs = np.concatenate([O.ring_commit_binary(RKEY, z[m * N:(m + 1) * N], N, 1).ravel() for m in range(M)])
+    wf = np.array([int(v) % Q for v in w], dtype=np.uint64)
+    assert np.array_equal(O.ring_commit_wide(RKEY, wf, N).ravel(), O.ring_multifold_commitment(cs, r_all, M))
+    assert np.abs(w.astype(np.int64)).max() <= M * 128
+
+
+def test_split_reconstructs_and_ternary_commit_homomorphism():
+    """split: v = sum 2^i (pos_i - neg_i), pos & neg = 0 (ajtai_integration.rs:873-1004); a
+    pos-only ternary commit is the binary commit, and sum 2^i c_i commits to v (:1164-1233);
+    mixed multifold is binary multifold plus the signed ternary terms (:1361-1458)."""
+    rng = np.random.default_rng(63)
+    N = 2
+    v = rng.integers(-8191, 8192, N * 64).astype(np.int16)
+    v[:6] = [0, 1, -1, 8191, -8191, -4096]
+    pos, neg = O.ring_split_witness(v, N)
+    assert not np.any(pos & neg)
+    rec = np.zeros(N * 64, dtype=np.int64)
+    for i in range(13):
+        for j in range(N):
+            for k in range(64):
+                rec[j * 64 + k] += (1 << i) * (((int(pos[i * N + j]) >> k) & 1) - ((int(neg[i * N + j]) >> k) & 1))
+    assert np.array_equal(rec, v.astype(np.int64))
+    c = O.ring_commit_ternary(RKEY, pos, neg, N)
+    acc = [0] * (42 * 64)
+    for i in range(13):
+        acc = [(a + (1 << i) * int(x)) % Q for a, x in zip(acc, c[i].ravel())]
+    vf = np.array([int(x) % Q for x in v.astype(np.int64)], dtype=np.uint64)
+    assert acc == [int(x) for x in O.ring_commit_wide(RKEY, vf, N).ravel()]
+    zero = np.zeros(13 * N, dtype=np.uint64)
+    cp = O.ring_commit_ternary(RKEY, pos, zero, N)
+    assert np.array_equal(cp, O.ring_commit_binary(RKEY, pos, N, 13))
+    # mixed multifold: 2 binary instances + the 13 chunks
+    zb = rng.integers(0, 1 << 64, 2 * N, dtype=np.uint64)
+    r_all = np.concatenate([rand_challenge(rng) for _ in range(15)])
+    mixed = O.ring_multifold_mixed_witness(zb, pos, neg, r_all, 2, 13, N).astype(np.int64)
+    plus = O.ring_multifold_mixed_witness(np.concatenate([zb, pos]), None, None, r_all, 15, 0, N).astype(np.int64)
+    minus = O.ring_multifold_mixed_witness(np.concatenate([np.zeros(2 * N, dtype=np.uint64), neg]), None, None,
+                                           r_all, 15, 0, N).astype(np.int64)
+    assert np.array_equal(mixed, plus - minus)
+
+
+def test_materialized_matrix_and_selective_add():
+    m = O.ring_materialize(RKEY, 3).reshape(3, 42, 64)
+    for j, i in ((0, 0), (2, 41), (1, 17)):
+        assert np.array_equal(m[j, i], O.ring_matrix_entry(RKEY, i, j))
+    rng = np.random.default_rng(64)
+    total, n_planes = 300, 3
+    packed = (total + 63) // 64
+    eq = rng.integers(0, Q, 2 * total, dtype=np.uint64)
+    planes = rng.integers(0, 1 << 64, n_planes * packed, dtype=np.uint64)
+    got = O.q_selective_add_planes(eq, planes, total, n_planes, packed)
+    for p in range(n_planes):
+        s0 = sum(int(eq[2 * i]) for i in range(total) if (int(planes[p * packed + i // 64]) >> (i % 64)) & 1) % Q
+        s1 = sum(int(eq[2 * i + 1]) for i in range(total) if (int(planes[p * packed + i // 64]) >> (i % 64)) & 1) % Q
+        assert [int(x) for x in got[p]] == [s0, s1]
