@@ -43,6 +43,38 @@ void mam_transcript_challenge_ext2(mam_transcript *t, const uint8_t *label, long
 void mam_poseidon2_permute_host(uint64_t *state8);                                  /* poseidon2_permute, :136 */
 void mam_monolith_permute_host(uint64_t *state12);       /* goldilocks-cuda-rs/src/cpu_monolith.rs:141 */
 
+/* ---- zk-torch-4 transcript: Monolith-w12 over q = 2^64 - 2^32 - 31, rate 8, capacity 4
+ * (zk-torch-4/src/transcript.rs:21-240, crypto/monolith.rs:33-213). Plain data: a copy is a
+ * clone. append_u64s serves append_scalar(s) / append_u64(_slice) / append_digest (label, then
+ * the values reduced mod q); challenge serves challenge_scalar (n = 1), challenge_ext2 (n = 2)
+ * and challenge_vector. */
+typedef struct {
+    uint64_t state[12];
+    uint64_t in_buf[8];
+    uint64_t out_buf[8];
+    int64_t in_count;
+    int64_t out_count;
+} mam_qtranscript;
+
+void mam_qtranscript_init(mam_qtranscript *t, const uint8_t *label, long len);      /* Transcript::new, :36 */
+void mam_qtranscript_append_u64s(mam_qtranscript *t, const uint8_t *label, long len, const uint64_t *v,
+                                 long n);                                           /* append_*, :67-110 */
+void mam_qtranscript_append_ext2(mam_qtranscript *t, const uint8_t *label, long len,
+                                 const uint64_t *v2);                               /* append_ext2, :99 */
+void mam_qtranscript_challenge(mam_qtranscript *t, const uint8_t *label, long len, uint64_t *out,
+                               long n);                                             /* challenge_*, :112-130 */
+void mam_qtranscript_fork(const mam_qtranscript *t, const uint8_t *family, long len, uint64_t branch_id,
+                          mam_qtranscript *child);                                  /* fork, :166 */
+uint64_t mam_qtranscript_fingerprint(const mam_qtranscript *t);                     /* fingerprint, :133 */
+void mam_monolith_q_permute(uint64_t *state12);                                     /* monolith.rs:194 */
+/* Whole linear sumcheck over the q Ext2 (X^2 - 3) with that transcript:
+ * zk-torch-4/src/sumcheck/gpu_prover.rs:26-112 (GpuLinearSumcheckProver::new + prove /
+ * prove_ext2), the proofs of cpu_ext2_prover.rs:23-100. Arguments and outputs as
+ * mam_sumcheck_prove below; num_polys <= 8. */
+int mam_q_sumcheck_prove(mam_qtranscript *t, const uint64_t *d_polys, int num_polys, int num_vars,
+                         int input_is_base, uint64_t *round_msgs, uint64_t *challenges, uint64_t *final_evals,
+                         uint64_t *final_eval);
+
 /* ---- linear sumcheck over Ext2 ----
  * Whole proof: zk-torch-3/src/sumcheck/gpu_prover.rs:21-89 (GpuLinearSumcheckProver::new
  * + prove / prove_ext2). d_polys holds num_polys polynomials of 2^num_vars elements back

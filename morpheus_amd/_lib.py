@@ -34,6 +34,14 @@ class Transcript_t(ctypes.Structure):
 trp = ctypes.POINTER(Transcript_t)
 
 
+class QTranscript_t(ctypes.Structure):
+    _fields_ = [("state", u64 * 12), ("in_buf", u64 * 8), ("out_buf", u64 * 8),
+                ("in_count", ctypes.c_int64), ("out_count", ctypes.c_int64)]
+
+
+qtrp = ctypes.POINTER(QTranscript_t)
+
+
 class MatmulInstance_t(ctypes.Structure):
     _fields_ = [("d_a", vp), ("d_b", vp), ("d_c", vp), ("lm", ci), ("lk", ci), ("ln", ci),
                 ("label", ctypes.c_char_p), ("label_len", cl), ("roots", u64p), ("ajtai", u64p), ("point", u64p),
@@ -249,6 +257,14 @@ _SIGS = {
     "mam_profile_enable": (ci, [ci]), "mam_profile_enable_mask": (ci, [ctypes.c_uint]),
     "mam_valu_probe": (ci, [ci, ci, ci, ctypes.POINTER(ctypes.c_double)]),
     "mam_release_scratch": (ci, []),
+    "mam_qtranscript_init": (None, [qtrp, ctypes.c_char_p, cl]),
+    "mam_qtranscript_append_u64s": (None, [qtrp, ctypes.c_char_p, cl, u64p, cl]),
+    "mam_qtranscript_append_ext2": (None, [qtrp, ctypes.c_char_p, cl, u64p]),
+    "mam_qtranscript_challenge": (None, [qtrp, ctypes.c_char_p, cl, u64p, cl]),
+    "mam_qtranscript_fork": (None, [qtrp, ctypes.c_char_p, cl, u64, qtrp]),
+    "mam_qtranscript_fingerprint": (u64, [qtrp]),
+    "mam_monolith_q_permute": (None, [u64p]),
+    "mam_q_sumcheck_prove": (ci, [qtrp, vp, ci, ci, ci, u64p, u64p, u64p, u64p]),
     "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),
 }
 

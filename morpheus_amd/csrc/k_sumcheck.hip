@@ -110,10 +110,14 @@ MAM_API int sumcheck_fold_ext2_ffi(const u64 *d_input, u64 *d_output, u64 challe
 // Fused prover kernels
 // ---------------------------------------------------------------------------
 
+// The kernels below take the field policy F (FieldGL: the Goldilocks prover; FieldQ: the
+// almost-Goldilocks prover of zk-torch-4), E = F::E its Ext2.
+
 // Round-0 message straight from the input polynomials (base field or Ext2).
-template <int NP, bool BASE>
+template <class F, int NP, bool BASE>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_msg(const u64 *__restrict__ in, size_t stride, size_t half,
                                                      u64 *partials, u64 *msg) {
+    using E = typename F::E;
     __shared__ u64 lds[2 * (NP + 1) * (MAM_BLOCK / WAVE)];
     u64 acc[2 * (NP + 1)];
 #pragma unroll
@@ -124,176 +128,206 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_msg(const u64 *__restrict__ in
 #pragma unroll
             for (int i = 0; i < NP; i++) {
                 const u64 *p = in + i * stride;
-                u64 a = gl_canon(p[2 * y]), b = gl_canon(p[2 * y + 1]);
+                u64 a = F::canon(p[2 * y]), b = F::canon(p[2 * y + 1]);
                 e[i] = a;
-                df[i] = gl_sub(b, a);
+                df[i] = F::sub(b, a);
             }
 #pragma unroll
             for (int c = 0; c <= NP; c++) {
                 u64 prod = e[0];
 #pragma unroll
-                for (int i = 1; i < NP; i++) prod = gl_mul(prod, e[i]);
-                acc[2 * c] = gl_add(acc[2 * c], prod);
+                for (int i = 1; i < NP; i++) prod = F::mul(prod, e[i]);
+                acc[2 * c] = F::add(acc[2 * c], prod);
                 if (c < NP) {
 #pragma unroll
-                    for (int i = 0; i < NP; i++) e[i] = gl_add(e[i], df[i]);
+                    for (int i = 0; i < NP; i++) e[i] = F::add(e[i], df[i]);
                 }
             }
         } else {
-            ext2 e[NP], df[NP];
+            E e[NP], df[NP];
 #pragma unroll
             for (int i = 0; i < NP; i++) {
                 const u64 *p = in + i * stride;
-                ext2 a = e2_load(p, 2 * y), b = e2_load(p, 2 * y + 1);
+                E a = F::e_load(p, 2 * y), b = F::e_load(p, 2 * y + 1);
                 e[i] = a;
-                df[i] = e2_sub(b, a);
+                df[i] = F::e_sub(b, a);
             }
 #pragma unroll
             for (int c = 0; c <= NP; c++) {
-                ext2 prod = e[0];
+                E prod = e[0];
 #pragma unroll
-                for (int i = 1; i < NP; i++) prod = e2_mul(prod, e[i]);
-                acc[2 * c] = gl_add(acc[2 * c], prod.c0);
-                acc[2 * c + 1] = gl_add(acc[2 * c + 1], prod.c1);
+                for (int i = 1; i < NP; i++) prod = F::e_mul(prod, e[i]);
+                acc[2 * c] = F::add(acc[2 * c], prod.c0);
+                acc[2 * c + 1] = F::add(acc[2 * c + 1], prod.c1);
                 if (c < NP) {
 #pragma unroll
-                    for (int i = 0; i < NP; i++) e[i] = e2_add(e[i], df[i]);
+                    for (int i = 0; i < NP; i++) e[i] = F::e_add(e[i], df[i]);
                 }
             }
         }
     }
-    block_sum<2 * (NP + 1)>(acc, lds);
+    block_sum<2 * (NP + 1), F>(acc, lds);
     grid_partials<2 * (NP + 1)>(acc, partials, msg);
 }
 
 // Fold the current arrays (each 4*half_out elements) at `ch` into `out` (2*half_out
 // Ext2 elements each) and accumulate the next round's message from each output pair.
-template <int NP, bool BASE_IN>
+template <class F, int NP, bool BASE_IN>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_fold_msg(const u64 *__restrict__ in, size_t in_stride,
                                                           u64 *__restrict__ out, size_t out_stride, size_t half_out,
-                                                          ext2 ch, u64 *partials, u64 *msg) {
+                                                          typename F::E ch, u64 *partials, u64 *msg) {
+    using E = typename F::E;
     __shared__ u64 lds[2 * (NP + 1) * (MAM_BLOCK / WAVE)];
     u64 acc[2 * (NP + 1)];
 #pragma unroll
     for (int k = 0; k < 2 * (NP + 1); k++) acc[k] = 0;
     for (size_t y = blockIdx.x * (size_t)blockDim.x + threadIdx.x; y < half_out;
          y += (size_t)gridDim.x * blockDim.x) {
-        ext2 e[NP], df[NP];
+        E e[NP], df[NP];
 #pragma unroll
         for (int i = 0; i < NP; i++) {
             const u64 *p = in + i * in_stride;
-            ext2 f0, f1;
+            E f0, f1;
             if (BASE_IN) {
                 // 4 consecutive base words; ch * (b - a) costs two base products
-                const u64 a0 = gl_canon(p[4 * y]), a1 = gl_canon(p[4 * y + 1]);
-                const u64 a2 = gl_canon(p[4 * y + 2]), a3 = gl_canon(p[4 * y + 3]);
-                f0 = e2_add(e2_base(a0), e2_scale(gl_sub(a1, a0), ch));
-                f1 = e2_add(e2_base(a2), e2_scale(gl_sub(a3, a2), ch));
+                const u64 a0 = F::canon(p[4 * y]), a1 = F::canon(p[4 * y + 1]);
+                const u64 a2 = F::canon(p[4 * y + 2]), a3 = F::canon(p[4 * y + 3]);
+                f0 = F::e_add(F::e_base(a0), F::e_scale(F::sub(a1, a0), ch));
+                f1 = F::e_add(F::e_base(a2), F::e_scale(F::sub(a3, a2), ch));
             } else {
-                const ext2 a0 = e2_load(p, 4 * y), a1 = e2_load(p, 4 * y + 1);
-                const ext2 a2 = e2_load(p, 4 * y + 2), a3 = e2_load(p, 4 * y + 3);
-                f0 = e2_add(a0, e2_mul(ch, e2_sub(a1, a0)));
-                f1 = e2_add(a2, e2_mul(ch, e2_sub(a3, a2)));
+                const E a0 = F::e_load(p, 4 * y), a1 = F::e_load(p, 4 * y + 1);
+                const E a2 = F::e_load(p, 4 * y + 2), a3 = F::e_load(p, 4 * y + 3);
+                f0 = F::e_add(a0, F::e_mul(ch, F::e_sub(a1, a0)));
+                f1 = F::e_add(a2, F::e_mul(ch, F::e_sub(a3, a2)));
             }
             u64 *q = out + i * out_stride;
-            e2_store(q, 2 * y, f0);
-            e2_store(q, 2 * y + 1, f1);
+            F::e_store(q, 2 * y, f0);
+            F::e_store(q, 2 * y + 1, f1);
             e[i] = f0;
-            df[i] = e2_sub(f1, f0);
+            df[i] = F::e_sub(f1, f0);
         }
 #pragma unroll
         for (int c = 0; c <= NP; c++) {
-            ext2 prod = e[0];
+            E prod = e[0];
 #pragma unroll
-            for (int i = 1; i < NP; i++) prod = e2_mul(prod, e[i]);
-            acc[2 * c] = gl_add(acc[2 * c], prod.c0);
-            acc[2 * c + 1] = gl_add(acc[2 * c + 1], prod.c1);
+            for (int i = 1; i < NP; i++) prod = F::e_mul(prod, e[i]);
+            acc[2 * c] = F::add(acc[2 * c], prod.c0);
+            acc[2 * c + 1] = F::add(acc[2 * c + 1], prod.c1);
             if (c < NP) {
 #pragma unroll
-                for (int i = 0; i < NP; i++) e[i] = e2_add(e[i], df[i]);
+                for (int i = 0; i < NP; i++) e[i] = F::e_add(e[i], df[i]);
             }
         }
     }
-    block_sum<2 * (NP + 1)>(acc, lds);
+    block_sum<2 * (NP + 1), F>(acc, lds);
     grid_partials<2 * (NP + 1)>(acc, partials, msg);
 }
 
 // Last fold: two elements per polynomial -> one final value each, written to `finals`.
-template <bool BASE_IN>
-__global__ void k_sc_fold_final(const u64 *__restrict__ in, size_t in_stride, int np, ext2 ch, u64 *finals) {
+template <class F, bool BASE_IN>
+__global__ void k_sc_fold_final(const u64 *__restrict__ in, size_t in_stride, int np, typename F::E ch, u64 *finals) {
+    using E = typename F::E;
     int i = threadIdx.x;
     if (i >= np) return;
     const u64 *p = in + i * in_stride;
-    ext2 a = BASE_IN ? e2_base(gl_canon(p[0])) : e2_load(p, 0);
-    ext2 b = BASE_IN ? e2_base(gl_canon(p[1])) : e2_load(p, 1);
-    e2_store(finals, i, e2_add(a, e2_mul(ch, e2_sub(b, a))));
+    E a = BASE_IN ? F::e_base(F::canon(p[0])) : F::e_load(p, 0);
+    E b = BASE_IN ? F::e_base(F::canon(p[1])) : F::e_load(p, 1);
+    F::e_store(finals, i, F::e_add(a, F::e_mul(ch, F::e_sub(b, a))));
 }
 
-template <int NP>
+template <class F, int NP>
 static int launch_msg(const u64 *in, size_t stride, size_t half, bool base, const ScScratch &s, hipStream_t st) {
     unsigned g = grid_reduce(half);
     if (base)
-        k_sc_msg<NP, true><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.msg);
+        k_sc_msg<F, NP, true><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.msg);
     else
-        k_sc_msg<NP, false><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.msg);
-    launch_sum_partials<2 * (NP + 1)>(s.partials, g, s.msg, st);
+        k_sc_msg<F, NP, false><<<g, MAM_BLOCK, 0, st>>>(in, stride, half, s.partials, s.msg);
+    launch_sum_partials<2 * (NP + 1), F>(s.partials, g, s.msg, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-template <int NP>
+template <class F, int NP>
 static int launch_fold_msg(const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t half_out, bool base_in,
-                           ext2 ch, const ScScratch &s, hipStream_t st) {
+                           typename F::E ch, const ScScratch &s, hipStream_t st) {
     unsigned g = grid_reduce(half_out);
     // per output pair and polynomial: read 4 inputs (base or Ext2), write 2 Ext2
     ProfScope prof(PROF_SUMCHECK, (double)half_out * NP * ((base_in ? 32.0 : 64.0) + 32.0), st);
     if (base_in)
-        k_sc_fold_msg<NP, true><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, half_out, ch, s.partials,
-                                                         s.msg);
+        k_sc_fold_msg<F, NP, true><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, half_out, ch, s.partials,
+                                                            s.msg);
     else
-        k_sc_fold_msg<NP, false><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, half_out, ch, s.partials,
-                                                          s.msg);
-    launch_sum_partials<2 * (NP + 1)>(s.partials, g, s.msg, st);
+        k_sc_fold_msg<F, NP, false><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, half_out, ch, s.partials,
+                                                             s.msg);
+    launch_sum_partials<2 * (NP + 1), F>(s.partials, g, s.msg, st);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <class F>
+static int sc_launch_msg(int np, const u64 *in, size_t stride, size_t half, bool base, const ScScratch &s,
+                         hipStream_t st) {
+    switch (np) {
+        case 1: return launch_msg<F, 1>(in, stride, half, base, s, st);
+        case 2: return launch_msg<F, 2>(in, stride, half, base, s, st);
+        case 3: return launch_msg<F, 3>(in, stride, half, base, s, st);
+        case 4: return launch_msg<F, 4>(in, stride, half, base, s, st);
+        case 5: return launch_msg<F, 5>(in, stride, half, base, s, st);
+        case 6: return launch_msg<F, 6>(in, stride, half, base, s, st);
+        case 7: return launch_msg<F, 7>(in, stride, half, base, s, st);
+        case 8: return launch_msg<F, 8>(in, stride, half, base, s, st);
+        default: return -2;
+    }
+}
+
+template <class F>
+static int sc_launch_fold_msg(int np, const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t half_out,
+                              bool base_in, typename F::E ch, const ScScratch &s, hipStream_t st) {
+    switch (np) {
+        case 1: return launch_fold_msg<F, 1>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        case 2: return launch_fold_msg<F, 2>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        case 3: return launch_fold_msg<F, 3>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        case 4: return launch_fold_msg<F, 4>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        case 5: return launch_fold_msg<F, 5>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        case 6: return launch_fold_msg<F, 6>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        case 7: return launch_fold_msg<F, 7>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        case 8: return launch_fold_msg<F, 8>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+        default: return -2;
+    }
+}
+
+template <class F>
+static int sc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_in, typename F::E ch, u64 *finals,
+                                hipStream_t st) {
+    if (np < 1 || np > MAM_MAX_POLYS) return -2;
+    if (base_in)
+        k_sc_fold_final<F, true><<<1, 64, 0, st>>>(in, in_stride, np, ch, finals);
+    else
+        k_sc_fold_final<F, false><<<1, 64, 0, st>>>(in, in_stride, np, ch, finals);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int mam_sc_launch_msg(int np, const u64 *in, size_t stride, size_t half, bool base, const ScScratch &s,
                       hipStream_t st) {
-    switch (np) {
-        case 1: return launch_msg<1>(in, stride, half, base, s, st);
-        case 2: return launch_msg<2>(in, stride, half, base, s, st);
-        case 3: return launch_msg<3>(in, stride, half, base, s, st);
-        case 4: return launch_msg<4>(in, stride, half, base, s, st);
-        case 5: return launch_msg<5>(in, stride, half, base, s, st);
-        case 6: return launch_msg<6>(in, stride, half, base, s, st);
-        case 7: return launch_msg<7>(in, stride, half, base, s, st);
-        case 8: return launch_msg<8>(in, stride, half, base, s, st);
-        default: return -2;
-    }
+    return sc_launch_msg<FieldGL>(np, in, stride, half, base, s, st);
 }
-
 int mam_sc_launch_fold_msg(int np, const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t half_out,
                            bool base_in, ext2 ch, const ScScratch &s, hipStream_t st) {
-    switch (np) {
-        case 1: return launch_fold_msg<1>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        case 2: return launch_fold_msg<2>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        case 3: return launch_fold_msg<3>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        case 4: return launch_fold_msg<4>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        case 5: return launch_fold_msg<5>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        case 6: return launch_fold_msg<6>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        case 7: return launch_fold_msg<7>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        case 8: return launch_fold_msg<8>(in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
-        default: return -2;
-    }
+    return sc_launch_fold_msg<FieldGL>(np, in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
 }
-
 int mam_sc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_in, ext2 ch, u64 *finals,
                              hipStream_t st) {
-    if (np < 1 || np > MAM_MAX_POLYS) return -2;
-    if (base_in)
-        k_sc_fold_final<true><<<1, 64, 0, st>>>(in, in_stride, np, ch, finals);
-    else
-        k_sc_fold_final<false><<<1, 64, 0, st>>>(in, in_stride, np, ch, finals);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    return sc_launch_fold_final<FieldGL>(np, in, in_stride, base_in, ch, finals, st);
+}
+int mam_qsc_launch_msg(int np, const u64 *in, size_t stride, size_t half, bool base, const ScScratch &s,
+                       hipStream_t st) {
+    return sc_launch_msg<FieldQ>(np, in, stride, half, base, s, st);
+}
+int mam_qsc_launch_fold_msg(int np, const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t half_out,
+                            bool base_in, qe2 ch, const ScScratch &s, hipStream_t st) {
+    return sc_launch_fold_msg<FieldQ>(np, in, in_stride, out, out_stride, half_out, base_in, ch, s, st);
+}
+int mam_qsc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_in, qe2 ch, u64 *finals,
+                              hipStream_t st) {
+    return sc_launch_fold_final<FieldQ>(np, in, in_stride, base_in, ch, finals, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -1,4 +1,5 @@
-// Workgroup and grid reductions of Goldilocks sums for 64-wide wavefronts.
+// Workgroup and grid reductions of field sums for 64-wide wavefronts (field policy F,
+// Goldilocks unless named).
 //
 // block_sum: wave-level butterfly with 32-bit lane shuffles (DPP/ds_swizzle on CDNA),
 // then one LDS slot per wave.
@@ -14,7 +15,7 @@
 #pragma once
 
 #include "common.h"
-#include "field.h"
+#include "fpolicy.h"
 #include "sync.h"
 
 static constexpr int WAVE = 64;
@@ -28,12 +29,12 @@ __device__ __forceinline__ u64 shfl_xor_u64(u64 v, int mask) {
 
 // Sums M field elements over the workgroup. The result is valid in thread 0 only.
 // `lds` must hold M * (blockDim.x / 64) words.
-template <int M>
+template <int M, class F = FieldGL>
 __device__ __forceinline__ void block_sum(u64 (&v)[M], u64 *lds) {
 #pragma unroll
     for (int off = WAVE / 2; off > 0; off >>= 1) {
 #pragma unroll
-        for (int k = 0; k < M; k++) v[k] = gl_add(v[k], shfl_xor_u64(v[k], off));
+        for (int k = 0; k < M; k++) v[k] = F::add(v[k], shfl_xor_u64(v[k], off));
     }
     const int lane = threadIdx.x & (WAVE - 1);
     const int wave = threadIdx.x / WAVE;
@@ -47,7 +48,7 @@ __device__ __forceinline__ void block_sum(u64 (&v)[M], u64 *lds) {
     if (threadIdx.x == 0) {
         for (int w = 1; w < nwaves; w++) {
 #pragma unroll
-            for (int k = 0; k < M; k++) v[k] = gl_add(v[k], lds[w * M + k]);
+            for (int k = 0; k < M; k++) v[k] = F::add(v[k], lds[w * M + k]);
         }
     }
 }
@@ -115,7 +116,7 @@ __device__ __forceinline__ void grid_partials(const u64 (&v)[M], u64 *partials, 
 // Second half: result[k] = sum of partials[b * M + k] over b < nblocks (one workgroup of
 // blockDim.x >= M threads, L = blockDim.x / M of them per word, eight loads in flight each).
 // With mj.box set, the M words are also posted to that host mailbox (mj.a is not used).
-template <int M>
+template <int M, class F = FieldGL>
 __global__ __launch_bounds__(256) void k_sum_partials(const u64 *__restrict__ partials, unsigned nblocks,
                                                      u64 *__restrict__ result, MailJob mj) {
     static_assert(M <= 256, "one thread per word");
@@ -130,15 +131,15 @@ __global__ __launch_bounds__(256) void k_sum_partials(const u64 *__restrict__ pa
 #pragma unroll
             for (int u = 0; u < 8; u++) t[u] = partials[(size_t)(b + u * L) * M + k];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v = gl_add(v, t[u]);
+            for (int u = 0; u < 8; u++) v = F::add(v, t[u]);
         }
-        for (; b < nblocks; b += L) v = gl_add(v, partials[(size_t)b * M + k]);
+        for (; b < nblocks; b += L) v = F::add(v, partials[(size_t)b * M + k]);
     }
     fin[threadIdx.x] = v;
     mam_barrier();
     if ((int)threadIdx.x < M) {
         u64 sum = 0;
-        for (int j = 0; j < L; j++) sum = gl_add(sum, fin[j * M + threadIdx.x]);
+        for (int j = 0; j < L; j++) sum = F::add(sum, fin[j * M + threadIdx.x]);
         result[threadIdx.x] = sum;
         if (mj.box) mj.box[threadIdx.x] = sum;
     }
@@ -147,11 +148,11 @@ __global__ __launch_bounds__(256) void k_sum_partials(const u64 *__restrict__ pa
 
 // Host side of the second half (nothing to add for a single workgroup). mj (may be null): a
 // mailbox post of the M words made by that launch; returns whether it posted.
-template <int M>
+template <int M, class F = FieldGL>
 static inline bool launch_sum_partials(const u64 *partials, unsigned nblocks, u64 *result, hipStream_t s,
                                        const MailJob *mj = nullptr) {
     if (nblocks <= 1) return false;
     const MailJob none = {};
-    k_sum_partials<M><<<1, 256, 0, s>>>(partials, nblocks, result, mj ? *mj : none);
+    k_sum_partials<M, F><<<1, 256, 0, s>>>(partials, nblocks, result, mj ? *mj : none);
     return mj && mj->box;
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "field.h"
+#include "fieldq.h"
 
 struct MailJob;
 
@@ -23,6 +24,13 @@ int mam_sc_launch_fold_msg(int np, const u64 *in, size_t in_stride, u64 *out, si
                            bool base_in, ext2 ch, const ScScratch &s, hipStream_t st);
 int mam_sc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_in, ext2 ch, u64 *finals,
                              hipStream_t st);
+// The same three launches over the almost-Goldilocks field q and its Ext2 (X^2 - 3).
+int mam_qsc_launch_msg(int np, const u64 *in, size_t stride, size_t half, bool base, const ScScratch &s,
+                       hipStream_t st);
+int mam_qsc_launch_fold_msg(int np, const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t half_out,
+                            bool base_in, qe2 ch, const ScScratch &s, hipStream_t st);
+int mam_qsc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base_in, qe2 ch, u64 *finals,
+                              hipStream_t st);
 // Two rounds per pass (whole-proof prover, np <= MAM_LOOK_MAX_POLYS): see k_sumcheck.hip.
 #define MAM_LOOK_MAX_POLYS 3
 // mj (may be null): a post of the pass's words to that host mailbox, made by the launch that

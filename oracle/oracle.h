@@ -145,6 +145,20 @@ void or_q2_mul(const uint64_t *a, const uint64_t *b, uint64_t *out);
 void or_q2_inv(const uint64_t *a, uint64_t *out);
 void or_q_eq_table(int ext, const uint64_t *r, int nv, uint64_t *out);
 void or_q_partial_eval(const uint64_t *in, int in_ext, int nv, const uint64_t *r, int r_ext, int m, uint64_t *out);
+/* zk-torch-4 transcript (Monolith-w12 over q): the round constants of rounds 1..5 (60 words)
+ * are carried in the struct */
+typedef struct {
+    uint64_t st[12], in[8], out[8];
+    int64_t nin, nout;
+    uint64_t rc[60];
+} or_qtr;
+void or_monoq_permute(uint64_t *x, const uint64_t *rc60);
+void or_qtr_init(or_qtr *t, const uint8_t *label, long len, const uint64_t *rc60);
+void or_qtr_absorb(or_qtr *t, uint64_t v);
+void or_qtr_append(or_qtr *t, const uint8_t *label, long len, const uint64_t *v, long n);
+void or_qtr_challenge(or_qtr *t, const uint8_t *label, long len, uint64_t *out, long n);
+void or_q_sumcheck_prove(or_qtr *t, int ext, const uint64_t *polys, int np, int nv, uint64_t *msgs, uint64_t *chs,
+                         uint64_t *finals, uint64_t *fe);
 void or_q_sumcheck_msg(int ext, const uint64_t *polys, int d, uint64_t original_size, uint64_t half, uint64_t *out);
 void or_q_sumcheck_fold(int ext, const uint64_t *in, uint64_t *out, const uint64_t *ch2, int d,
                         uint64_t original_size, uint64_t half);

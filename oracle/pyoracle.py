@@ -520,3 +520,99 @@ def q_selective_add_planes(eq2, planes, total, n_planes, packed):
     out = np.zeros(2 * n_planes, dtype=np.uint64)
     L.or_q_selective_add_planes(e.ctypes.data, pl.ctypes.data, total, n_planes, packed, out.ctypes.data)
     return out.reshape(n_planes, 2)
+
+
+# ---- zk-torch-4 transcript (Monolith-w12 over q) and the q sumcheck prover ----
+import hashlib as _hashlib
+import struct as _struct
+
+
+def monoq_round_constants():
+    """[5][12] round constants of rounds 1..5 (monolith.rs:67-110): the first 8 bytes
+    (little-endian) of SHA-256("AGL-Monolith-RC-v1" || round || pos || salt), u32 LE each,
+    for the first salt giving a value below q."""
+    out = np.zeros((5, 12), dtype=np.uint64)
+    for r in range(1, 6):
+        for pos in range(12):
+            salt = 0
+            while True:
+                d = _hashlib.sha256(b"AGL-Monolith-RC-v1" + _struct.pack("<III", r, pos, salt)).digest()
+                v = int.from_bytes(d[:8], "little")
+                if v < Q:
+                    break
+                salt += 1
+            out[r - 1, pos] = v
+    return out
+
+
+MONOQ_RC = monoq_round_constants().ravel()
+
+
+class QTr_t(ctypes.Structure):
+    _fields_ = [("st", u64 * 12), ("inb", u64 * 8), ("outb", u64 * 8), ("nin", ctypes.c_int64),
+                ("nout", ctypes.c_int64), ("rc", u64 * 60)]
+
+
+_qtrp = ctypes.POINTER(QTr_t)
+for _n, (_r, _a) in {
+    "or_monoq_permute": (None, [u64p, u64p]),
+    "or_qtr_init": (None, [_qtrp, ctypes.c_char_p, cl, u64p]),
+    "or_qtr_append": (None, [_qtrp, ctypes.c_char_p, cl, u64p, cl]),
+    "or_qtr_challenge": (None, [_qtrp, ctypes.c_char_p, cl, u64p, cl]),
+    "or_q_sumcheck_prove": (None, [_qtrp, ci, u64p, ci, ci, u64p, u64p, u64p, u64p]),
+}.items():
+    _f = getattr(L, _n)
+    _f.restype = _r
+    _f.argtypes = _a
+
+
+def monoq_permute(state12):
+    s = arr(state12).copy()
+    L.or_monoq_permute(_p(s), _p(MONOQ_RC))
+    return s
+
+
+class QTranscript:
+    """zk-torch-4 Transcript (transcript.rs:21-240)."""
+
+    def __init__(self, label=b""):
+        self.t = QTr_t()
+        L.or_qtr_init(ctypes.byref(self.t), label, len(label), _p(MONOQ_RC))
+
+    def clone(self):
+        c = QTranscript.__new__(QTranscript)
+        c.t = QTr_t()
+        ctypes.pointer(c.t)[0] = self.t
+        return c
+
+    def append_u64s(self, label, v):
+        a = arr(np.atleast_1d(np.asarray(v, dtype=np.uint64)))
+        L.or_qtr_append(ctypes.byref(self.t), label, len(label), _p(a), a.size)
+
+    def append_ext2(self, label, v2):
+        self.append_u64s(label, v2)
+
+    def challenge(self, label, n):
+        o = np.zeros(n, dtype=np.uint64)
+        L.or_qtr_challenge(ctypes.byref(self.t), label, len(label), _p(o), n)
+        return o
+
+    def fork(self, family, branch_id):
+        c = self.clone()
+        c.append_u64s(family, [branch_id])
+        return c
+
+    def state(self):
+        return np.array(self.t.st, dtype=np.uint64)
+
+
+def q_sumcheck_prove(t, polys, np_, nv, ext):
+    """CpuLinearSumcheckProverExt2 (cpu_ext2_prover.rs:22-100) on a QTranscript: returns
+    (round messages [nv][np+1][2], challenges [nv][2], finals [np][2], final_eval [2])."""
+    p = arr(polys)
+    msgs = np.zeros(nv * (np_ + 1) * 2, dtype=np.uint64)
+    chs = np.zeros(2 * nv, dtype=np.uint64)
+    fin = np.zeros(2 * np_, dtype=np.uint64)
+    fe = np.zeros(2, dtype=np.uint64)
+    L.or_q_sumcheck_prove(ctypes.byref(t.t), 1 if ext else 0, _p(p), np_, nv, _p(msgs), _p(chs), _p(fin), _p(fe))
+    return msgs.reshape(nv, np_ + 1, 2), chs.reshape(nv, 2), fin.reshape(np_, 2), fe

@@ -395,3 +395,117 @@ void or_fused_permute_peval(uint64_t mod, const uint64_t *evals, const uint64_t 
     }
     (void)n;
 }
+
+/* ---- zk-torch-4 transcript: Monolith-w12 over q and the duplex sponge ----
+ * monolith.rs:33-213 (permutation; the round constants are the caller's, derived from
+ * SHA-256 in pyoracle.py as monolith.rs:67-110 does) and transcript.rs:21-240 (sponge). */
+static const uint64_t MQ_ROW0[12] = {7, 23, 8, 26, 13, 10, 9, 7, 6, 22, 21, 8};
+
+static uint8_t rotl8(uint8_t b, int k) { return (uint8_t)((b << k) | (b >> (8 - k))); }
+
+/* bar_64 byte by byte: b ^ (rotl1(~b) & rotl2(b) & rotl3(b)), then rotl1 (monolith.rs:146-153) */
+static uint64_t mq_bar(uint64_t v) {
+    uint64_t r = 0;
+    for (int k = 0; k < 8; k++) {
+        uint8_t b = (uint8_t)(v >> (8 * k));
+        uint8_t t = b ^ (rotl8((uint8_t)~b, 1) & rotl8(b, 2) & rotl8(b, 3));
+        r |= (uint64_t)rotl8(t, 1) << (8 * k);
+    }
+    return r;
+}
+
+/* y[row] = sum_col M[row][col] x[col] + rc[row], M[row][col] = ROW0[(col - row) mod 12] */
+static void mq_concrete(uint64_t *x, const uint64_t *rc) {
+    uint64_t y[12];
+    for (int row = 0; row < 12; row++) {
+        u128 acc = rc ? rc[row] : 0;
+        for (int col = 0; col < 12; col++) acc += (u128)cq(x[col]) * MQ_ROW0[(col - row + 12) % 12];
+        y[row] = (uint64_t)(acc % QP);
+    }
+    memcpy(x, y, sizeof(y));
+}
+
+void or_monoq_permute(uint64_t *x, const uint64_t *rc60) {
+    mq_concrete(x, NULL);
+    for (int r = 1; r <= 6; r++) {
+        for (int i = 0; i < 4; i++) x[i] = mq_bar(x[i]);
+        for (int i = 11; i >= 1; i--) x[i] = or_q_add(x[i], or_q_mul(x[i - 1], x[i - 1]));
+        mq_concrete(x, r <= 5 ? rc60 + 12 * (r - 1) : NULL);
+    }
+}
+
+void or_qtr_init(or_qtr *t, const uint8_t *label, long len, const uint64_t *rc60) {
+    memset(t, 0, sizeof(*t));
+    memcpy(t->rc, rc60, sizeof(t->rc));
+    for (long i = 0; i < len; i++) or_qtr_absorb(t, label[i]);
+}
+
+static void qtr_duplex(or_qtr *t) {
+    for (int i = 0; i < 8; i++) t->st[i] = i < t->nin ? t->in[i] : 0;
+    t->nin = 0;
+    or_monoq_permute(t->st, t->rc);
+    for (int i = 0; i < 8; i++) t->out[7 - i] = t->st[i];
+    t->nout = 8;
+}
+
+void or_qtr_absorb(or_qtr *t, uint64_t v) {
+    t->nout = 0;
+    t->in[t->nin++] = cq(v);
+    if (t->nin == 8) qtr_duplex(t);
+}
+
+static void qtr_label(or_qtr *t, const uint8_t *label, long len) {
+    if (len == 0) return;
+    or_qtr_absorb(t, (uint64_t)len);
+    for (long i = 0; i < len; i++) or_qtr_absorb(t, label[i]);
+}
+
+void or_qtr_append(or_qtr *t, const uint8_t *label, long len, const uint64_t *v, long n) {
+    qtr_label(t, label, len);
+    for (long i = 0; i < n; i++) or_qtr_absorb(t, v[i]);
+}
+
+void or_qtr_challenge(or_qtr *t, const uint8_t *label, long len, uint64_t *out, long n) {
+    qtr_label(t, label, len);
+    for (long i = 0; i < n; i++) {
+        if (t->nout == 0) qtr_duplex(t);
+        out[i] = t->out[--t->nout];
+    }
+}
+
+/* CpuLinearSumcheckProverExt2::new + prove (zk-torch-4/src/sumcheck/cpu_ext2_prover.rs:23-100):
+ * polys [np][2^nv] base words (ext = 0, lifted to Ext2 as GpuSumcheckStateExt2::new_from_base
+ * does) or Ext2 pairs; outputs msgs [nv][np + 1][2], chs [nv][2], finals [np][2], fe [2]. */
+void or_q_sumcheck_prove(or_qtr *t, int ext, const uint64_t *polys, int np, int nv, uint64_t *msgs, uint64_t *chs,
+                         uint64_t *finals, uint64_t *fe) {
+    const uint64_t n = 1ULL << nv;
+    uint64_t *w = (uint64_t *)malloc(sizeof(uint64_t) * 2 * n * (uint64_t)np);
+    for (int i = 0; i < np; i++)
+        for (uint64_t j = 0; j < n; j++) {
+            w[2 * (i * n + j)] = cq(ext ? polys[2 * (i * n + j)] : polys[i * n + j]);
+            w[2 * (i * n + j) + 1] = ext ? cq(polys[2 * (i * n + j) + 1]) : 0;
+        }
+    uint64_t v = (uint64_t)nv;
+    or_qtr_append(t, (const uint8_t *)"num_var", 7, &v, 1);
+    v = (uint64_t)np;
+    or_qtr_append(t, (const uint8_t *)"num_poly", 8, &v, 1);
+    uint64_t size = n;
+    for (int r = 0; r < nv; r++) {
+        uint64_t *m = msgs + (uint64_t)r * 2 * (np + 1);
+        /* arrays of `size` elements at stride n: view them as polynomials of original size n */
+        or_q_sumcheck_msg(1, w, np, n, size / 2, m);
+        for (int c = 0; c <= np; c++) or_qtr_append(t, (const uint8_t *)"round_message", 13, m + 2 * c, 2);
+        or_qtr_challenge(t, (const uint8_t *)"challenge", 9, chs + 2 * r, 2);
+        or_q_sumcheck_fold(1, w, w, chs + 2 * r, np, n, size / 2);
+        size /= 2;
+    }
+    uint64_t acc[2] = {1, 0};
+    for (int i = 0; i < np; i++) {
+        finals[2 * i] = w[2 * i * n];
+        finals[2 * i + 1] = w[2 * i * n + 1];
+        or_q2_mul(acc, finals + 2 * i, acc);
+    }
+    fe[0] = acc[0];
+    fe[1] = acc[1];
+    free(w);
+}
