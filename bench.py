@@ -586,7 +586,8 @@ def bench_batch(args, M):
     """64 sumcheck + Basefold instances of log-uniform size 2^12..2^22, proved concurrently by
     persistent worker streams (count autotuned) against one-at-a-time proving."""
     from morpheus_amd import workloads as W
-    from morpheus_amd.batch import Instance, autotune_workers, prove_batch
+    from morpheus_amd.batch import Instance, autotune_graph, autotune_workers, graph_phase_ms, prove_batch, \
+        prove_batch_graph
     lr, nq = 1, args.queries
     data = W.batch_instance_data(args.instances)
     sizes, nvs = W.batch_sizes(args.instances)
@@ -618,30 +619,59 @@ def bench_batch(args, M):
         bat_t.append(dt)
         bat_lat += [o["end_ms"] - o["start_ms"] for o in bat_out]
     seq_s, bat_s = float(np.mean(seq_t)), float(np.mean(bat_t))
-    # verification: batched proofs equal one-at-a-time proofs word for word, and the host
-    # verifiers accept the largest, the smallest and two middle instances
-    same = all(np.array_equal(a[k], b[k]) for a, b in zip(seq_out, bat_out)
-               for k in ("round_msgs", "challenges", "final_evals", "root", "proof"))
+    # graph form: settings autotuned on the whole batch (each setting captures its graph once)
+    gset, gtune = autotune_graph(insts, table, lr, 0, nq, branches=(0, 2, 4), max_wgs=(64, 256),
+                                 workers=(8, 12, 16), repeats=2)
+    g_t, g_lat, g_phase = [], [], []
+    g_out = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        g_out = prove_batch_graph(insts, table, lr, 0, nq, gset["workers"], gset["branches"], gset["max_wg"])
+        g_t.append(time.perf_counter() - t0)
+        g_lat += [o["end_ms"] - o["start_ms"] for o in g_out]
+        g_phase.append(graph_phase_ms())
+    g_s = float(np.mean(g_t))
+    # verification: batched proofs (both forms) equal one-at-a-time proofs word for word, and
+    # the host verifiers accept the largest, the smallest and two middle instances
+    keys = ("round_msgs", "challenges", "final_evals", "root", "proof")
+    same = all(np.array_equal(a[k], b[k]) for a, b in zip(seq_out, bat_out) for k in keys)
+    g_same = all(np.array_equal(a[k], b[k]) for a, b in zip(seq_out, g_out) for k in keys)
+    graph_rec = {"value": total / g_s, "ms_per_batch": g_s * 1e3, "proofs_per_s": len(insts) / g_s,
+                 "p50_ms": float(np.percentile(g_lat, 50)), "p99_ms": float(np.percentile(g_lat, 99)),
+                 "all_ready_ms": float(np.mean([p[0] for p in g_phase])),
+                 "note": "openings start per instance as its commitment and sumcheck finish inside the graph; "
+                         "all_ready_ms is when the last instance was ready",
+                 "settings": gset, "autotune_s": {f"b{k[0]}_wg{k[1]}_w{k[2]}": v for k, v in gtune.items()},
+                 "equals_one_at_a_time": bool(g_same)}
+    # the graph form is the headline only when it is clearly faster (its p99 is the batch's end)
+    form = "graph" if g_s < 0.97 * bat_s else "threads"
+    best_s = min(g_s, bat_s)
+    best_lat = g_lat if form == "graph" else bat_lat
     order = np.argsort(sizes)
     sample = sorted({int(order[-1]), int(order[0]), int(order[len(order) // 2]), int(order[len(order) // 4])})
     host_ok = all(verify_batch_instance(M, bat_out[i], data[i][0], data[i][1], data[i][2], table, lr, nq)
                   for i in sample)
-    ach = perms * ALG_OPS_PER_PERM / bat_s / 1e12
-    return {"metric": METRIC, "value": total / bat_s, "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": bat_s * 1e3, "higher_is_better": True, "scaling": "weak",
+    ach = perms * ALG_OPS_PER_PERM / best_s / 1e12
+    return {"metric": METRIC, "value": total / best_s, "unit": "elems/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": best_s * 1e3, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "u64 (Goldilocks, Ext2 challenges)", "data": "synthetic",
             "config": {"workload": "batched_sumcheck_basefold", "instances": len(insts), "total_elems": total,
                        "log2_size": "U[12,22] seeded, zero-padded to 2^ceil", "num_vars": [int(v) for v in nvs],
                        "log_rate": lr, "num_queries": nq, "merkle_hash": "monolith-w12", "workers": best,
                        "autotune_s": {str(k): v for k, v in tune.items()}, "parallelism": "single-gpu"},
-            "proofs_per_s": len(insts) / bat_s,
-            "verified": bool(same and host_ok),
-            "verification": {"batched_equals_one_at_a_time": bool(same), "host_verifiers_accept": bool(host_ok),
-                             "host_checked_instances": sample},
+            "form": form,
+            "proofs_per_s": len(insts) / best_s,
+            "verified": bool(same and g_same and host_ok),
+            "verification": {"batched_equals_one_at_a_time": bool(same), "graph_equals_one_at_a_time": bool(g_same),
+                             "host_verifiers_accept": bool(host_ok), "host_checked_instances": sample},
+            "threads": {"value": total / bat_s, "ms_per_batch": bat_s * 1e3, "proofs_per_s": len(insts) / bat_s,
+                        "p50_ms": float(np.percentile(bat_lat, 50)), "p99_ms": float(np.percentile(bat_lat, 99)),
+                        "workers": best},
+            "graph": graph_rec,
             "sequential": {"value": total / seq_s, "ms_per_batch": seq_s * 1e3, "proofs_per_s": len(insts) / seq_s,
                            "p50_ms": float(np.percentile(seq_lat, 50)), "p99_ms": float(np.percentile(seq_lat, 99))},
-            "speedup_vs_sequential": seq_s / bat_s,
-            "p50_ms": float(np.percentile(bat_lat, 50)), "p99_ms": float(np.percentile(bat_lat, 99)),
+            "speedup_vs_sequential": seq_s / best_s,
+            "p50_ms": float(np.percentile(best_lat, 50)), "p99_ms": float(np.percentile(best_lat, 99)),
             "roofline": {"kernel": "all Monolith Merkle work of the batch over its wall time", "bound": "valu",
                          "achieved": ach, "peak": VALU_NOMINAL_TOPS, "unit": "Tops/s", "frac": ach / VALU_NOMINAL_TOPS,
                          "traffic": None, "perms": perms, "alg_ops_per_perm": ALG_OPS_PER_PERM}}

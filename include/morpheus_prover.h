@@ -43,6 +43,20 @@ void mam_transcript_challenge_ext2(mam_transcript *t, const uint8_t *label, long
 void mam_poseidon2_permute_host(uint64_t *state8);                                  /* poseidon2_permute, :136 */
 void mam_monolith_permute_host(uint64_t *state12);       /* goldilocks-cuda-rs/src/cpu_monolith.rs:141 */
 
+/* ---- the zk-torch-3 transcript on the device (no reference counterpart: the reference's
+ * device challengers, challenger_*, take no labels) ----
+ * d_state: a mam_transcript (18 words) in device memory. The program (u64 words) is a list of
+ * [kind, label_len, label bytes packed little-endian into ceil(label_len / 8) words, n]:
+ * kind 0 appends the next n words of d_src (append_u64 / append_scalars / append_ext2 of
+ * transcript.rs:217-260), kind 1 draws n challenges into the next n words of d_dst
+ * (challenge_scalar / challenge_ext2, :231-253). One wavefront runs it, queued on the
+ * calling thread's stream; *d_err (optional) becomes 1 for a malformed program or an
+ * overrun of n_src / n_dst. */
+int mam_dtranscript_run(uint64_t *d_state, const uint64_t *d_prog, long prog_words, const uint64_t *d_src,
+                        long n_src, uint64_t *d_dst, long n_dst, int *d_err);
+/* n Poseidon2-w8 permutations of 8-word states in place, 8 lanes per state (synchronizes). */
+int mam_poseidon2_wide_permute(uint64_t *d_states, long n);
+
 /* ---- zk-torch-4 transcript: Monolith-w12 over q = 2^64 - 2^32 - 31, rate 8, capacity 4
  * (zk-torch-4/src/transcript.rs:21-240, crypto/monolith.rs:33-213). Plain data: a copy is a
  * clone. append_u64s serves append_scalar(s) / append_u64(_slice) / append_digest (label, then
@@ -147,6 +161,18 @@ typedef struct {
 } mam_batch_instance;
 int mam_prove_batch(mam_batch_instance *inst, int n, const uint64_t *d_table, int table_nv, int log_rate, int hash,
                     int num_queries, int workers);
+/* Graph form, the same proofs word for word: one hipGraph launch runs every instance's
+ * sumcheck with its transcript on the device (one launch pair per two rounds for the whole
+ * batch) and every commitment (on `branches` side streams, 0..4; 0: after the sumchecks),
+ * then `workers` threads open the commitments, each transcript continuing from the device.
+ * max_wg (1..4096) caps the workgroups of one instance in one pass. The graph and its
+ * buffers are kept for the next call with the same instance buffers and settings;
+ * mam_batch_graph_release frees them. num_vars + log_rate <= 30. */
+int mam_prove_batch_graph(mam_batch_instance *inst, int n, const uint64_t *d_table, int table_nv, int log_rate,
+                          int hash, int num_queries, int workers, int branches, int max_wg);
+int mam_batch_graph_release(void);
+/* Wall time (ms) of the last graph-form call: out2[0] the graph phase, out2[1] the openings. */
+int mam_batch_graph_phase_ms(double *out2);
 
 /* ---- proof wire format: bincode 1.x bytes of zk-torch-3's SumcheckProof
  * (zk-torch-3/src/sumcheck/mod.rs:16-23): final_eval (c0, c1 as u64 LE) | u64 rounds |

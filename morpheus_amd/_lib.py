@@ -237,6 +237,9 @@ _SIGS = {
     "aext2_selective_add_batched_planes_ffi": (ci, [vp, vp, vp, csz, ci, csz, ci]),
     "mam_ajtai_ring_matrix_entry": (ci, [ctypes.c_char_p, ctypes.c_uint32, u64, u64p]),
     "mam_prove_batch": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci]),
+    "mam_prove_batch_graph": (ci, [ctypes.POINTER(BatchInstance_t), ci, vp, ci, ci, ci, ci, ci, ci, ci]),
+    "mam_batch_graph_release": (ci, []),
+    "mam_batch_graph_phase_ms": (ci, [ctypes.POINTER(ctypes.c_double)]),
     "mam_matmul_prove": (ci, [ctypes.POINTER(MatmulInstance_t), vp, ci, ci, ci, ci, vp, ci]),
     "challenger_state_size": (ci, []), "challenger_alloc_states": (ci, [ctypes.POINTER(vp), ci]),
     "challenger_init_states": (ci, [vp, ci]), "challenger_observe_ffi": (ci, [vp, vp, ci]),
@@ -265,6 +268,8 @@ _SIGS = {
     "mam_qtranscript_fingerprint": (u64, [qtrp]),
     "mam_monolith_q_permute": (None, [u64p]),
     "mam_q_sumcheck_prove": (ci, [qtrp, vp, ci, ci, ci, u64p, u64p, u64p, u64p]),
+    "mam_dtranscript_run": (ci, [vp, vp, cl, vp, cl, vp, cl, vp]),
+    "mam_poseidon2_wide_permute": (ci, [vp, cl]),
     "mam_profile_collect": (ci, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(cl), ctypes.POINTER(ctypes.c_double)]),
 }
 

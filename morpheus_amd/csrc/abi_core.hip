@@ -256,7 +256,9 @@ std::vector<ProfRec> g_prof_recs;
 unsigned g_prof_mask = 0;  // bit c enables category c
 }  // namespace
 
-bool mam_prof_enabled(int cat) { return cat >= 0 && cat < 32 && ((g_prof_mask >> cat) & 1u); }
+thread_local bool t_prof_suspended = false;
+bool mam_prof_enabled(int cat) { return !t_prof_suspended && cat >= 0 && cat < 32 && ((g_prof_mask >> cat) & 1u); }
+void mam_prof_suspend(bool on) { t_prof_suspended = on; }
 
 void mam_prof_push(int cat, double work, hipEvent_t a, hipEvent_t b) {
     std::lock_guard<std::mutex> g(g_prof_mu);

@@ -41,10 +41,17 @@ MAM_HD u64 gl_sub(u64 a, u64 b) {
 MAM_HD u64 gl_neg(u64 a) { return a ? GL_P - a : 0; }
 MAM_HD u64 gl_double(u64 a) { return gl_add(a, a); }
 
+// The full 128-bit product. On the device: four 32 x 32 -> 64 multiply-adds
+// (v_mad_u64_u32) chained through the middle words, which yield both halves at once (a * b
+// and __umul64hi separately take two more 32-bit multiplies for the low half).
 MAM_HD void mul_wide(u64 a, u64 b, u64 &lo, u64 &hi) {
 #ifdef __HIP_DEVICE_COMPILE__
-    lo = a * b;
-    hi = __umul64hi(a, b);
+    const u32 a0 = (u32)a, a1 = (u32)(a >> 32), b0 = (u32)b, b1 = (u32)(b >> 32);
+    const u64 p00 = (u64)a0 * b0;
+    const u64 p01 = (u64)a0 * b1 + (p00 >> 32);  // < 2^64
+    const u64 p10 = (u64)a1 * b0 + (u32)p01;     // < 2^64
+    hi = (u64)a1 * b1 + (p01 >> 32) + (p10 >> 32);
+    lo = (p10 << 32) | (u32)p00;
 #else
     unsigned __int128 m = (unsigned __int128)a * b;
     lo = (u64)m;

@@ -15,6 +15,7 @@
 #include <stdlib.h>
 
 #include "common.h"
+#include "dtranscript.h"
 #include "profile.h"
 #include "reduce.h"
 #include "sumcheck.h"
@@ -365,21 +366,20 @@ __device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, u64 c
     return e2_add(f0, e2_mul_fixed(cb, cb7, e2_sub(f1, f0)));
 }
 
+// The pass as workgroup `bid` of `nblk` (the grid-stride loop runs over the nblk workgroups
+// given); the workgroup's W reduced words are left in acc of thread 0 after block_sum.
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
-__global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ in, size_t in_stride,
-                                                      u64 *__restrict__ out, size_t out_stride, size_t items,
-                                                      ext2 ca, ext2 cb, u64 *partials, u64 *res) {
+__device__ __forceinline__ void sc_pass_body(const u64 *__restrict__ in, size_t in_stride, u64 *__restrict__ out,
+                                             size_t out_stride, size_t items, ext2 ca, ext2 cb, unsigned bid,
+                                             unsigned nblk, u64 (&acc)[PassWords<NP, LOOK>::value], u64 *lds) {
     constexpr int G = LOOK ? 4 : 2;
     constexpr int W = PassWords<NP, LOOK>::value;
-    __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
-    u64 acc[W];
 #pragma unroll
     for (int k = 0; k < W; k++) acc[k] = 0;
     const u64 ca7 = gl_mul7(ca.c1), cb7 = gl_mul7(cb.c1);
     if constexpr (BASE_IN && FOLDS == 0) {
         // unfolded base inputs: every product is a base-field product (the c1 words stay 0)
-        for (size_t z = blockIdx.x * (size_t)blockDim.x + threadIdx.x; z < items;
-             z += (size_t)gridDim.x * blockDim.x) {
+        for (size_t z = bid * (size_t)blockDim.x + threadIdx.x; z < items; z += (size_t)nblk * blockDim.x) {
             u64 e[NP][G];
 #pragma unroll
             for (int i = 0; i < NP; i++) {
@@ -444,7 +444,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ i
             }
         }
     } else
-    for (size_t z = blockIdx.x * (size_t)blockDim.x + threadIdx.x; z < items; z += (size_t)gridDim.x * blockDim.x) {
+    for (size_t z = bid * (size_t)blockDim.x + threadIdx.x; z < items; z += (size_t)nblk * blockDim.x) {
         ext2 e[NP][G];
 #pragma unroll
         for (int i = 0; i < NP; i++) {
@@ -518,6 +518,17 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ i
         }
     }
     block_sum<W>(acc, lds);
+}
+
+template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
+__global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ in, size_t in_stride,
+                                                      u64 *__restrict__ out, size_t out_stride, size_t items,
+                                                      ext2 ca, ext2 cb, u64 *partials, u64 *res) {
+    constexpr int W = PassWords<NP, LOOK>::value;
+    __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
+    u64 acc[W];
+    sc_pass_body<NP, BASE_IN, FOLDS, LOOK>(in, in_stride, out, out_stride, items, ca, cb, blockIdx.x, gridDim.x, acc,
+                                           lds);
     grid_partials<W>(acc, partials, res);
 }
 
@@ -758,3 +769,155 @@ int mam_sc_launch_pass_final(int np, const u64 *in, size_t in_stride, bool base_
     else k_sc_pass_final<false, 1><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
+
+// ---------------------------------------------------------------------------
+// Batched product sumcheck (two polynomials) with the transcript on the device.
+//
+// k_bsc_pass: workgroup b of the launch finds its job (instance) and runs the same pass body
+// as k_sc_pass over the job's workgroups, with the pending challenges read from the
+// instance's slot; it leaves its reduced words in partials[b]. k_bsc_round: one wavefront per
+// job adds the job's partials, then plays the prover's transcript steps of the pass
+// (gpu_prover.rs:21-89: each value of the round message appended under "round_message", the
+// challenge drawn under "challenge"; for a two-round pass the second message is evaluated
+// from the first challenge, as the whole-proof host prover does), stores the messages and
+// challenges, refreshes the challenge slot and, after the instance's last pass, the final
+// values. No host round trip anywhere, so the batch is a fixed launch sequence.
+// ---------------------------------------------------------------------------
+namespace {
+
+__constant__ uint8_t kLRound[13] = {'r', 'o', 'u', 'n', 'd', '_', 'm', 'e', 's', 's', 'a', 'g', 'e'};
+__constant__ uint8_t kLChallenge[9] = {'c', 'h', 'a', 'l', 'l', 'e', 'n', 'g', 'e'};
+
+__device__ __forceinline__ int bsc_find(const BscJob *jobs, int njobs, unsigned b) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (jobs[mid].blk0 <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <bool BASE_IN, int FOLDS, bool LOOK>
+__device__ __forceinline__ void bsc_body(const BscJob &J, ext2 ca, ext2 cb, unsigned local, u64 *slot) {
+    constexpr int W = PassWords<2, LOOK>::value;
+    __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
+    u64 acc[W];
+    sc_pass_body<2, BASE_IN, FOLDS, LOOK>(J.in, J.in_stride, J.out, J.out_stride, J.items, ca, cb, local, J.nblk, acc,
+                                          lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < W; k++) slot[k] = acc[k];
+    }
+}
+
+__global__ __launch_bounds__(MAM_BLOCK) void k_bsc_pass(const BscJob *__restrict__ jobs, int njobs,
+                                                       const BscInst *__restrict__ insts, u64 *partials) {
+    const int ji = bsc_find(jobs, njobs, blockIdx.x);
+    const BscJob J = jobs[ji];
+    const unsigned local = blockIdx.x - J.blk0;
+    if (local >= J.nblk) return;  // cannot happen for a well-formed table; keeps writes in range
+    const u64 *c = insts[J.inst].ch;
+    const ext2 ca{gl_canon(c[0]), gl_canon(c[1])}, cb{gl_canon(c[2]), gl_canon(c[3])};
+    u64 *slot = partials + (size_t)blockIdx.x * MAM_BSC_WORDS;
+    // the pass shapes of a two-rounds-per-pass schedule: the first pass reads the base inputs
+    // as they are, the second reads them with two folds applied, later passes read Ext2 arrays
+    if (J.base_in && J.folds == 0) {
+        if (J.look) bsc_body<true, 0, true>(J, ca, cb, local, slot);
+        else bsc_body<true, 0, false>(J, ca, cb, local, slot);
+    } else if (J.base_in) {
+        if (J.look) bsc_body<true, 2, true>(J, ca, cb, local, slot);
+        else bsc_body<true, 2, false>(J, ca, cb, local, slot);
+    } else {
+        if (J.look) bsc_body<false, 2, true>(J, ca, cb, local, slot);
+        else bsc_body<false, 2, false>(J, ca, cb, local, slot);
+    }
+}
+
+__global__ __launch_bounds__(64) void k_bsc_round(const BscJob *__restrict__ jobs, const BscInst *__restrict__ insts,
+                                                  const u64 *__restrict__ partials) {
+    constexpr int NP = 2, DP1 = NP + 1;
+    const BscJob J = jobs[blockIdx.x];
+    const BscInst I = insts[J.inst];
+    const int W = J.look ? PassWords<NP, true>::value : PassWords<NP, false>::value;
+    // the job's reduced words: word k summed over its workgroups (lanes stride the workgroups)
+    u64 words[MAM_BSC_WORDS];
+    for (int k = 0; k < MAM_BSC_WORDS; k++) {
+        u64 v = 0;
+        if (k < W)
+            for (unsigned b = threadIdx.x; b < J.nblk; b += 64) v = gl_add(v, partials[(size_t)(J.blk0 + b) * MAM_BSC_WORDS + k]);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v = gl_add(v, shfl_xor_u64(v, off));
+        words[k] = v;
+    }
+    dtr::Wave w;
+    w.load(I.tr);
+    auto draw = [&](const u64 *m, int r) {
+        for (int c = 0; c < DP1; c++) {
+            w.label(kLRound, 13);
+            w.absorb(m[2 * c]);
+            w.absorb(m[2 * c + 1]);
+        }
+        w.label(kLChallenge, 9);
+        const u64 c0 = w.squeeze(), c1 = w.squeeze();
+        if (threadIdx.x < 2 * DP1) I.msgs[(size_t)r * 2 * DP1 + threadIdx.x] = m[threadIdx.x];
+        if (threadIdx.x == 0) {
+            I.chs[2 * r] = c0;
+            I.chs[2 * r + 1] = c1;
+        }
+        return ext2{c0, c1};
+    };
+    u64 m0[2 * DP1];
+    for (int k = 0; k < 2 * DP1; k++) m0[k] = words[k];
+    const ext2 ca = draw(m0, J.round);
+    ext2 cb = e2_zero();
+    if (J.look) {
+        // round r + 1: msg(c) = sum_j ca^j sym[c][j] (Horner)
+        u64 m1[2 * DP1];
+        for (int c = 0; c < DP1; c++) {
+            const int o = 2 * DP1 + 2 * (c * DP1);
+            ext2 v{words[o + 2 * NP], words[o + 2 * NP + 1]};
+            for (int j = NP - 1; j >= 0; j--) v = e2_add(e2_mul(v, ca), ext2{words[o + 2 * j], words[o + 2 * j + 1]});
+            m1[2 * c] = v.c0;
+            m1[2 * c + 1] = v.c1;
+        }
+        cb = draw(m1, J.round + 1);
+    }
+    w.store(I.tr);
+    if (threadIdx.x == 0) {
+        I.ch[0] = ca.c0;
+        I.ch[1] = ca.c1;
+        I.ch[2] = cb.c0;
+        I.ch[3] = cb.c1;
+    }
+    if (J.last && threadIdx.x < NP) {
+        const u64 *p = J.fin_src + threadIdx.x * J.fin_stride;
+        ext2 v;
+        if (J.fin_folds == 2) {
+            v = J.fin_base ? pass_load<true, 2>(p, 0, ca, gl_mul7(ca.c1), cb, gl_mul7(cb.c1))
+                           : pass_load<false, 2>(p, 0, ca, gl_mul7(ca.c1), cb, gl_mul7(cb.c1));
+        } else {
+            const ext2 a = J.fin_base ? e2_base(gl_canon(p[0])) : e2_load(p, 0);
+            const ext2 b = J.fin_base ? e2_base(gl_canon(p[1])) : e2_load(p, 1);
+            v = e2_add(a, e2_mul(ca, e2_sub(b, a)));
+        }
+        e2_store(I.finals, threadIdx.x, v);
+    }
+}
+
+}  // namespace
+
+int mam_bsc_launch_pass(const BscJob *d_jobs, int njobs, unsigned nblk_total, const BscInst *d_insts, u64 *partials,
+                        hipStream_t st) {
+    if (njobs <= 0 || nblk_total == 0) return -2;
+    k_bsc_pass<<<nblk_total, MAM_BLOCK, 0, st>>>(d_jobs, njobs, d_insts, partials);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int mam_bsc_launch_round(const BscJob *d_jobs, int njobs, const BscInst *d_insts, const u64 *partials,
+                         hipStream_t st) {
+    if (njobs <= 0) return -2;
+    k_bsc_round<<<njobs, 64, 0, st>>>(d_jobs, d_insts, partials);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+

@@ -18,6 +18,8 @@ enum MamProfCat {
 };
 
 bool mam_prof_enabled(int cat);
+// Suspends the timing on the calling thread (for launches recorded into a graph).
+void mam_prof_suspend(bool on);
 void mam_prof_push(int cat, double work, hipEvent_t a, hipEvent_t b);
 
 struct ProfScope {

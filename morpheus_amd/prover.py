@@ -74,6 +74,57 @@ class Transcript:
         return np.array(self.t.state, dtype=np.uint64)
 
 
+def transcript_program(ops):
+    """Encode [("append", label, values) | ("challenge", label, n), ...] for the device
+    transcript: per operation [kind, label_len, label words, n]."""
+    words, src = [], []
+    for op in ops:
+        kind, label = op[0], _b(op[1])
+        lw = (len(label) + 7) // 8
+        packed = np.frombuffer(label + bytes(8 * lw - len(label)), dtype="<u8") if lw else np.zeros(0, np.uint64)
+        if kind == "append":
+            vals = [int(v) for v in np.atleast_1d(np.asarray(op[2], dtype=np.uint64))]
+            words += [0, len(label)] + [int(v) for v in packed] + [len(vals)]
+            src += vals
+        elif kind == "challenge":
+            words += [1, len(label)] + [int(v) for v in packed] + [int(op[2])]
+        else:
+            raise ValueError(f"unknown transcript operation {kind!r}")
+    return np.array(words, dtype=np.uint64), np.array(src, dtype=np.uint64)
+
+
+class DeviceTranscript:
+    """The zk-torch-3 transcript held in device memory; ``run`` queues a program of labelled
+    appends and challenges for one wavefront (no host round trip per operation)."""
+
+    def __init__(self, transcript=None):
+        t = transcript if transcript is not None else Transcript()
+        self.state = DeviceBuffer.from_numpy(self._words(t))
+
+    @staticmethod
+    def _words(t):
+        return np.frombuffer(bytes(t.t), dtype=np.uint64).copy()
+
+    def run(self, ops):
+        prog, src = transcript_program(ops)
+        n_out = sum(int(op[2]) for op in ops if op[0] == "challenge")
+        dp = DeviceBuffer.from_numpy(prog) if prog.size else None
+        ds = DeviceBuffer.from_numpy(src) if src.size else None
+        dd = DeviceBuffer(max(n_out, 1))
+        err = DeviceBuffer(1)
+        check(lib.mam_dtranscript_run(self.state.ptr, dp.ptr if dp else None, prog.size, ds.ptr if ds else None,
+                                      src.size, dd.ptr, n_out, err.ptr), "mam_dtranscript_run")
+        e = int(err.to_numpy()[0]) & 0xFFFFFFFF
+        if e:
+            raise ValueError("device transcript program was rejected")
+        return dd.to_numpy()[:n_out]
+
+    def to_host(self):
+        t = Transcript()
+        ctypes.memmove(ctypes.addressof(t.t), self.state.to_numpy().ctypes.data, ctypes.sizeof(t.t))
+        return t
+
+
 def sumcheck_prove(transcript, d_polys, num_polys, num_vars, base=True):
     """Whole linear sumcheck proof over Ext2 on the GPU.
 

@@ -5,7 +5,7 @@ inputs, morpheus_amd.workloads). Every instance of at most 2^16 entries equals t
 proof word for word; every instance's sumcheck and Basefold opening are accepted by the
 oracle's verifiers on a replayed transcript, its claim g(0) + g(1) equals sum f g computed
 in numpy mod p, f(r) and g(r) equal the oracle's MLEs at the challenges, and f(r) is the
-opening's evaluation. One worker and the autotuned worker count give identical proofs.
+opening's evaluation. One worker, the autotuned worker count and the graph form give identical proofs.
 
 Proving service: requests m x k x n = 256^3, 256 x 4096 x 256, 4096^3 and 1024 x 512 x 2048
 (2^24-entry matrices, a 12-round degree-3 sumcheck, the int8 Ajtai prefix commit against a
@@ -27,7 +27,7 @@ from refmath import P, e2mul, gl_dot_np
 import morpheus_amd as M
 from morpheus_amd import workloads as W
 from morpheus_amd.ajtai import AjtaiI8, key_from_seed
-from morpheus_amd.batch import Instance, autotune_workers, prove_batch
+from morpheus_amd.batch import Instance, autotune_workers, prove_batch, prove_batch_graph
 from morpheus_amd.matmul import MatmulJob, lift_int, prove_matmul, serve
 
 pytestmark = pytest.mark.gpu
@@ -78,6 +78,14 @@ def test_batch_64_instances_at_config_scale():
         for i in range(len(insts)):
             for k in ("round_msgs", "challenges", "final_evals", "root", "proof"):
                 assert np.array_equal(mix[i][k], seq[i][k]), (w, i, k)
+    # the graph form (device transcripts, one graph launch for the sumchecks and commitments)
+    for b, m in ((2, 256), (0, 64)):
+        say("c4: graph form, branches", b, "max_wg", m)
+        gr = prove_batch_graph(insts, table, LR, 0, NQ, workers=8, branches=b, max_wg=m)
+        for i in range(len(insts)):
+            for k in ("round_msgs", "challenges", "final_evals", "root", "proof"):
+                assert np.array_equal(gr[i][k], seq[i][k]), ("graph", b, m, i, k)
+    assert M.lib.mam_batch_graph_release() == 0
     say("c4: verifying every instance")
     for i, (f, g, label) in enumerate(data):
         nv, o = insts[i].num_vars, seq[i]
