@@ -754,10 +754,25 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_open_u(UJob j) {
     grid_partials<4>(acc, j.partials, j.result);
 }
 
+// Workgroups of the opening's streaming reductions (two-launch form, up to RED_BLOCKS
+// partials): MAM_OPEN_GRID, default 512. Measured on the 2^22 proof: 512 -> 3.81 ms,
+// 1024 -> 3.81 ms, 2048 -> 3.90 ms per proof; these launches are 10-30 us long and more
+// workgroups only lengthen their ramp and the partial sums.
+static unsigned open_grid(size_t items) {
+    static const unsigned cap = [] {
+        const char *e = getenv("MAM_OPEN_GRID");
+        long x = e ? strtol(e, nullptr, 10) : 512L;
+        if (x < 1) x = 1;
+        if (x > RED_BLOCKS) x = RED_BLOCKS;
+        return (unsigned)x;
+    }();
+    return grid_stride(items, MAM_BLOCK, cap);
+}
+
 int mam_open_u(const u64 *bh, bool bh_base, bool fold, ext2 ch, u64 *bh_out, size_t pairs, const u64 *r, int nb,
                const RedScratch &rs, hipStream_t s) {
     if (nb < 0 || nb > 30 || pairs == 0 || pairs > ((size_t)1 << nb)) return -2;
-    const unsigned g = grid_reduce(pairs);
+    const unsigned g = open_grid(pairs);
     const UJob j = {bh, bh_out, ch, pairs, r, nb, rs.partials, rs.result};
     ProfScope prof(PROF_OPEN_SUMCHECK, (double)pairs * ((bh_base ? 8.0 : 16.0) * (fold ? 4 : 2) + (fold ? 32.0 : 0.0)),
                    s);
@@ -780,7 +795,7 @@ int mam_eq_table_scaled(const u64 *d_r, int nv, ext2 s0, ext2 s1, u64 *d_out, hi
 
 int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, const RedScratch &rs,
                      hipStream_t s) {
-    unsigned g = grid_reduce(pairs);
+    unsigned g = open_grid(pairs);
     if (bh_base)
         k_open_oracle0<true><<<g, MAM_BLOCK, 0, s>>>(eq, bh, pairs, rs.partials, rs.result);
     else
@@ -790,7 +805,7 @@ int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, c
 }
 
 int mam_open_fold_oracle_job(const OracleJob &j, hipStream_t s) {
-    const unsigned g = grid_reduce(j.half_out);
+    const unsigned g = open_grid(j.half_out);
     ProfScope prof(PROF_OPEN_SUMCHECK, (double)j.half_out * (64.0 + (j.bh_base ? 32.0 : 64.0) + 128.0), s);
     if (j.bh_base)
         k_open_fold_oracle<true><<<g, MAM_BLOCK, 0, s>>>(j);

@@ -51,7 +51,7 @@ struct FoldSrc {
 // Writes `nlev` consecutive levels. The first written level has `n_first` nodes and
 // starts at digest index `first_off` of `tree`; each next level follows the previous one.
 template <class H, int SRC>
-__global__ __launch_bounds__(TREE_BLOCK) void k_tree_levels(const u64 *__restrict__ src, u64 *__restrict__ tree,
+__global__ __launch_bounds__(TREE_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_tree_levels(const u64 *__restrict__ src, u64 *__restrict__ tree,
                                                             size_t first_off, size_t n_first, int nlev, FoldSrc fs) {
     __shared__ u64 sh[TREE_BLOCK * 4];
     const int t = threadIdx.x;
