@@ -63,16 +63,19 @@ __device__ __forceinline__ u64 concrete(u64 x, int j, u64 *st, const LaneConsts 
 #pragma unroll
     for (int c = 0; c < 12; c++) v[c] = st[c];
     __builtin_amdgcn_wave_barrier();
-    // the round constant's halves start the accumulators (as in mono::concrete)
+    // the round constant's halves start the accumulators (as in mono::concrete); even and odd
+    // columns go to separate accumulators, which halves the dependent multiply-add chain
     constexpr bool HAS_RC = ROUND >= 1 && ROUND <= 5;
     const u64 k = HAS_RC ? lc.rc[HAS_RC ? ROUND - 1 : 0] : 0;
-    u64 a = (u32)k, b = k >> 32;
+    u64 a0 = (u32)k, b0 = k >> 32, a1 = 0, b1 = 0;
 #pragma unroll
-    for (int c = 0; c < 12; c++) {
-        a += (u64)(u32)v[c] * lc.coef[c];
-        b += (u64)(u32)(v[c] >> 32) * lc.coef[c];
+    for (int c = 0; c < 12; c += 2) {
+        a0 += (u64)(u32)v[c] * lc.coef[c];
+        b0 += (u64)(u32)(v[c] >> 32) * lc.coef[c];
+        a1 += (u64)(u32)v[c + 1] * lc.coef[c + 1];
+        b1 += (u64)(u32)(v[c + 1] >> 32) * lc.coef[c + 1];
     }
-    const u64 y = mono::fold_halves(a, b);
+    const u64 y = mono::fold_halves(a0 + a1, b0 + b1);
     return j < 4 ? gl_canon(y) : y;
 }
 
