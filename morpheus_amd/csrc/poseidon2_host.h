@@ -65,11 +65,12 @@ static inline void external_linear(u64 *x) {
     }
 }
 
+// x_i d_i + sum: the sum (< 2^67) is added to each 128-bit product unreduced, so one
+// reduction per output follows the S-box on the round's critical path
 static inline void internal_linear(u64 *x, const u64 *diag) {
     u128 s = 0;
     for (int i = 0; i < 8; i++) s += x[i];
-    const u64 sr = red(s);
-    for (int i = 0; i < 8; i++) x[i] = red((u128)x[i] * diag[i] + sr);
+    for (int i = 0; i < 8; i++) x[i] = red((u128)x[i] * diag[i] + s);
 }
 
 // Accepts any 64-bit words; returns canonical residues.
