@@ -65,6 +65,9 @@ struct Commitment {
     u64 *tree = nullptr;      // (2L - 1) * 4 words, L = 2^(nv+lr-1)
     u64 root[4] = {0, 0, 0, 0};
     bool owns = false;
+    // the codeword came from this library's commit transform, so it is 2^lr identical
+    // blocks (buffers wrapped from elsewhere are not assumed to be)
+    bool replicated = false;
 };
 
 const int MAX_LEVELS = 40;
@@ -154,8 +157,10 @@ ext2 sample_ext2(mam_transcript *t) {
 int commit_into(Commitment &C, const u64 *d_evals, hipStream_t s) {
     if (mam_commit_codeword(d_evals, C.bh_evals, C.codeword, C.nv, C.lr, s)) return -1;
     const size_t leaves = (size_t)1 << (C.nv + C.lr - 1);
-    int rc = mam_merkle_build(C.codeword, C.tree, leaves, C.hash, 0, s);
+    // the codeword is 2^lr copies of one block: hash one copy's subtree
+    int rc = mam_merkle_build_copies(C.codeword, C.tree, leaves, C.hash, 0, 1 << C.lr, s);
     if (rc) return rc;
+    C.replicated = true;
     u64 *h = pinned_scratch();
     if (!h) return -1;
     if (read_back(h, C.tree + mam_merkle_root_offset(leaves), 4, s)) return -1;
@@ -748,7 +753,10 @@ int batch_open_ext2(const std::vector<const Commitment *> &C, const u64 *points,
     rc = lincomb(cws, sc, true, cw_len, ccw, s);
     if (!rc) rc = lincomb(bhs, sc, true, n, cbh1, s);
     if (!rc) rc = mam_bitrev(cbh1, cbhn, nv, 2, s);
-    if (!rc) rc = mam_merkle_build(ccw, ctree, cw_len / 2, hash, 1, s);
+    // a combination of replicated codewords is replicated
+    bool all_rep = true;
+    for (int e = 0; e < ne; e++) all_rep = all_rep && C[ev[e].poly]->replicated;
+    if (!rc) rc = mam_merkle_build_copies(ccw, ctree, cw_len / 2, hash, 1, all_rep ? 1 << lr : 1, s);
     if (!rc) rc = read_back(h, ctree + mam_merkle_root_offset(cw_len / 2), 4, s);
     if (rc) return rc;
     u64 *croot = proof + 6L * nv;

@@ -431,7 +431,7 @@ int graph_build(GraphBatch &G, const mam_batch_instance *inst, int n, int lr, in
         const int nv = inst[i].num_vars;
         const size_t leaves = (size_t)1 << (nv + lr - 1);
         if (mam_commit_codeword(inst[i].d_fg, G.bh[i], G.cw[i], nv, lr, st)) return -1;
-        if (mam_merkle_build(G.cw[i], G.tree[i], leaves, hash, 0, st)) return -1;
+        if (mam_merkle_build_copies(G.cw[i], G.tree[i], leaves, hash, 0, 1 << lr, st)) return -1;
         u64 *root = G.out + G.off[i] + 26 + 8 * (size_t)nv;
         if (hipMemcpyAsync(root, G.tree[i] + mam_merkle_root_offset(leaves), 4 * 8, hipMemcpyDeviceToDevice, st) !=
             hipSuccess)

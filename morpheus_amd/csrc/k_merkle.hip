@@ -11,6 +11,17 @@
 // then keeps compressing inside LDS while every wave of the workgroup is still full
 // (256 -> 128 -> 64 nodes), so a 2^22-leaf tree is built by 8 launches with no host
 // synchronization in between, and each digest is written to HBM exactly once.
+//
+// Replicated codewords. A Basefold commitment's codeword is 2^lr identical copies of a
+// 2^nv-word block: the reference's encode repeats every coefficient 2^lr times, its
+// butterflies act only on the bits above the low lr ones, and the bit reversal to Type1
+// order moves those repetition bits to the top (goldilocks-cuda-rs/cuda/wrapper.cu:1277-1312,
+// cuda/basefold.cuh:157-205). Leaf i and leaf i + L / 2^lr then hash the same words, so
+// every level of the tree is 2^lr copies of the subtree over the first copy's leaves, and
+// the top lr levels compress a node with itself. Builders given `rep` = 2^lr hash the first
+// copy's subtree once, store each digest at all rep positions of its level, and form the
+// top levels from the subtree root (k_tree_self_top): the same tree with half the
+// permutations at rate 1/2.
 #include <stdlib.h>
 
 #include "basefold.h"
@@ -48,11 +59,14 @@ struct FoldSrc {
     u64 *out;          // folded codeword (Ext2)
 };
 
-// Writes `nlev` consecutive levels. The first written level has `n_first` nodes and
-// starts at digest index `first_off` of `tree`; each next level follows the previous one.
+// Writes `nlev` consecutive levels. The first written level has `n_first` computed nodes
+// (rep copies of them make the level) and starts at digest index `first_off` of `tree`;
+// each next level follows the previous one. Node k of a level is stored at k + c * n for
+// every copy c < rep, n being the level's computed node count.
 template <class H, int SRC>
 __global__ __launch_bounds__(TREE_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_tree_levels(const u64 *__restrict__ src, u64 *__restrict__ tree,
-                                                            size_t first_off, size_t n_first, int nlev, FoldSrc fs) {
+                                                            size_t first_off, size_t n_first, int nlev, FoldSrc fs,
+                                                            int rep) {
     __shared__ u64 sh[TREE_BLOCK * 4];
     const int t = threadIdx.x;
     const size_t i = (size_t)blockIdx.x * TREE_BLOCK + t;
@@ -86,31 +100,33 @@ __global__ __launch_bounds__(TREE_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
             u64 ab[4] = {f[0].c0, f[0].c1, f[1].c0, f[1].c1};
             H::ext2_leaf(ab, d);
         }
-        u64 *o = tree + 4 * (first_off + i);
+        for (int c = 0; c < rep; c++) {
+            u64 *o = tree + 4 * (first_off + (size_t)c * n_first + i);
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            o[k] = d[k];
-            sh[4 * t + k] = d[k];
+            for (int k = 0; k < 4; k++) o[k] = d[k];
         }
+#pragma unroll
+        for (int k = 0; k < 4; k++) sh[4 * t + k] = d[k];
     }
     size_t level_off = first_off;
     size_t level_n = n_first;
     int block_n = (int)(n_first < TREE_BLOCK ? n_first : TREE_BLOCK);
     for (int l = 1; l < nlev; l++) {
         mam_barrier();
-        level_off += level_n;
+        level_off += level_n * rep;
         level_n >>= 1;
         block_n >>= 1;
         u64 r[4];
         if (t < block_n) H::node(sh + 8 * t, sh + 8 * t + 4, r);
         mam_barrier();
         if (t < block_n) {
-            u64 *o = tree + 4 * (level_off + (size_t)blockIdx.x * (TREE_BLOCK >> l) + t);
+            for (int c = 0; c < rep; c++) {
+                u64 *o = tree + 4 * (level_off + (size_t)c * level_n + (size_t)blockIdx.x * (TREE_BLOCK >> l) + t);
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                o[k] = r[k];
-                sh[4 * t + k] = r[k];
+                for (int k = 0; k < 4; k++) o[k] = r[k];
             }
+#pragma unroll
+            for (int k = 0; k < 4; k++) sh[4 * t + k] = r[k];
         }
     }
 }
@@ -129,7 +145,7 @@ static constexpr int WIDE_GROUPS = 32;
 template <int SRC>
 __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__restrict__ src, u64 *__restrict__ tree,
                                                                 size_t first_off, size_t n_first, int nlev,
-                                                                FoldSrc fs, OracleJob oj, MailJob mj) {
+                                                                FoldSrc fs, OracleJob oj, MailJob mj, int rep) {
     __shared__ u64 sh[WIDE_GROUPS * 4];
     __shared__ __align__(16) u64 wst[WIDE_GROUPS][16];  // per-group permutation slots
     if (oj.blocks) {
@@ -174,14 +190,14 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
     }
     x = mono_wide::permute(x, j, wst[g], lc);
     if (active && j < 4) {
-        tree[4 * (first_off + i) + j] = x;
+        for (int c = 0; c < rep; c++) tree[4 * (first_off + (size_t)c * n_first + i) + j] = x;
         sh[4 * g + j] = x;
     }
     size_t level_off = first_off, level_n = n_first;
     int block_n = (int)(n_first < WIDE_GROUPS ? n_first : WIDE_GROUPS);
     for (int l = 1; l < nlev; l++) {
         mam_barrier();
-        level_off += level_n;
+        level_off += level_n * rep;
         level_n >>= 1;
         block_n >>= 1;
         u64 y = (g < block_n && j < 8) ? sh[8 * g + j] : 0;
@@ -189,7 +205,8 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
         // wave-uniform: a wave (4 groups) runs the permutation if its first group is active
         if ((g & ~3) < block_n) y = mono_wide::permute(y, j, wst[g], lc);
         if (g < block_n && j < 4) {
-            tree[4 * (level_off + (size_t)blockIdx.x * (WIDE_GROUPS >> l) + g) + j] = y;
+            for (int c = 0; c < rep; c++)
+                tree[4 * (level_off + (size_t)c * level_n + (size_t)blockIdx.x * (WIDE_GROUPS >> l) + g) + j] = y;
             sh[4 * g + j] = y;
         }
     }
@@ -198,6 +215,44 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
         const int t = threadIdx.x;
         if (t < mj.na + 4) mj.box[t] = t < mj.na ? mj.a[t] : sh[t - mj.na];
         mam_mailbox_raise(mj.box, mj.seq);
+    }
+}
+
+// Top levels of a tree over rep copies of one subtree: the level at digest `prev_off` holds
+// rep copies of the subtree root; each level above holds half as many copies of the
+// compression of a node with itself, up to the root.
+template <class H>
+__global__ void k_tree_self_top(u64 *__restrict__ tree, size_t prev_off, int rep) {
+    if (H::kWide) {
+        // one 16-lane group
+        __shared__ __align__(16) u64 wst[16];
+        const int j = threadIdx.x & 15;
+        mono_wide::LaneConsts lc;
+        mono_wide::lane_consts(j, lc);
+        u64 h = j < 8 ? tree[4 * prev_off + (j & 3)] : 0;
+        size_t off = prev_off + rep;
+        for (int n = rep >> 1; n >= 1; n >>= 1) {
+            h = mono_wide::permute(h, j, wst, lc);
+            if (j < 4)
+                for (int c = 0; c < n; c++) tree[4 * (off + c) + j] = h;
+            // the next level compresses this digest with itself: lanes 4..7 take lanes 0..3
+            wst[j] = h;
+            __builtin_amdgcn_wave_barrier();
+            h = j < 8 ? wst[j & 3] : 0;
+            __builtin_amdgcn_wave_barrier();
+            off += n;
+        }
+    } else if (threadIdx.x == 0) {
+        u64 h[4] = {tree[4 * prev_off], tree[4 * prev_off + 1], tree[4 * prev_off + 2], tree[4 * prev_off + 3]};
+        size_t off = prev_off + rep;
+        for (int n = rep >> 1; n >= 1; n >>= 1) {
+            u64 r[4];
+            H::node(h, h, r);
+            for (int c = 0; c < n; c++)
+                for (int k = 0; k < 4; k++) tree[4 * (off + c) + k] = r[k];
+            for (int k = 0; k < 4; k++) h[k] = r[k];
+            off += n;
+        }
     }
 }
 
@@ -229,12 +284,15 @@ static int levels_per_launch(size_t n) {
 // when that launch is lane-parallel (workgroups of the same size), else it is launched on
 // its own before the tree. mj (may be null): a mailbox post that the final launch makes
 // after the root when that launch is one lane-parallel workgroup without the oracle job;
-// *mail_posted tells the caller whether it did.
+// *mail_posted tells the caller whether it did. rep (a power of two): the leaves are rep
+// copies of the first num_leaves / rep (a replicated codeword); only that subtree is hashed.
 template <class H>
 static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_src, const FoldSrc &fs,
                       hipStream_t s, const OracleJob *oj = nullptr, const MailJob *mj = nullptr,
-                      bool *mail_posted = nullptr) {
+                      bool *mail_posted = nullptr, int rep = 1) {
     if (num_leaves == 0 || (num_leaves & (num_leaves - 1))) return -2;
+    if (rep < 1 || (rep & (rep - 1)) || (size_t)rep > num_leaves) return -2;
+    if (rep > 1 && (oj || mj)) return -2;  // replicated trees are commitments: no opening jobs
     if (mail_posted) *mail_posted = false;
     const OracleJob none = {};
     const MailJob no_mail = {};
@@ -247,9 +305,10 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
             return -1;
         }
     }
-    ProfScope prof(PROF_MERKLE, 2.0 * (double)num_leaves - 1.0, s);
-    size_t n = num_leaves, off = 0;
-    int total_levels = ilog2_sz(num_leaves) + 1;
+    // permutations executed: the subtree over one copy, then one per level above it
+    ProfScope prof(PROF_MERKLE, 2.0 * (double)(num_leaves / rep) - 1.0 + ilog2_sz((size_t)rep), s);
+    size_t n = num_leaves / rep, off = 0;  // n: computed nodes of the level being written
+    int total_levels = ilog2_sz(n) + 1;
     int done = 0;
     bool first = true;
     const bool can_wide = H::kWide;
@@ -267,26 +326,27 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
                 const unsigned g2 = grid + j.blocks;
                 switch (leaf_src) {
                     case SRC_GL:
-                        k_tree_wide<SRC_GL><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
+                        k_tree_wide<SRC_GL><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m, rep);
                         break;
                     case SRC_EXT2:
-                        k_tree_wide<SRC_EXT2><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
+                        k_tree_wide<SRC_EXT2><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m, rep);
                         break;
                     case SRC_FOLD_GL:
-                        k_tree_wide<SRC_FOLD_GL><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
+                        k_tree_wide<SRC_FOLD_GL><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m, rep);
                         break;
                     default:
-                        k_tree_wide<SRC_FOLD_EXT2><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m);
+                        k_tree_wide<SRC_FOLD_EXT2><<<g2, threads, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, j, m, rep);
                         break;
                 }
             } else {
-                const u64 *child = d_tree + 4 * (off - 2 * n);
-                k_tree_wide<SRC_NODES><<<grid, threads, 0, s>>>(child, d_tree, off, n, nl, fs, none, m);
+                // children: the computed part of the level that ends right before `off`
+                const u64 *child = d_tree + 4 * (off - 2 * n * rep);
+                k_tree_wide<SRC_NODES><<<grid, threads, 0, s>>>(child, d_tree, off, n, nl, fs, none, m, rep);
             }
             if (post && mail_posted) *mail_posted = true;
             if (hipGetLastError() != hipSuccess) return -1;
             for (int l = 0; l < nl; l++) {
-                off += n;
+                off += n * rep;
                 n >>= 1;
             }
             done += nl;
@@ -302,39 +362,53 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
             for (int l = 0; l < nl; l++) launch_perms += (double)(n >> l);
             ProfScope leaf_prof(leaf_src == SRC_GL && H::kWide ? PROF_LEAF_LAUNCH : -1, launch_perms, s);
             switch (leaf_src) {
-                case SRC_GL: k_tree_levels<H, SRC_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs); break;
+                case SRC_GL:
+                    k_tree_levels<H, SRC_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
+                    break;
                 case SRC_EXT2:
-                    k_tree_levels<H, SRC_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                    k_tree_levels<H, SRC_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
                     break;
                 case SRC_FOLD_GL:
-                    k_tree_levels<H, SRC_FOLD_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                    k_tree_levels<H, SRC_FOLD_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
                     break;
                 default:
-                    k_tree_levels<H, SRC_FOLD_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs);
+                    k_tree_levels<H, SRC_FOLD_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
                     break;
             }
         } else {
-            // children are the level ending right before `off`
-            const u64 *child = d_tree + 4 * (off - 2 * n);
-            k_tree_levels<H, SRC_NODES><<<grid, TREE_BLOCK, 0, s>>>(child, d_tree, off, n, nl, fs);
+            // children: the computed part of the level that ends right before `off`
+            const u64 *child = d_tree + 4 * (off - 2 * n * rep);
+            k_tree_levels<H, SRC_NODES><<<grid, TREE_BLOCK, 0, s>>>(child, d_tree, off, n, nl, fs, rep);
         }
         if (hipGetLastError() != hipSuccess) return -1;
         for (int l = 0; l < nl; l++) {
-            off += n;
+            off += n * rep;
             n >>= 1;
         }
         done += nl;
         first = false;
     }
+    if (rep > 1) {
+        // the subtree roots are the rep nodes ending at `off`
+        k_tree_self_top<H><<<1, 16, 0, s>>>(d_tree, off - rep, rep);
+        if (hipGetLastError() != hipSuccess) return -1;
+    }
     return 0;
 }
 
-int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, int ext2_leaves, hipStream_t s) {
+int mam_merkle_build_copies(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, int ext2_leaves, int copies,
+                            hipStream_t s) {
     FoldSrc fs = {nullptr, e2_zero(), nullptr};
     int src = ext2_leaves ? SRC_EXT2 : SRC_GL;
-    if (hash == MAM_HASH_MONOLITH) return build_tree<MonoHash>(d_cw, d_tree, num_leaves, src, fs, s);
-    if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(d_cw, d_tree, num_leaves, src, fs, s);
+    if (hash == MAM_HASH_MONOLITH)
+        return build_tree<MonoHash>(d_cw, d_tree, num_leaves, src, fs, s, nullptr, nullptr, nullptr, copies);
+    if (hash == MAM_HASH_POSEIDON2)
+        return build_tree<P2Hash>(d_cw, d_tree, num_leaves, src, fs, s, nullptr, nullptr, nullptr, copies);
     return -2;
+}
+
+int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, int ext2_leaves, hipStream_t s) {
+    return mam_merkle_build_copies(d_cw, d_tree, num_leaves, hash, ext2_leaves, 1, s);
 }
 
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,

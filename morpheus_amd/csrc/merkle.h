@@ -13,5 +13,11 @@
 int mam_merkle_build(const uint64_t *d_cw, uint64_t *d_tree, size_t num_leaves, int hash, int ext2_leaves,
                      hipStream_t s);
 
+// The same tree for leaves that are `copies` (a power of two) repetitions of the first
+// num_leaves / copies leaves, as over a Basefold commitment's codeword (2^log_rate identical
+// blocks): hashes only the first copy's subtree and stores every digest at all copies.
+int mam_merkle_build_copies(const uint64_t *d_cw, uint64_t *d_tree, size_t num_leaves, int hash, int ext2_leaves,
+                            int copies, hipStream_t s);
+
 // u64 offset of the root digest in a tree with `num_leaves` leaves.
 size_t mam_merkle_root_offset(size_t num_leaves);

@@ -177,7 +177,12 @@ def test_commit_pieces(nv, lr):
     assert np.array_equal(d_bh.to_numpy(), np.array([int(x) % P for x in ev], dtype=np.uint64))
 
 
-@pytest.mark.parametrize("nv,lr,hash_kind", [(1, 1, 0), (2, 1, 0), (5, 2, 1), (9, 1, 0), (12, 3, 0), (14, 1, 1)])
+# The commitment hashes one copy of its replicated codeword and stores every digest at all
+# 2^lr positions: the whole tree is compared with the oracle's full hash of the codeword.
+# Sizes reach every tree kernel: lane-parallel only (small), the one-lane leaf and node
+# launches with the lane-parallel top (nv 17, 18), and log rates 0 to 3.
+@pytest.mark.parametrize("nv,lr,hash_kind", [(1, 1, 0), (2, 1, 0), (5, 2, 1), (9, 1, 0), (12, 3, 0), (14, 1, 1),
+                                             (13, 0, 0), (17, 1, 0), (18, 2, 0), (16, 1, 1)])
 def test_commit_root(nv, lr, hash_kind):
     say("commit", nv, lr, hash_kind)
     rng = np.random.default_rng(600 + nv)
@@ -187,6 +192,10 @@ def test_commit_root(nv, lr, hash_kind):
     assert np.array_equal(c.root, tree_o[-4:])
     cw, bh = c.to_host_cache()
     assert np.array_equal(cw, cw_o) and np.array_equal(bh, bh_o)
+    _, _, d_tree = c.buffers()
+    tree = np.empty(tree_o.size, dtype=np.uint64)
+    ok(M.lib.cuda_memcpy_dtoh(tree.ctypes.data, d_tree, tree.size * 8))
+    assert np.array_equal(tree, tree_o)
 
 
 def test_codeword_folds_and_sumcheck_pieces():
