@@ -3,6 +3,7 @@
 // Entry points replace goldilocks-cuda-rs/src/ffi.rs:11-26, 30-53, 306-317, 159-296
 // (the `gl_batch_*`, `ext2_batch_*` and conversion symbols) with the same names,
 // arguments and return codes; implementations follow goldilocks-cuda-rs/cuda/wrapper.cu:27-97.
+#include <stdlib.h>
 #include <string.h>
 
 #include "common.h"
@@ -395,6 +396,81 @@ __global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, 
     __threadfence_system();
     mam_barrier();
     if (t == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// ---------------------------------------------------------------------------
+// Gates for pre-queued rounds (common.h). Per thread: pinned words [flag | 2 slots of 8]
+// and a device copy of each slot; slot seq & 1 serves release seq, so the host never
+// rewrites a slot that a queued copy may still read.
+// ---------------------------------------------------------------------------
+namespace {
+struct Gate {
+    u64 *pin = nullptr;  // [0]: released number; [8 + 8 k]: slot k
+    u64 *dev = nullptr;  // 2 x 8 words
+    u64 armed = 0;       // last number queued
+    bool ok = false;
+    ~Gate() {
+        if (pin) (void)hipHostFree(pin);
+        if (dev) (void)hipFree(dev);
+    }
+};
+thread_local Gate t_gate;
+thread_local int t_prequeue = -1;  // -1: not set on this thread (the environment decides)
+
+bool gate_ready() {
+    if (t_gate.ok) return true;
+    if (hipHostMalloc((void **)&t_gate.pin, 24 * sizeof(u64), hipHostMallocCoherent) != hipSuccess) {
+        t_gate.pin = nullptr;
+        return false;
+    }
+    if (hipMalloc((void **)&t_gate.dev, 16 * sizeof(u64)) != hipSuccess) {
+        (void)hipHostFree(t_gate.pin);
+        t_gate.pin = nullptr;
+        t_gate.dev = nullptr;
+        return false;
+    }
+    memset(t_gate.pin, 0, 24 * sizeof(u64));
+    t_gate.armed = 0;
+    t_gate.ok = true;
+    return true;
+}
+}  // namespace
+
+bool mam_prequeue_enabled() {
+    static const bool env_on = [] {
+        const char *e = getenv("MAM_PREQUEUE");
+        return !(e && e[0] == '0');
+    }();
+    if (t_prequeue >= 0) return t_prequeue == 1;
+    return env_on;
+}
+
+void mam_prequeue_set(bool on) { t_prequeue = on ? 1 : 0; }
+
+const u64 *mam_gate_queue(hipStream_t s, int nwords, u64 *seq) {
+    if (nwords < 1 || nwords > 8 || !gate_ready()) return nullptr;
+    const u64 n = t_gate.armed + 1;
+    const int k = (int)(n & 1);
+    if (hipStreamWaitValue64(s, t_gate.pin, n, hipStreamWaitValueGte, ~(u64)0) != hipSuccess) return nullptr;
+    t_gate.armed = n;
+    if (hipMemcpyAsync(t_gate.dev + 8 * k, t_gate.pin + 8 + 8 * k, nwords * sizeof(u64), hipMemcpyHostToDevice, s) !=
+        hipSuccess) {
+        mam_gate_open_all();
+        return nullptr;
+    }
+    *seq = n;
+    return t_gate.dev + 8 * k;
+}
+
+void mam_gate_open(u64 seq, const u64 *vals, int nwords) {
+    if (!t_gate.ok) return;
+    u64 *slot = t_gate.pin + 8 + 8 * (seq & 1);
+    for (int i = 0; i < nwords && i < 8; i++) slot[i] = vals[i];
+    __atomic_store_n(&t_gate.pin[0], seq, __ATOMIC_RELEASE);
+}
+
+void mam_gate_open_all() {
+    if (t_gate.ok) __atomic_store_n(&t_gate.pin[0], t_gate.armed, __ATOMIC_RELEASE);
 }
 
 u64 *mam_mailbox_arm(u64 *seq) {

@@ -97,6 +97,25 @@ __device__ __forceinline__ void mam_mailbox_raise(u64 *box, u64 seq) {
     if (threadIdx.x == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
+// Pre-queued rounds. A host-driven proving round needs values the host computes from the
+// previous round's result (Fiat-Shamir challenges). Launching the round's kernels only after
+// that costs an idle-queue launch (~13 us until the first kernel runs on the box); queuing
+// them beforehand behind a wait on a pinned flag costs ~6 us from the host's release to the
+// kernel's result. The gate of the calling thread: mam_gate_queue queues on stream s a wait
+// for the next release and a copy of nwords (<= 8) scalars from the pinned slot into device
+// memory, and returns that device address (the queued kernels read their scalars there) and
+// the release number; mam_gate_open writes the scalars and releases. Every queued gate must
+// be opened (mam_gate_open_all on error paths), or the stream stays blocked.
+// A stream waiting at a gate also holds up the other streams that share its hardware queue,
+// so threads that prove side by side with others (the worker pools) do not pre-queue:
+// mam_prequeue_enabled() is false on them (mam_prequeue_set), and MAM_PREQUEUE=0 turns it
+// off everywhere.
+bool mam_prequeue_enabled();
+void mam_prequeue_set(bool on);
+const u64 *mam_gate_queue(hipStream_t s, int nwords, u64 *seq);
+void mam_gate_open(u64 seq, const u64 *vals, int nwords);
+void mam_gate_open_all();
+
 // A mailbox post handed to a kernel: na words from a, then the kernel's own words.
 struct MailJob {
     const u64 *a;

@@ -225,19 +225,65 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
     };
     ext2 ca = e2_zero(), cb = e2_zero();
     int rc = 0;
+    // Pass k + 1 (or the final values) is queued behind a gate before pass k's words are
+    // read, and released with its challenges once they are drawn (common.h).
+    const bool pre = mam_prequeue_enabled();
+    const Pass &last = passes[P - 1];
+    const u64 *fsrc = last.folds ? last.out : last.src;
+    const size_t fstride = last.folds ? last.out_stride : last.src_stride;
+    const bool fbase = last.folds ? false : last.src_base;
+    struct Queued {
+        u64 gate = 0, seq = 0;
+        u64 *box = nullptr;
+        bool posted = false;
+    };
+    // queue pass k (k == P: the final values) with its challenges from device memory
+    auto queue = [&](int k, Queued &q) -> int {
+        const u64 *d_ch = mam_gate_queue(S.st, 4, &q.gate);
+        if (!d_ch) return -1;
+        if (k == P)
+            return mam_sc_launch_pass_final(np, fsrc, fstride, fbase, last.look ? 2 : 1, ca, cb, S.finals, S.st, d_ch);
+        const Pass &p = passes[k];
+        q.box = mam_mailbox_arm(&q.seq);
+        if (!q.box) return -1;
+        const MailJob mail = {nullptr, 0, q.box, q.seq};
+        int r = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride, p.items,
+                                   ca, cb, S.s, S.st, &mail, &q.posted, d_ch);
+        if (!r && !q.posted) r = mam_mailbox_post(S.s.msg, words(p), nullptr, 0, q.box, q.seq, S.st);
+        return r;
+    };
+    Queued next;
+    bool have_next = false;
     for (int k = 0; k < P && !rc; k++) {
         const Pass &p = passes[k];
         u64 seq = 0;
-        u64 *box = mam_mailbox_arm(&seq);
-        if (!box) rc = -1;
-        const MailJob mail = {nullptr, 0, box, seq};
-        bool posted = false;
-        if (!rc)
-            rc = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride, p.items,
-                                    ca, cb, S.s, S.st, &mail, &posted);
-        if (!rc && !posted) rc = mam_mailbox_post(S.s.msg, words(p), nullptr, 0, box, seq, S.st);
+        u64 *box = nullptr;
+        if (have_next) {
+            box = next.box;
+            seq = next.seq;
+            have_next = false;
+        } else {
+            box = mam_mailbox_arm(&seq);
+            if (!box) rc = -1;
+            const MailJob mail = {nullptr, 0, box, seq};
+            bool posted = false;
+            if (!rc)
+                rc = mam_sc_launch_pass(np, p.src, p.src_stride, p.src_base, p.folds, p.look, p.out, p.out_stride,
+                                        p.items, ca, cb, S.s, S.st, &mail, &posted);
+            if (!rc && !posted) rc = mam_mailbox_post(S.s.msg, words(p), nullptr, 0, box, seq, S.st);
+        }
+        Queued q;
+        if (!rc && pre) {
+            if (queue(k + 1, q)) {
+                mam_gate_open_all();
+                rc = -1;
+            }
+        }
         if (!rc) rc = mam_mailbox_wait(box, seq, words(p), S.host, S.st);
-        if (rc) break;
+        if (rc) {
+            mam_gate_open_all();
+            break;
+        }
         u64 *m0 = round_msgs + (size_t)p.r * dp1 * 2;
         memcpy(m0, S.host, dp1 * 2 * sizeof(u64));
         ca = draw(m0, p.r);
@@ -254,16 +300,21 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
             }
             cb = draw(m1, p.r + 1);
         }
+        if (pre) {
+            const u64 w[4] = {ca.c0, ca.c1, cb.c0, cb.c1};
+            mam_gate_open(q.gate, w, 4);
+            if (k + 1 < P) {
+                next = q;
+                have_next = true;
+            }
+        }
     }
     if (rc) {
         S.finished = true;
         return rc;
     }
-    const Pass &last = passes[P - 1];
-    const u64 *fsrc = last.folds ? last.out : last.src;
-    const size_t fstride = last.folds ? last.out_stride : last.src_stride;
-    const bool fbase = last.folds ? false : last.src_base;
-    rc = mam_sc_launch_pass_final(np, fsrc, fstride, fbase, last.look ? 2 : 1, ca, cb, S.finals, S.st);
+    // the final values (already queued behind the last gate when pre-queuing)
+    if (!pre) rc = mam_sc_launch_pass_final(np, fsrc, fstride, fbase, last.look ? 2 : 1, ca, cb, S.finals, S.st);
     if (!rc) rc = mam_publish_wait(S.finals, 2 * np, nullptr, 0, S.host, S.st);
     if (!rc) memcpy(final_evals, S.host, 2 * np * sizeof(u64));
     S.finished = true;

@@ -354,6 +354,13 @@ struct PassWords {
     static constexpr int value = 2 * (NP + 1) + (LOOK ? 2 * (NP + 1) * (NP + 1) : 0);
 };
 
+// The pending challenges from device memory (a pre-queued pass): ca at chp[0..1], cb at
+// chp[2..3].
+__device__ __forceinline__ void pass_challenges(const u64 *chp, ext2 &ca, ext2 &cb) {
+    ca = ext2{gl_canon(chp[0]), gl_canon(chp[1])};
+    cb = ext2{gl_canon(chp[2]), gl_canon(chp[3])};
+}
+
 // ca7 = 7 ca.c1, cb7 = 7 cb.c1 (formed once per thread, see e2_mul_fixed)
 template <bool BASE_IN, int FOLDS>
 __device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, u64 ca7, ext2 cb, u64 cb7) {
@@ -523,10 +530,11 @@ __device__ __forceinline__ void sc_pass_body(const u64 *__restrict__ in, size_t 
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ in, size_t in_stride,
                                                       u64 *__restrict__ out, size_t out_stride, size_t items,
-                                                      ext2 ca, ext2 cb, u64 *partials, u64 *res) {
+                                                      ext2 ca, ext2 cb, const u64 *chp, u64 *partials, u64 *res) {
     constexpr int W = PassWords<NP, LOOK>::value;
     __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
     u64 acc[W];
+    if (chp) pass_challenges(chp, ca, cb);
     sc_pass_body<NP, BASE_IN, FOLDS, LOOK>(in, in_stride, out, out_stride, items, ca, cb, blockIdx.x, gridDim.x, acc,
                                            lds);
     grid_partials<W>(acc, partials, res);
@@ -557,7 +565,8 @@ struct LpTasks {
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict__ in, size_t in_stride,
                                                          u64 *__restrict__ out, size_t out_stride, size_t items,
-                                                         ext2 ca, ext2 cb, u64 *partials, u64 *res) {
+                                                         ext2 ca, ext2 cb, const u64 *chp, u64 *partials,
+                                                         u64 *res) {
     using T = LpTasks<NP, LOOK>;
     constexpr int G = T::G, AW = T::WORDS, W = PassWords<NP, LOOK>::value;
     static_assert(NP * G <= LP_LANES && T::ALL <= LP_LANES, "an item must fit its lanes");
@@ -568,6 +577,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass_lp(const u64 *__restrict_
     u64 acc[AW];
 #pragma unroll
     for (int k = 0; k < AW; k++) acc[k] = 0;
+    if (chp) pass_challenges(chp, ca, cb);
     const u64 ca7 = gl_mul7(ca.c1), cb7 = gl_mul7(cb.c1);
     for (size_t base = (size_t)blockIdx.x * LP_ITEMS; base < items; base += (size_t)gridDim.x * LP_ITEMS) {
         const size_t z = base + slot;
@@ -683,9 +693,10 @@ static size_t lp_max_items() {
 // thread per polynomial).
 template <bool BASE_IN, int FOLDS>
 __global__ void k_sc_pass_final(const u64 *__restrict__ in, size_t in_stride, int np, ext2 ca, ext2 cb,
-                                u64 *finals) {
+                                const u64 *chp, u64 *finals) {
     const int i = threadIdx.x;
     if (i >= np) return;
+    if (chp) pass_challenges(chp, ca, cb);
     const u64 *p = in + i * in_stride;
     ext2 v;
     if (FOLDS == 2) {
@@ -700,7 +711,8 @@ __global__ void k_sc_pass_final(const u64 *__restrict__ in, size_t in_stride, in
 
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
 static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_stride, size_t items, ext2 ca,
-                         ext2 cb, const ScScratch &s, hipStream_t st, const MailJob *mj, bool *posted) {
+                         ext2 cb, const u64 *chp, const ScScratch &s, hipStream_t st, const MailJob *mj,
+                         bool *posted) {
     unsigned g = grid_stride(items, MAM_BLOCK, MAM_RED_MAX_BLOCKS);
     ProfScope prof(PROF_SUMCHECK,
                    (double)items * NP * ((LOOK ? 4 : 2) * (FOLDS ? 4 : 1) * (BASE_IN ? 8.0 : 16.0) +
@@ -713,10 +725,10 @@ static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_s
         const unsigned glp = (unsigned)(want < cap ? (want ? want : 1) : cap);
         g = glp;
         k_sc_pass_lp<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca,
-                                                                       cb, s.partials, s.msg);
+                                                                       cb, chp, s.partials, s.msg);
     } else {
         k_sc_pass<NP, BASE_IN, FOLDS, LOOK><<<g, MAM_BLOCK, 0, st>>>(in, in_stride, out, out_stride, items, ca, cb,
-                                                                     s.partials, s.msg);
+                                                                     chp, s.partials, s.msg);
     }
     const bool p = launch_sum_partials<PassWords<NP, LOOK>::value>(s.partials, g, s.msg, st, mj);
     if (posted) *posted = p;
@@ -725,11 +737,11 @@ static int launch_pass_t(const u64 *in, size_t in_stride, u64 *out, size_t out_s
 
 template <int NP>
 static int launch_pass_np(const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
-                          size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
-                          const MailJob *mj, bool *posted) {
+                          size_t out_stride, size_t items, ext2 ca, ext2 cb, const u64 *chp, const ScScratch &s,
+                          hipStream_t st, const MailJob *mj, bool *posted) {
 #define MAM_PASS(B, F, L)                                                                            \
     if (base_in == B && folds == F && look == L)                                                     \
-        return launch_pass_t<NP, B, F, L>(in, in_stride, out, out_stride, items, ca, cb, s, st, mj, posted);
+        return launch_pass_t<NP, B, F, L>(in, in_stride, out, out_stride, items, ca, cb, chp, s, st, mj, posted);
     MAM_PASS(true, 0, true)
     MAM_PASS(true, 0, false)
     MAM_PASS(true, 2, true)
@@ -744,29 +756,29 @@ static int launch_pass_np(const u64 *in, size_t in_stride, bool base_in, int fol
 
 int mam_sc_launch_pass(int np, const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
                        size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
-                       const MailJob *mj, bool *posted) {
+                       const MailJob *mj, bool *posted, const u64 *d_ch) {
     if (posted) *posted = false;
     switch (np) {
         case 1:
-            return launch_pass_np<1>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, mj,
-                                     posted);
+            return launch_pass_np<1>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, d_ch, s, st,
+                                     mj, posted);
         case 2:
-            return launch_pass_np<2>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, mj,
-                                     posted);
+            return launch_pass_np<2>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, d_ch, s, st,
+                                     mj, posted);
         case 3:
-            return launch_pass_np<3>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, s, st, mj,
-                                     posted);
+            return launch_pass_np<3>(in, in_stride, base_in, folds, look, out, out_stride, items, ca, cb, d_ch, s, st,
+                                     mj, posted);
         default: return -2;
     }
 }
 
 int mam_sc_launch_pass_final(int np, const u64 *in, size_t in_stride, bool base_in, int folds, ext2 ca, ext2 cb,
-                             u64 *finals, hipStream_t st) {
+                             u64 *finals, hipStream_t st, const u64 *d_ch) {
     if (np < 1 || np > MAM_MAX_POLYS || (folds != 1 && folds != 2)) return -2;
-    if (base_in && folds == 2) k_sc_pass_final<true, 2><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
-    else if (base_in) k_sc_pass_final<true, 1><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
-    else if (folds == 2) k_sc_pass_final<false, 2><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
-    else k_sc_pass_final<false, 1><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, finals);
+    if (base_in && folds == 2) k_sc_pass_final<true, 2><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, d_ch, finals);
+    else if (base_in) k_sc_pass_final<true, 1><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, d_ch, finals);
+    else if (folds == 2) k_sc_pass_final<false, 2><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, d_ch, finals);
+    else k_sc_pass_final<false, 1><<<1, 64, 0, st>>>(in, in_stride, np, ca, cb, d_ch, finals);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

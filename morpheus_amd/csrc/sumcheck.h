@@ -34,12 +34,14 @@ int mam_qsc_launch_fold_final(int np, const u64 *in, size_t in_stride, bool base
 // Two rounds per pass (whole-proof prover, np <= MAM_LOOK_MAX_POLYS): see k_sumcheck.hip.
 #define MAM_LOOK_MAX_POLYS 3
 // mj (may be null): a post of the pass's words to that host mailbox, made by the launch that
-// adds the workgroup partials when there is one; *posted tells whether it did.
+// adds the workgroup partials when there is one; *posted tells whether it did. d_ch (may be
+// null): device words ca.c0, ca.c1, cb.c0, cb.c1 that replace ca and cb (a pre-queued pass
+// whose challenges arrive through a gate).
 int mam_sc_launch_pass(int np, const u64 *in, size_t in_stride, bool base_in, int folds, bool look, u64 *out,
                        size_t out_stride, size_t items, ext2 ca, ext2 cb, const ScScratch &s, hipStream_t st,
-                       const MailJob *mj = nullptr, bool *posted = nullptr);
+                       const MailJob *mj = nullptr, bool *posted = nullptr, const u64 *d_ch = nullptr);
 int mam_sc_launch_pass_final(int np, const u64 *in, size_t in_stride, bool base_in, int folds, ext2 ca, ext2 cb,
-                             u64 *finals, hipStream_t st);
+                             u64 *finals, hipStream_t st, const u64 *d_ch = nullptr);
 
 // ---------------------------------------------------------------------------
 // Batched product sumcheck with the transcript on the device (k_sumcheck.hip): every

@@ -9,6 +9,8 @@
 #include <mutex>
 #include <thread>
 
+#include "common.h"
+
 class WorkerPool {
    public:
     // Run f on `n` pool threads (created on first need) and wait until every one returned.
@@ -29,6 +31,8 @@ class WorkerPool {
 
    private:
     void loop(int id) {
+        // workers share the hardware queues with each other: no pre-queued rounds here
+        mam_prequeue_set(false);
         unsigned seen = 0;
         for (;;) {
             std::function<void()> j;
