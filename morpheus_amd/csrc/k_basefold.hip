@@ -12,8 +12,9 @@
 // MI355X design.
 // * The interpolation (Moebius transform: hi -= lo per bit) and the encode (Walsh-Hadamard
 //   butterflies lo+hi, lo-hi per bit over the repetition code, then bit reversal) are each
-//   a tensor product of one 2x2 matrix per bit, so each runs as ceil(bits/8) passes of an
-//   LDS-tiled radix-256 transform (k_bfly_pass) instead of one strided launch per bit.
+//   a tensor product of one 2x2 matrix per bit, so each runs as ceil(bits/8) passes of a
+//   radix-256 transform held in registers with one LDS exchange per pass (k_bfly_reg)
+//   instead of one strided launch per bit.
 // * Bit reversal commutes with any transform that applies the same 2x2 matrix to every bit.
 //   Hence encode(coeffs) = replicate(WHT(bitrev(coeffs))), and the whole commit codeword
 //   equals replicate(T(bitrev(evals))) with T = (lo, hi) -> (hi, 2 lo - hi) per bit
@@ -49,78 +50,119 @@ __device__ __forceinline__ void bfly(u64 &lo, u64 &hi) {
     }
 }
 
-static constexpr int BF_COLS = 16;                 // columns per workgroup
-static constexpr int BF_MAXKB = 8;                 // row bits per pass
-static constexpr int BF_TILE = BF_COLS << BF_MAXKB;  // 4096 words = 32 KiB of LDS
+static constexpr int BF_MAXKB = 8;  // row bits per pass
 
-// One pass over bits [b0, b0 + kb) of an array of 2^nbits words. Column c of the pass is
-// (high = c >> b0, low = c & (2^b0 - 1)); element (row, c) sits at
-// high * 2^(b0+kb) + row * 2^b0 + low. A workgroup owns 16 consecutive columns. The last
-// pass may write `reps` copies, the r-th at offset r * rep_stride.
-template <int OP>
-// src may alias dst: every element is read into LDS before any is written back.
-__global__ __launch_bounds__(256) void k_bfly_pass(const u64 *src, u64 *dst, int b0, int kb,
-                                                   int reps, size_t rep_stride) {
-    __shared__ u64 t[BF_TILE + BF_TILE / 32];
-    const int R = 1 << kb;
-    const int T = R * BF_COLS;
-    const size_t c0 = (size_t)blockIdx.x * BF_COLS;
-    // padded LDS index: one extra word every 32 breaks power-of-two strides
+// Register-resident form of one pass over bits [b0, b0 + KB) (KB = 4..8): the workgroup's
+// tile is 2^KB rows (the pass's bits) x C = 4096 / 2^KB columns (the other bits), column
+// gc = (hi, lo) holding element hi * 2^(b0+KB) + row * 2^b0 + lo. Every thread keeps 16 of
+// the tile's words in registers twice: first the 16 rows that differ in the top 4 row bits
+// (those 4 butterfly stages run in registers), then, after one exchange through LDS, 2^(KB-4)
+// rows x 2^(8-KB) columns (the remaining KB - 4 stages). Each word crosses LDS once per pass,
+// where the radix-2 form above reads and writes it once per bit, which left the pass bound by
+// LDS bandwidth rather than by HBM. Global accesses are coalesced for b0 = 0 (columns are
+// contiguous runs of rows) and for b0 >= 4 (rows are runs of at least 16 columns).
+// src may alias dst: the workgroup reads its whole tile before it writes.
+//
+// BREV (first pass of the commit, b0 = 0): the array transformed is bh = bitrev(src) over
+// nb bits, written to `side` as well. Column m of the workgroup is bh's column
+// gc = bitrev(m) (nb - KB bits), whose row r is src[bitrev_KB(r) * 2^(nb-KB) + m], so the
+// workgroup reads runs of consecutive source words, and the bit reversal costs no pass of
+// its own.
+template <int OP, int KB, bool BREV>
+__global__ __launch_bounds__(256) void k_bfly_reg(const u64 *src, u64 *dst, int b0, int reps, size_t rep_stride,
+                                                  u64 *side, int nb) {
+    constexpr int C = 4096 >> KB;        // columns per workgroup
+    constexpr int RA = 1 << (KB - 4);    // phase A: rows per register step
+    constexpr int KBB = KB - 4;          // phase B: row bits in registers
+    constexpr int CB = 1 << (8 - KB);    // phase B: columns per thread
+    __shared__ u64 t[4096 + 4096 / 32];
 #define LIDX(e) ((e) + ((e) >> 5))
+    const int tid = threadIdx.x;
+    const size_t lowmask = ((size_t)1 << b0) - 1;
+    const size_t gc0 = (size_t)blockIdx.x * C;
+    // address of (row, column c of this workgroup)
+    auto addr = [&](int row, int c) -> size_t {
+        size_t gc = gc0 + c;
+        if (BREV) gc = __brevll(gc) >> (64 - (nb - KB));
+        return ((gc >> b0) << (b0 + KB)) + ((size_t)row << b0) + (gc & lowmask);
+    };
+    // phase A: rows i * RA + rl, i < 16, of column c
+    int rl, c;
     if (b0 == 0) {
-        // columns are contiguous runs of R words: the tile is one contiguous span
-        const size_t base = c0 << kb;
-        for (int e = threadIdx.x; e < T; e += blockDim.x) t[LIDX(e)] = gl_canon(src[base + e]);
+        rl = tid % RA;
+        c = tid / RA;
     } else {
-        const size_t lowmask = ((size_t)1 << b0) - 1;
-        const size_t high = c0 >> b0, low0 = c0 & lowmask;
-        const size_t base = (high << (b0 + kb)) + low0;
-        for (int e = threadIdx.x; e < T; e += blockDim.x) {
-            int col = e & (BF_COLS - 1), row = e >> 4;
-            t[LIDX(row * BF_COLS + col)] = gl_canon(src[base + ((size_t)row << b0) + col]);
-        }
+        c = tid % C;
+        rl = tid / C;
     }
-    mam_barrier();
-    for (int j = 0; j < kb; j++) {
-        for (int p = threadIdx.x; p < T / 2; p += blockDim.x) {
-            int col, rp;
-            if (b0 == 0) {
-                col = p >> (kb - 1);
-                rp = p & ((R >> 1) - 1);
-            } else {
-                col = p & (BF_COLS - 1);
-                rp = p >> 4;
-            }
-            int rlo = ((rp >> j) << (j + 1)) | (rp & ((1 << j) - 1));
-            int rhi = rlo | (1 << j);
-            int elo, ehi;
-            if (b0 == 0) {
-                elo = col * R + rlo;
-                ehi = col * R + rhi;
-            } else {
-                elo = rlo * BF_COLS + col;
-                ehi = rhi * BF_COLS + col;
-            }
-            u64 lo = t[LIDX(elo)], hi = t[LIDX(ehi)];
-            bfly<OP>(lo, hi);
-            t[LIDX(elo)] = lo;
-            t[LIDX(ehi)] = hi;
+    u64 v[16];
+    if (BREV) {
+        // gather with consecutive lanes on consecutive source words (columns), then turn
+        // the tile through LDS into the phase-A layout and write bh from it
+        const int cc = tid % C, rr = tid / C;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int row = i * RA + rr;
+            const size_t sidx = ((size_t)(__brev((u32)row) >> (32 - KB)) << (nb - KB)) + gc0 + cc;
+            t[LIDX(row * C + cc)] = gl_canon(src[sidx]);
         }
         mam_barrier();
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            v[i] = t[LIDX((i * RA + rl) * C + c)];
+            side[addr(i * RA + rl, c)] = v[i];
+        }
+        mam_barrier();
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = gl_canon(src[addr(i * RA + rl, c)]);
     }
-    for (int r = 0; r < reps; r++) {
-        u64 *d = dst + r * rep_stride;
-        if (b0 == 0) {
-            const size_t base = c0 << kb;
-            for (int e = threadIdx.x; e < T; e += blockDim.x) d[base + e] = t[LIDX(e)];
-        } else {
-            const size_t lowmask = ((size_t)1 << b0) - 1;
-            const size_t high = c0 >> b0, low0 = c0 & lowmask;
-            const size_t base = (high << (b0 + kb)) + low0;
-            for (int e = threadIdx.x; e < T; e += blockDim.x) {
-                int col = e & (BF_COLS - 1), row = e >> 4;
-                d[base + ((size_t)row << b0) + col] = t[LIDX(row * BF_COLS + col)];
-            }
+#pragma unroll
+    for (int st = 0; st < 4; st++) {
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (!(i & (1 << st))) bfly<OP>(v[i], v[i | (1 << st)]);
+    }
+#pragma unroll
+    for (int i = 0; i < 16; i++) t[LIDX((i * RA + rl) * C + c)] = v[i];
+    mam_barrier();
+    // phase B: rows (rh << KBB) + rlow, rlow < 2^KBB, of columns cb * CB + cp, cp < CB;
+    // register k = rlow * CB + cp
+    int rh, cb;
+    if (b0 == 0) {
+        rh = tid % 16;
+        cb = tid / 16;
+    } else {
+        cb = tid % (C / CB);
+        rh = tid / (C / CB);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = t[LIDX((((rh << KBB) + k / CB)) * C + cb * CB + k % CB)];
+#pragma unroll
+    for (int st = 0; st < KBB; st++) {
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (!(k & (CB << st))) bfly<OP>(v[k], v[k | (CB << st)]);
+    }
+    if (b0 == 0) {
+        // columns are runs of rows: store from the phase-A layout, where consecutive lanes
+        // hold consecutive rows (in the phase-B layout each lane holds a run of its own)
+        mam_barrier();
+#pragma unroll
+        for (int k = 0; k < 16; k++) t[LIDX((((rh << KBB) + k / CB)) * C + cb * CB + k % CB)] = v[k];
+        mam_barrier();
+#pragma unroll
+        for (int i = 0; i < 16; i++) v[i] = t[LIDX((i * RA + rl) * C + c)];
+        for (int r = 0; r < reps; r++) {
+            u64 *d = dst + r * rep_stride;
+#pragma unroll
+            for (int i = 0; i < 16; i++) d[addr(i * RA + rl, c)] = v[i];
+        }
+    } else {
+        for (int r = 0; r < reps; r++) {
+            u64 *d = dst + r * rep_stride;
+#pragma unroll
+            for (int k = 0; k < 16; k++) d[addr((rh << KBB) + k / CB, cb * CB + k % CB)] = v[k];
         }
     }
 #undef LIDX
@@ -151,9 +193,12 @@ __global__ void k_bfly_small(const u64 *src, u64 *dst, int nbits, int op, int re
 }
 
 // Transform src (2^nbits words) into dst; the final result is also replicated `reps`
-// times at stride 2^nbits in dst. src may equal dst.
-static int bfly_transform(int op, const u64 *src, u64 *dst, int nbits, int reps, hipStream_t s) {
+// times at stride 2^nbits in dst. src may equal dst. brev_side (op BF_T, nbits > 12 only):
+// transform bitrev(src) instead and write bitrev(src) to brev_side on the way (the commit).
+static int bfly_transform(int op, const u64 *src, u64 *dst, int nbits, int reps, hipStream_t s,
+                          u64 *brev_side = nullptr) {
     const size_t n = (size_t)1 << nbits;
+    if (brev_side && (op != BF_T || nbits <= 12)) return -2;
     if (nbits <= 12) {
         k_bfly_small<<<1, 256, 0, s>>>(src, dst, nbits, op, reps, n);
         return hipGetLastError() == hipSuccess ? 0 : -1;
@@ -162,15 +207,28 @@ static int bfly_transform(int op, const u64 *src, u64 *dst, int nbits, int reps,
     const u64 *in = src;
     int passes = (nbits + BF_MAXKB - 1) / BF_MAXKB;
     while (b0 < nbits) {
-        // split the bits evenly over the remaining passes (each at most 8 bits)
+        // split the bits evenly over the remaining passes (each at most 8 bits; with
+        // nbits >= 13 every pass has 5 to 8, and each b0 after the first is at least 5)
         int kb = (nbits - b0 + passes - 1) / passes;
         passes--;
         bool last = b0 + kb == nbits;
-        unsigned grid = (unsigned)((n >> kb) / BF_COLS);
         int r = last ? reps : 1;
-        if (op == BF_MOBIUS) k_bfly_pass<BF_MOBIUS><<<grid, 256, 0, s>>>(in, dst, b0, kb, r, n);
-        else if (op == BF_WHT) k_bfly_pass<BF_WHT><<<grid, 256, 0, s>>>(in, dst, b0, kb, r, n);
-        else k_bfly_pass<BF_T><<<grid, 256, 0, s>>>(in, dst, b0, kb, r, n);
+        const unsigned grid = (unsigned)((n >> kb) >> (12 - kb));
+#define MAM_BF_REG(K)                                                                                        \
+    case K:                                                                                                   \
+        if (brev_side && b0 == 0) k_bfly_reg<BF_T, K, true><<<grid, 256, 0, s>>>(in, dst, b0, r, n, brev_side, nbits); \
+        else if (op == BF_MOBIUS) k_bfly_reg<BF_MOBIUS, K, false><<<grid, 256, 0, s>>>(in, dst, b0, r, n, nullptr, 0); \
+        else if (op == BF_WHT) k_bfly_reg<BF_WHT, K, false><<<grid, 256, 0, s>>>(in, dst, b0, r, n, nullptr, 0);      \
+        else k_bfly_reg<BF_T, K, false><<<grid, 256, 0, s>>>(in, dst, b0, r, n, nullptr, 0);                          \
+        break;
+        switch (kb) {
+            MAM_BF_REG(5)
+            MAM_BF_REG(6)
+            MAM_BF_REG(7)
+            MAM_BF_REG(8)
+            default: return -2;
+        }
+#undef MAM_BF_REG
         if (hipGetLastError() != hipSuccess) return -1;
         in = dst;
         b0 += kb;
@@ -277,6 +335,7 @@ MAM_API int basefold_encode_ffi(const u64 *d_coeffs, u64 *d_codeword, int num_va
 int mam_commit_codeword(const u64 *d_evals, u64 *d_bh_evals, u64 *d_codeword, int nv, int lr, hipStream_t s) {
     // algorithmic bytes: read evals, write bh_evals, write 2^lr codeword copies
     ProfScope prof(PROF_COMMIT_TRANSFORM, 8.0 * (double)((size_t)1 << nv) * (2.0 + (double)(1 << lr)), s);
+    if (nv > 12) return bfly_transform(BF_T, d_evals, d_codeword, nv, 1 << lr, s, d_bh_evals);
     if (mam_bitrev(d_evals, d_bh_evals, nv, 1, s)) return -1;
     return bfly_transform(BF_T, d_bh_evals, d_codeword, nv, 1 << lr, s);
 }

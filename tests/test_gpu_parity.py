@@ -160,7 +160,7 @@ def test_partial_eval(log_n, m):
     assert np.array_equal(d_data.to_numpy(1 << (log_n - m)), want)
 
 
-@pytest.mark.parametrize("nv,lr", [(1, 1), (4, 1), (10, 1), (12, 2), (13, 1), (16, 3), (17, 1)])
+@pytest.mark.parametrize("nv,lr", [(1, 1), (4, 1), (10, 1), (12, 2), (13, 1), (16, 3), (17, 1), (20, 1), (22, 1)])
 def test_commit_pieces(nv, lr):
     say("commit pieces", nv, lr)
     rng = np.random.default_rng(500 + nv)
