@@ -274,10 +274,23 @@ static int ilog2_sz(size_t n) {
 }
 
 // Levels written by one launch whose first level has n nodes: all remaining levels when
-// the whole rest fits one workgroup, else the levels that keep every wave full.
+// the whole rest fits one workgroup, else MAM_TREE_LPL levels (default 1), 1 below
+// MAM_TREE_LPL_MIN nodes. A launch that goes on compressing in LDS keeps only half, then a
+// quarter of its waves busy, and on the box two workgroups of a CU often put those waves on
+// the same SIMDs; one level per launch measured faster at every size (2^22 Basefold proof:
+// 3.21 ms with three levels per launch, 3.17 with two, 3.14 with one).
 static int levels_per_launch(size_t n) {
+    static const int lpl = [] {
+        const char *e = getenv("MAM_TREE_LPL");
+        const int v = e ? atoi(e) : 1;
+        return v >= 1 && v <= 3 ? v : 1;
+    }();
+    static const size_t lpl_min = [] {
+        const char *e = getenv("MAM_TREE_LPL_MIN");
+        return e ? (size_t)strtoull(e, nullptr, 10) : (size_t)0;
+    }();
     if (n <= (size_t)TREE_BLOCK) return ilog2_sz(n) + 1;
-    return 3;
+    return n < lpl_min ? 1 : lpl;
 }
 
 // oj (may be null): an opening round's fold + oracle job. It rides along the first launch
