@@ -581,8 +581,11 @@ def bench_hash(args, M):
 
 
 def merkle_perms(nv, lr):
-    """Monolith permutations of one commit (2^(nv+lr) - 1) plus one opening (sum of flen - 1)."""
-    total = (1 << (nv + lr)) - 1
+    """Monolith permutations executed by one commit plus one opening. The commitment's
+    codeword is 2^lr copies of one block, so its tree hashes one copy's subtree
+    (2^nv - 1 permutations) and lr levels above it; the opening's folded trees hash in full
+    (sum of flen - 1)."""
+    total = (1 << nv) - 1 + lr
     for r in range(nv - 1):
         total += (1 << (nv + lr - 1 - r)) - 1
     return total

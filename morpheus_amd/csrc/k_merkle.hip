@@ -267,6 +267,17 @@ static size_t narrow_min() {
 
 size_t mam_merkle_narrow_min() { return narrow_min(); }
 
+// Lane-parallel launches over more than one workgroup write at most this many levels
+// (MAM_WIDE_LPL, default 6: down to one node per workgroup).
+static int wide_lpl() {
+    static const int v = [] {
+        const char *e = getenv("MAM_WIDE_LPL");
+        const int x = e ? atoi(e) : 6;
+        return x >= 1 && x <= 6 ? x : 6;
+    }();
+    return v;
+}
+
 static int ilog2_sz(size_t n) {
     int l = 0;
     while ((size_t(1) << l) < n) l++;
@@ -328,6 +339,7 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
     while (done < total_levels) {
         if (can_wide && n < narrow_min()) {
             int nl = n <= (size_t)WIDE_GROUPS ? ilog2_sz(n) + 1 : ilog2_sz(WIDE_GROUPS) + 1;
+            if (n > (size_t)WIDE_GROUPS && nl > wide_lpl()) nl = wide_lpl();
             if (nl > total_levels - done) nl = total_levels - done;
             unsigned grid = (unsigned)((n + WIDE_GROUPS - 1) / WIDE_GROUPS);
             const int threads = WIDE_GROUPS * 16;
