@@ -72,9 +72,13 @@ int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u6
 // (open_fold.h, may be null): the round's eq / bh fold and oracle job, run along with the
 // tree's first launch where it can be, else launched before it. `mj` (may be null): a
 // mailbox post of mj->a's words followed by the root, made by the tree's final launch when
-// it can (*mail_posted true); otherwise the caller posts.
+// it can (*mail_posted true); otherwise the caller posts. `uj` (may be null): a product-form
+// round's bh fold and U sums (open_fold.h), run on extra workgroups of the tree's first
+// launch, which is a one-lane launch whenever the round is in product form (the result is in
+// uj->result before any later launch of the stream starts).
 struct OracleJob;
 struct MailJob;
+struct UJob;
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
                       size_t folded_len, int hash, hipStream_t s, const OracleJob *oj = nullptr,
-                      const MailJob *mj = nullptr, bool *mail_posted = nullptr);
+                      const MailJob *mj = nullptr, bool *mail_posted = nullptr, const UJob *uj = nullptr);

@@ -205,6 +205,14 @@ static void oracle_from_u(const u64 *U, ext2 sc, ext2 rk, u64 *o6) {
     }
 }
 
+static bool uj_ride() {
+    static const bool v = [] {
+        const char *e = getenv("MAM_UJ_RIDE");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
 int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript *t, int nq, u64 *out) {
     const int nv = S.nv, lr = S.lr;
     hipStream_t s = mam_stream();
@@ -299,8 +307,18 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         // the eq / bh fold with the next oracle, run along with the folded codeword's tree
         const OracleJob job = {eq_cur, bh_cur, ch, eqp[pp], bhp[pp], sc_size / 4, rs.partials,
                                rs.counter, rs.result, 0, bh_base ? 1 : 0};
-        if (prod) rc = mam_open_u(bh_cur, bh_base, true, ch, bhp[pp], sc_size / 4, S.pt + 2 * (round + 2), nv - round - 2,
-                                  rs, s);
+        // product form: the bh fold and U sums ride along the tree's first launch (or run on a
+        // launch of their own before it with MAM_UJ_RIDE=0)
+        const UJob ujob = {bh_cur, bhp[pp], ch, sc_size / 4, S.pt + 2 * (round + 2), nv - round - 2,
+                           rs.partials, rs.result, rs.counter};
+        if (prod && (nv - round - 2 > 30 || sc_size / 4 > ((size_t)1 << (nv - round - 2)))) {
+            rc = -2;
+            break;
+        }
+        const bool ride = prod && uj_ride();
+        if (prod && !ride)
+            rc = mam_open_u(bh_cur, bh_base, true, ch, bhp[pp], sc_size / 4, S.pt + 2 * (round + 2), nv - round - 2,
+                            rs, s);
         if (rc) break;
         eq_cur = eqp[pp];
         bh_cur = bhp[pp];
@@ -319,7 +337,7 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         const MailJob mail = {rs.result, nres, box, seq};
         bool posted = false;
         rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s, prod ? nullptr : &job,
-                               &mail, &posted);
+                               &mail, &posted, ride ? &ujob : nullptr);
         if (rc) break;
         const size_t leaves = flen / 2;
         if ((!posted && mam_mailbox_post(rs.result, nres, ftree[round] + mam_merkle_root_offset(leaves), 4, box, seq,

@@ -48,6 +48,7 @@ struct P2Hash {
 };
 
 static constexpr int TREE_BLOCK = 256;
+static constexpr unsigned UJ_BLOCKS = 256;  // workgroups of a product-form job riding along a tree launch
 
 // Source kinds for the first level a launch writes: codeword leaves (base or Ext2),
 // children nodes, or leaves of a codeword folded on the fly (base or Ext2 input).
@@ -63,13 +64,23 @@ struct FoldSrc {
 // (rep copies of them make the level) and starts at digest index `first_off` of `tree`;
 // each next level follows the previous one. Node k of a level is stored at k + c * n for
 // every copy c < rep, n being the level's computed node count.
+// ujb > 0 (folded-codeword launches only): the first ujb workgroups run the round's
+// product-form bh fold and U sums (uj, open_fold.h) instead of tree work; the tree part is
+// the rest of the grid.
 template <class H, int SRC>
 __global__ __launch_bounds__(TREE_BLOCK) __attribute__((amdgpu_waves_per_eu(4))) void k_tree_levels(const u64 *__restrict__ src, u64 *__restrict__ tree,
                                                             size_t first_off, size_t n_first, int nlev, FoldSrc fs,
-                                                            int rep) {
+                                                            int rep, UJob uj, unsigned ujb) {
     __shared__ u64 sh[TREE_BLOCK * 4];
     const int t = threadIdx.x;
-    const size_t i = (size_t)blockIdx.x * TREE_BLOCK + t;
+    if constexpr (SRC == SRC_FOLD_GL || SRC == SRC_FOLD_EXT2) {
+        if (blockIdx.x < ujb) {  // workgroup-uniform
+            open_u_run<SRC == SRC_FOLD_GL, true, true>(uj, blockIdx.x, ujb);
+            return;
+        }
+    }
+    const unsigned tb = blockIdx.x - ujb;
+    const size_t i = (size_t)tb * TREE_BLOCK + t;
     u64 d[4];
     if (i < n_first) {
         if (SRC == SRC_GL) {
@@ -121,7 +132,7 @@ __global__ __launch_bounds__(TREE_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
         mam_barrier();
         if (t < block_n) {
             for (int c = 0; c < rep; c++) {
-                u64 *o = tree + 4 * (level_off + (size_t)c * level_n + (size_t)blockIdx.x * (TREE_BLOCK >> l) + t);
+                u64 *o = tree + 4 * (level_off + (size_t)c * level_n + (size_t)tb * (TREE_BLOCK >> l) + t);
 #pragma unroll
                 for (int k = 0; k < 4; k++) o[k] = r[k];
             }
@@ -310,13 +321,20 @@ static int levels_per_launch(size_t n) {
 // after the root when that launch is one lane-parallel workgroup without the oracle job;
 // *mail_posted tells the caller whether it did. rep (a power of two): the leaves are rep
 // copies of the first num_leaves / rep (a replicated codeword); only that subtree is hashed.
+// uj (may be null, folded-codeword sources only): an opening round's product-form job, run on
+// extra workgroups of the first launch, which must be a one-lane launch.
 template <class H>
 static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_src, const FoldSrc &fs,
                       hipStream_t s, const OracleJob *oj = nullptr, const MailJob *mj = nullptr,
-                      bool *mail_posted = nullptr, int rep = 1) {
+                      bool *mail_posted = nullptr, int rep = 1, const UJob *uj = nullptr) {
     if (num_leaves == 0 || (num_leaves & (num_leaves - 1))) return -2;
     if (rep < 1 || (rep & (rep - 1)) || (size_t)rep > num_leaves) return -2;
-    if (rep > 1 && (oj || mj)) return -2;  // replicated trees are commitments: no opening jobs
+    if (rep > 1 && (oj || mj || uj)) return -2;  // replicated trees are commitments: no opening jobs
+    if (uj && (oj || (leaf_src != SRC_FOLD_GL && leaf_src != SRC_FOLD_EXT2) ||
+               (H::kWide && num_leaves < narrow_min())))
+        return -2;
+    const UJob no_uj = {};
+    const unsigned ujb = uj ? grid_stride(uj->pairs, TREE_BLOCK, UJ_BLOCKS) : 0;
     if (mail_posted) *mail_posted = false;
     const OracleJob none = {};
     const MailJob no_mail = {};
@@ -386,24 +404,25 @@ static int build_tree(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int leaf_
             double launch_perms = 0;
             for (int l = 0; l < nl; l++) launch_perms += (double)(n >> l);
             ProfScope leaf_prof(leaf_src == SRC_GL && H::kWide ? PROF_LEAF_LAUNCH : -1, launch_perms, s);
+            const UJob &u = uj ? *uj : no_uj;
             switch (leaf_src) {
                 case SRC_GL:
-                    k_tree_levels<H, SRC_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
+                    k_tree_levels<H, SRC_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep, no_uj, 0);
                     break;
                 case SRC_EXT2:
-                    k_tree_levels<H, SRC_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
+                    k_tree_levels<H, SRC_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep, no_uj, 0);
                     break;
                 case SRC_FOLD_GL:
-                    k_tree_levels<H, SRC_FOLD_GL><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
+                    k_tree_levels<H, SRC_FOLD_GL><<<grid + ujb, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep, u, ujb);
                     break;
                 default:
-                    k_tree_levels<H, SRC_FOLD_EXT2><<<grid, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep);
+                    k_tree_levels<H, SRC_FOLD_EXT2><<<grid + ujb, TREE_BLOCK, 0, s>>>(d_cw, d_tree, 0, n, nl, fs, rep, u, ujb);
                     break;
             }
         } else {
             // children: the computed part of the level that ends right before `off`
             const u64 *child = d_tree + 4 * (off - 2 * n * rep);
-            k_tree_levels<H, SRC_NODES><<<grid, TREE_BLOCK, 0, s>>>(child, d_tree, off, n, nl, fs, rep);
+            k_tree_levels<H, SRC_NODES><<<grid, TREE_BLOCK, 0, s>>>(child, d_tree, off, n, nl, fs, rep, no_uj, 0);
         }
         if (hipGetLastError() != hipSuccess) return -1;
         for (int l = 0; l < nl; l++) {
@@ -438,14 +457,15 @@ int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, 
 
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
                       size_t folded_len, int hash, hipStream_t s, const OracleJob *oj, const MailJob *mj,
-                      bool *mail_posted) {
+                      bool *mail_posted, const UJob *uj) {
     if (mail_posted) *mail_posted = false;
     if (folded_len < 2) return -2;
     FoldSrc fs = {table, alpha, fold_out};
     int src = base_cw ? SRC_FOLD_GL : SRC_FOLD_EXT2;
     if (hash == MAM_HASH_MONOLITH)
-        return build_tree<MonoHash>(cw, tree, folded_len / 2, src, fs, s, oj, mj, mail_posted);
-    if (hash == MAM_HASH_POSEIDON2) return build_tree<P2Hash>(cw, tree, folded_len / 2, src, fs, s, oj, mj, mail_posted);
+        return build_tree<MonoHash>(cw, tree, folded_len / 2, src, fs, s, oj, mj, mail_posted, 1, uj);
+    if (hash == MAM_HASH_POSEIDON2)
+        return build_tree<P2Hash>(cw, tree, folded_len / 2, src, fs, s, oj, mj, mail_posted, 1, uj);
     return -2;
 }
 

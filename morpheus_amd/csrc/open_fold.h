@@ -64,3 +64,102 @@ __device__ __forceinline__ void open_fold_oracle_run(const OracleJob &j, unsigne
 
 // The job as its own launch when it cannot ride along a tree launch (k_basefold.hip).
 int mam_open_fold_oracle_job(const OracleJob &j, hipStream_t s);
+
+// Product form of an opening round (single point): eq never stored. Folding eq at ch on its
+// lowest variable gives the eq table of the remaining variables times a scalar, so a round's
+// oracle needs only U0 = sum_y E(y) b(2y) and U1 = sum_y E(y) b(2y+1), E the eq table of the
+// variables above the pair bit; the host applies the scalars. E(y) is the product of one
+// entry per 6-bit group of y from 64-entry tables built in LDS by each workgroup.
+struct UJob {
+    const u64 *bh;
+    u64 *bh_out;
+    ext2 ch;
+    size_t pairs;
+    const u64 *r;  // Ext2 point words of E's variables
+    int nb;        // E's variables (bits of y), <= 30
+    u64 *partials, *result;
+    u32 *counter;  // ticket of the one-launch reduction (a job riding along a tree launch)
+};
+
+template <bool BH_BASE, bool FOLD, bool TICKET>
+__device__ __forceinline__ void open_u_run(const UJob &j, unsigned bid, unsigned nblocks) {
+    __shared__ u64 tab[5][64][2];
+    __shared__ u64 lds[4 * (MAM_BLOCK / WAVE)];
+    for (int e = threadIdx.x; e < 5 * 64; e += blockDim.x) {
+        const int g = e >> 6, m = e & 63;
+        ext2 v = e2_one();
+        for (int i = 0; i < 6 && 6 * g + i < j.nb; i++) {
+            const ext2 ri = e2_load(j.r, 6 * g + i);
+            v = e2_mul(v, ((m >> i) & 1) ? ri : e2_sub(e2_one(), ri));
+        }
+        tab[g][m][0] = v.c0;
+        tab[g][m][1] = v.c1;
+    }
+    mam_barrier();
+    const int ng = j.nb > 0 ? (j.nb + 5) / 6 : 1;
+    const ext2 ch = j.ch;
+    const u64 ch7 = gl_mul7(ch.c1);
+    // A wave takes chunks of 64 * st consecutive y (st steps of 64 lanes, st a power of two <= 16
+    // chosen so that every wave of the grid has a chunk). Within a chunk a lane's y keeps its
+    // low group (the lane) and its groups above the second (the chunk), so the lane adds
+    // T1[group 1] * b over the chunk and multiplies the sum by T0[lane] times the chunk's high
+    // factor once: at st = 16, about one product per input pair instead of four.
+    const int lane = threadIdx.x & 63;
+    const ext2 t0{tab[0][lane][0], tab[0][lane][1]};
+    const size_t nwaves = (size_t)nblocks * (blockDim.x >> 6);
+    const size_t wave = (size_t)bid * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    int lst = 0;  // log2 st
+    while (lst < 4 && ((j.pairs >> (6 + lst + 1)) >= nwaves)) lst++;
+    const int lc = 6 + lst;  // log2 of the chunk length
+    const size_t chunks = (j.pairs + ((size_t)1 << lc) - 1) >> lc;
+    ext2 s0 = e2_zero(), s1 = e2_zero();
+    for (size_t c = wave; c < chunks; c += nwaves) {
+        ext2 c0 = e2_zero(), c1 = e2_zero();
+        for (int i = 0; i < (1 << lst); i++) {
+            const size_t y = (c << lc) + ((size_t)i << 6) + lane;
+            if (y >= j.pairs) break;
+            ext2 b0, b1;
+            if (FOLD) {
+                if (BH_BASE) {
+                    const u64 a0 = gl_canon(j.bh[4 * y]), a1 = gl_canon(j.bh[4 * y + 1]);
+                    const u64 a2 = gl_canon(j.bh[4 * y + 2]), a3 = gl_canon(j.bh[4 * y + 3]);
+                    b0 = e2_add(e2_base(a0), e2_scale(gl_sub(a1, a0), ch));
+                    b1 = e2_add(e2_base(a2), e2_scale(gl_sub(a3, a2), ch));
+                } else {
+                    const ext2 a0 = e2_load(j.bh, 4 * y), a1 = e2_load(j.bh, 4 * y + 1);
+                    const ext2 a2 = e2_load(j.bh, 4 * y + 2), a3 = e2_load(j.bh, 4 * y + 3);
+                    b0 = e2_add(a0, e2_mul_fixed(ch, ch7, e2_sub(a1, a0)));
+                    b1 = e2_add(a2, e2_mul_fixed(ch, ch7, e2_sub(a3, a2)));
+                }
+                e2_store(j.bh_out, 2 * y, b0);
+                e2_store(j.bh_out, 2 * y + 1, b1);
+            } else {
+                b0 = BH_BASE ? e2_base(gl_canon(j.bh[2 * y])) : e2_load(j.bh, 2 * y);
+                b1 = BH_BASE ? e2_base(gl_canon(j.bh[2 * y + 1])) : e2_load(j.bh, 2 * y + 1);
+            }
+            const size_t m1 = (y >> 6) & 63;
+            const ext2 t1{tab[1][m1][0], tab[1][m1][1]};
+            if (BH_BASE && !FOLD) {
+                c0 = e2_add(c0, e2_scale(b0.c0, t1));
+                c1 = e2_add(c1, e2_scale(b1.c0, t1));
+            } else {
+                c0 = e2_add(c0, e2_mul(t1, b0));
+                c1 = e2_add(c1, e2_mul(t1, b1));
+            }
+        }
+        ext2 H = t0;
+        for (int g = 2; g < ng; g++) {
+            const size_t m = ((c << lc) >> (6 * g)) & 63;
+            H = e2_mul(H, ext2{tab[g][m][0], tab[g][m][1]});
+        }
+        s0 = e2_add(s0, e2_mul(H, c0));
+        s1 = e2_add(s1, e2_mul(H, c1));
+    }
+    u64 acc[4] = {s0.c0, s0.c1, s1.c0, s1.c1};
+    block_sum<4>(acc, lds);
+    if (TICKET)
+        grid_finish_n<4>(acc, lds, j.partials, j.counter, j.result, bid, nblocks);
+    else
+        grid_partials<4>(acc, j.partials, j.result);
+}
+
