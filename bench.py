@@ -66,14 +66,15 @@ ALG_OPS_PER_PERM = float(sum(MONO_ALG_OPS.values()))
 # VALU lane-operations the leaf kernel executes per Monolith permutation: SQ_INSTS_VALU x 64
 # over the permutations of one k_tree_levels<MonoHash, SRC_GL> dispatch (rocprofv3 --pmc,
 # profiles/merkle_leaf_kernel_2p22.md). Reported beside the algorithmic count.
-VALU_OPS_PER_PERM = 4982.0
+VALU_OPS_PER_PERM = 4979.0
 # VALU instructions one lane executes for one full Monolith permutation with every lane
 # active: SQ_INSTS_VALU / SQ_WAVES of k_mono_permute_n (rocprofv3 --pmc,
 # profiles/monolith_permutation_valu.md). Lane-operations/s = permutations/s x this.
 MONO_PERM_VALU = 5291.0
-# HBM bytes of that dispatch at 2^22 leaves: FETCH_SIZE x 2 (a 256 MiB copy reads back as
+# HBM bytes of that dispatch in the 2^22 proof (2^21 leaves: one copy of the replicated
+# codeword, digests stored at both copies): FETCH_SIZE x 2 (a 256 MiB copy reads back as
 # 128 MiB on this counter) + WRITE_SIZE (exact for the copy), same profile file.
-LEAF_LAUNCH_TRAFFIC_BYTES = (65.2 + 224.0) * 2 ** 20
+LEAF_LAUNCH_TRAFFIC_BYTES = (32.6 + 128.0) * 2 ** 20
 # HBM bytes of one int8 Ajtai GEMM launch (witness 2^22, 256 rows, batch 1): FETCH_SIZE
 # 540758 KiB x 2.0 (copy calibration) + WRITE_SIZE 8192 KiB, from the rocprofv3 --pmc runs.
 AJTAI_GEMM_TRAFFIC_BYTES = (540758.0 * 2.0 + 8192.0) * 1024
@@ -385,14 +386,15 @@ def bench_basefold(args, M):
                 "roofline": None,
                 "merkle": {"perms_per_s": mk_perm / (mk_ms / 1e3) if mk_ms > 0 else 0.0,
                            "avg_build_ms": mk_ms / max(mk_n, 1), "builds_per_step": mk_n / 3}}
-    # Dominant kernel: the commit tree's first launch, k_tree_levels<MonoHash, SRC_GL> hashing
-    # 2^22 base-field leaves and two in-LDS levels (7 * 2^20 Monolith permutations).
+    # Dominant kernel: the commit tree's leaf launch, k_tree_levels<MonoHash, SRC_GL> hashing the
+    # 2^21 base-field leaves of one copy of the replicated codeword (2^21 Monolith permutations).
     lk_ms, lk_n, lk_perm = prof[4]
     lk_avg_ms = lk_ms / max(lk_n, 1)
     perms_per_launch = lk_perm / max(lk_n, 1)
     perm_rate = perms_per_launch / (lk_avg_ms / 1e3) if lk_ms > 0 else 0.0
     ach = perm_rate * ALG_OPS_PER_PERM / 1e12
-    roof = {"kernel": "k_tree_levels<MonoHash, SRC_GL> (2^22 leaves + 2 in-LDS levels)",
+    roof = {"kernel": f"k_tree_levels<MonoHash, SRC_GL> (the commitment's leaf level: 2^{nv - 1} leaves of one "
+                      f"codeword copy)",
             "bound": "valu", "unit": "Tops/s", "achieved": ach, "peak": VALU_NOMINAL_TOPS,
             "frac": ach / VALU_NOMINAL_TOPS,
             "peak_source": "MI355X FP32 vector 157.3 TFLOP/s / 4 (no packing, no FMA): one 32-bit lane-op per "
