@@ -81,10 +81,18 @@ struct QuerySrc {
     int nv, lr;
 };
 
+// Query indices passed in the kernel's arguments (no copy of their own), up to 128 queries.
+static constexpr int QIDX_INLINE = 128;
+struct QueryIdx {
+    u64 v[QIDX_INLINE];
+};
+
 // One workgroup per query: gathers values and authentication paths into the flat layout.
-__global__ void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx, u64 *__restrict__ out, long qlen) {
+// The indices come from `idx` in device memory, or from qi_inline when idx is null.
+__global__ void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx, u64 *__restrict__ out, long qlen,
+                                 QueryIdx qi_inline) {
     const int qi = blockIdx.x;
-    const u64 leaf = idx[qi];
+    const u64 leaf = idx ? idx[qi] : qi_inline.v[qi];
     u64 *o = out + (size_t)qi * qlen;
     const int nv = q.nv, lr = q.lr;
     if (threadIdx.x == 0) o[0] = leaf;
@@ -259,12 +267,27 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     RedScratch rs = {red, (u32 *)cnt, red + RED_BLOCKS * 6};
     if (!rc && hipMemsetAsync(cnt, 0, 64, s) != hipSuccess) rc = -1;
     auto point_var = [&](int k) { return ext2{gl_canon(S.pt_host[2 * k]), gl_canon(S.pt_host[2 * k + 1])}; };
+    // The rounds from K on ride along tree launches and fold an eq table. Their table is
+    // scale_K * eq(r_K, ..., r_{nv-1}; .); the unscaled table depends on the point only and is
+    // queued behind the first oracle's post, so the device builds it while the host runs its
+    // first transcript step, and the host multiplies those rounds' oracles (linear in eq) by
+    // scale_K.
+    const bool late_eq = !S.eq && S.K < nv - 1;
+    ext2 kscale = e2_one();
     if (S.eq) {
         if (!rc) rc = mam_open_oracle0(S.eq, S.bh, S.bh_base, n / 2, rs, s);
         if (!rc) rc = read_back(h, rs.result, 6, s);
     } else {
         if (!rc) rc = mam_open_u(S.bh, S.bh_base, false, e2_zero(), nullptr, n / 2, S.pt + 2, nv - 1, rs, s);
-        if (!rc) rc = read_back(h, rs.result, 4, s);
+        u64 seq0 = 0;
+        u64 *box0 = rc ? nullptr : mam_mailbox_arm(&seq0);
+        if (!rc && !box0) rc = -1;
+        if (!rc) rc = mam_mailbox_post(rs.result, 4, nullptr, 0, box0, seq0, s);
+        if (!rc && late_eq) {
+            const ext2 rk = point_var(S.K);
+            rc = mam_eq_table_scaled(S.pt + 2 * S.K, nv - S.K, e2_sub(e2_one(), rk), rk, eqp[(S.K & 1) ^ 1], s);
+        }
+        if (!rc) rc = mam_mailbox_wait(box0, seq0, 4, h, s);
         if (!rc) oracle_from_u(h, e2_one(), point_var(0), h);
     }
     u64 *o_eval = out;
@@ -297,10 +320,8 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
             const ext2 rr = point_var(round);
             next_scale = e2_mul(scale, e2_add(e2_mul(e2_sub(e2_one(), rr), e2_sub(e2_one(), ch)), e2_mul(rr, ch)));
             if (round == S.K) {
-                // the rounds from here ride along tree launches: materialize the scaled eq table
-                rc = mam_eq_table_scaled(S.pt + 2 * round, nv - round, e2_mul(scale, e2_sub(e2_one(), rr)),
-                                         e2_mul(scale, rr), eqp[pp ^ 1], s);
-                if (rc) break;
+                // the rounds from here ride along tree launches on the eq table built above
+                kscale = scale;
                 eq_cur = eqp[pp ^ 1];
             }
         }
@@ -347,8 +368,17 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
             break;
         }
         u64 *o = o_oracles + 6L * (round + 1);
-        if (prod) oracle_from_u(h, next_scale, point_var(round + 1), o);
-        else memcpy(o, h, 6 * sizeof(u64));
+        if (prod) {
+            oracle_from_u(h, next_scale, point_var(round + 1), o);
+        } else if (late_eq) {
+            for (int k = 0; k < 3; k++) {
+                const ext2 c = e2_mul(kscale, ext2{gl_canon(h[2 * k]), gl_canon(h[2 * k + 1])});
+                o[2 * k] = c.c0;
+                o[2 * k + 1] = c.c1;
+            }
+        } else {
+            memcpy(o, h, 6 * sizeof(u64));
+        }
         for (int k = 0; k < 3; k++) observe_ext2(t, o + 2 * k);
         memcpy(o_roots + 4L * round, h + nres, 4 * sizeof(u64));
         scale = next_scale;
@@ -382,10 +412,14 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         }
         qs.nv = nv;
         qs.lr = lr;
-        if (!rc && hipMemcpyAsync(d_idx, idx.data(), nq * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
+        QueryIdx qin;
+        const bool inl = nq <= QIDX_INLINE;
+        if (inl)
+            for (int q = 0; q < nq; q++) qin.v[q] = idx[q];
+        else if (!rc && hipMemcpyAsync(d_idx, idx.data(), nq * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
             rc = -1;
         if (!rc) {
-            k_gather_queries<<<nq, 64, 0, s>>>(qs, d_idx, d_out, qlen);
+            k_gather_queries<<<nq, 64, 0, s>>>(qs, inl ? nullptr : d_idx, d_out, qlen, qin);
             if (hipGetLastError() != hipSuccess) rc = -1;
         }
         if (!rc && hipMemcpyAsync(o_queries, d_out, (size_t)nq * qlen * sizeof(u64), hipMemcpyDeviceToHost, s) !=
