@@ -387,7 +387,7 @@ MAM_API int mam_release_scratch() {
 // fine-grained pinned host memory and then sets a sequence flag with a system-scope
 // release store; the host spins on the flag instead of a copy + stream synchronization.
 // ---------------------------------------------------------------------------
-__global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq) {
+__global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq, GateJob g) {
     const int t = threadIdx.x;
     if (t < na)
         box[t] = a[t];
@@ -396,18 +396,23 @@ __global__ void k_publish(const u64 *a, int na, const u64 *b, int nb, u64 *box, 
     __threadfence_system();
     mam_barrier();
     if (t == 0) __hip_atomic_store(&box[MAM_MAILBOX_WORDS], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    mam_gate_wait(g);
 }
 
+__global__ void k_gate(GateJob g) { mam_gate_wait(g); }
+
 // ---------------------------------------------------------------------------
-// Gates for pre-queued rounds (common.h). Per thread: pinned words [flag | 2 slots of 8]
-// and a device copy of each slot; slot seq & 1 serves release seq, so the host never
-// rewrites a slot that a queued copy may still read.
+// Gates for pre-queued rounds (common.h). Per thread: pinned words [flag | err | 6 unused |
+// 2 slots of 8] and a device copy of each slot; slot seq & 1 serves release seq, so with at
+// most two gates armed ahead of the release the host never rewrites a slot that a waiting
+// gate may still read.
 // ---------------------------------------------------------------------------
 namespace {
 struct Gate {
-    u64 *pin = nullptr;  // [0]: released number; [8 + 8 k]: slot k
+    u64 *pin = nullptr;  // [0]: released number; [1]: spin gave up; [8 + 8 k]: slot k
     u64 *dev = nullptr;  // 2 x 8 words
-    u64 armed = 0;       // last number queued
+    u64 armed = 0;       // last number armed
+    u64 limit = 0;       // spin limit in wall-clock ticks
     bool ok = false;
     ~Gate() {
         if (pin) (void)hipHostFree(pin);
@@ -419,6 +424,10 @@ thread_local int t_prequeue = -1;  // -1: not set on this thread (the environmen
 
 bool gate_ready() {
     if (t_gate.ok) return true;
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) != hipSuccess ||
+        khz <= 0)
+        return false;
     if (hipHostMalloc((void **)&t_gate.pin, 24 * sizeof(u64), hipHostMallocCoherent) != hipSuccess) {
         t_gate.pin = nullptr;
         return false;
@@ -431,6 +440,7 @@ bool gate_ready() {
     }
     memset(t_gate.pin, 0, 24 * sizeof(u64));
     t_gate.armed = 0;
+    t_gate.limit = (u64)(MAM_GATE_LIMIT_S * 1000.0 * khz);
     t_gate.ok = true;
     return true;
 }
@@ -447,19 +457,31 @@ bool mam_prequeue_enabled() {
 
 void mam_prequeue_set(bool on) { t_prequeue = on ? 1 : 0; }
 
-const u64 *mam_gate_queue(hipStream_t s, int nwords, u64 *seq) {
-    if (nwords < 1 || nwords > 8 || !gate_ready()) return nullptr;
-    const u64 n = t_gate.armed + 1;
+bool mam_gate_ready() { return gate_ready(); }
+
+const u64 *mam_gate_arm(int nwords, GateJob *g, u64 *seq) {
+    if (nwords < 1 || nwords > 8 || !g || !gate_ready()) return nullptr;
+    const u64 n = ++t_gate.armed;
     const int k = (int)(n & 1);
-    if (hipStreamWaitValue64(s, t_gate.pin, n, hipStreamWaitValueGte, ~(u64)0) != hipSuccess) return nullptr;
-    t_gate.armed = n;
-    if (hipMemcpyAsync(t_gate.dev + 8 * k, t_gate.pin + 8 + 8 * k, nwords * sizeof(u64), hipMemcpyHostToDevice, s) !=
-        hipSuccess) {
+    *g = GateJob{t_gate.pin, n, t_gate.pin + 8 + 8 * k, t_gate.dev + 8 * k, nwords, t_gate.pin + 1, t_gate.limit};
+    *seq = n;
+    return g->dst;
+}
+
+int mam_gate_launch(const GateJob &g, hipStream_t s) {
+    k_gate<<<1, 64, 0, s>>>(g);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+const u64 *mam_gate_queue(hipStream_t s, int nwords, u64 *seq) {
+    GateJob g;
+    const u64 *d = mam_gate_arm(nwords, &g, seq);
+    if (!d) return nullptr;
+    if (mam_gate_launch(g, s)) {
         mam_gate_open_all();
         return nullptr;
     }
-    *seq = n;
-    return t_gate.dev + 8 * k;
+    return d;
 }
 
 void mam_gate_open(u64 seq, const u64 *vals, int nwords) {
@@ -471,6 +493,11 @@ void mam_gate_open(u64 seq, const u64 *vals, int nwords) {
 
 void mam_gate_open_all() {
     if (t_gate.ok) __atomic_store_n(&t_gate.pin[0], t_gate.armed, __ATOMIC_RELEASE);
+}
+
+bool mam_gate_take_failure() {
+    if (!t_gate.ok) return false;
+    return __atomic_exchange_n(&t_gate.pin[1], (u64)0, __ATOMIC_ACQ_REL) != 0;
 }
 
 u64 *mam_mailbox_arm(u64 *seq) {
@@ -487,9 +514,10 @@ u64 *mam_mailbox_arm(u64 *seq) {
     return t_arenas.mailbox;
 }
 
-int mam_mailbox_post(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq, hipStream_t s) {
+int mam_mailbox_post(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq, hipStream_t s,
+                     const GateJob *g) {
     if (na < 0 || nb < 0 || na + nb > MAM_MAILBOX_WORDS || !box) return -2;
-    k_publish<<<1, 64, 0, s>>>(a, na, b, nb, box, seq);
+    k_publish<<<1, 64, 0, s>>>(a, na, b, nb, box, seq, g ? *g : GateJob{});
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

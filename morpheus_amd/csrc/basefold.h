@@ -50,7 +50,9 @@ MAM_HD void mam_oracle_accumulate_ends(u64 *acc, ext2 e0, ext2 e1, ext2 b0, ext2
 
 int mam_bitrev(const u64 *src, u64 *dst, int nb, int words, hipStream_t s);
 int mam_commit_codeword(const u64 *d_evals, u64 *d_bh_evals, u64 *d_codeword, int nv, int lr, hipStream_t s);
-int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s);
+// alpha_p (optional): the challenge is read from device memory there instead of alpha.
+int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s,
+                const u64 *alpha_p = nullptr);
 int mam_eq_table(const u64 *d_r, int nv, u64 *d_out, bool ext, hipStream_t s);
 // s * eq(r, x) over nv variables (Ext2 table): variable 0 contributes s (1 - r_0) / s r_0.
 int mam_eq_table_scaled(const u64 *d_r, int nv, ext2 s0, ext2 s1, u64 *d_out, hipStream_t s);
@@ -60,9 +62,10 @@ size_t mam_merkle_narrow_min();
 // One opening round in product form (single evaluation point): with FOLD, fold bh (4y .. 4y+3
 // -> folded pair y, written to bh_out) at ch; then U0 = sum_y E(y) b(2y), U1 = sum_y E(y)
 // b(2y+1) over `pairs` pairs y, E(y) = eq(r, y) over nb variables (r: device Ext2 point
-// words). result: U0, U1 (4 words); partials: RedScratch partials.
+// words). result: U0, U1 (4 words); partials: RedScratch partials. chp (optional): ch is read
+// from device memory there.
 int mam_open_u(const u64 *bh, bool bh_base, bool fold, ext2 ch, u64 *bh_out, size_t pairs, const u64 *r, int nb,
-               const RedScratch &rs, hipStream_t s);
+               const RedScratch &rs, hipStream_t s, const u64 *chp = nullptr);
 int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, const RedScratch &rs,
                      hipStream_t s);
 int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,
@@ -75,10 +78,12 @@ int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u6
 // it can (*mail_posted true); otherwise the caller posts. `uj` (may be null): a product-form
 // round's bh fold and U sums (open_fold.h), run on extra workgroups of the tree's first
 // launch, which is a one-lane launch whenever the round is in product form (the result is in
-// uj->result before any later launch of the stream starts).
+// uj->result before any later launch of the stream starts). alpha_p (may be null): the fold
+// reads alpha from device memory there (a value written on the device after the launch).
 struct OracleJob;
 struct MailJob;
 struct UJob;
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
                       size_t folded_len, int hash, hipStream_t s, const OracleJob *oj = nullptr,
-                      const MailJob *mj = nullptr, bool *mail_posted = nullptr, const UJob *uj = nullptr);
+                      const MailJob *mj = nullptr, bool *mail_posted = nullptr, const UJob *uj = nullptr,
+                      const u64 *alpha_p = nullptr);

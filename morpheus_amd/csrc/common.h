@@ -86,7 +86,10 @@ int mam_publish_wait(const u64 *a, int na, const u64 *b, int nb, u64 *out, hipSt
 // copy kernel; a kernel may instead write the words itself and call mam_mailbox_raise;
 // wait spins until the value arrived and copies n words.
 u64 *mam_mailbox_arm(u64 *seq);
-int mam_mailbox_post(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq, hipStream_t s);
+struct GateJob;
+// g (may be null): a gate the posting kernel waits for after the post.
+int mam_mailbox_post(const u64 *a, int na, const u64 *b, int nb, u64 *box, u64 seq, hipStream_t s,
+                     const GateJob *g = nullptr);
 int mam_mailbox_wait(const u64 *box, u64 seq, int n, u64 *out, hipStream_t s);
 
 // Device side of a post by one workgroup whose threads have written the mailbox words:
@@ -100,21 +103,54 @@ __device__ __forceinline__ void mam_mailbox_raise(u64 *box, u64 seq) {
 // Pre-queued rounds. A host-driven proving round needs values the host computes from the
 // previous round's result (Fiat-Shamir challenges). Launching the round's kernels only after
 // that costs an idle-queue launch (~13 us until the first kernel runs on the box); queuing
-// them beforehand behind a wait on a pinned flag costs ~6 us from the host's release to the
-// kernel's result. The gate of the calling thread: mam_gate_queue queues on stream s a wait
-// for the next release and a copy of nwords (<= 8) scalars from the pinned slot into device
-// memory, and returns that device address (the queued kernels read their scalars there) and
-// the release number; mam_gate_open writes the scalars and releases. Every queued gate must
-// be opened (mam_gate_open_all on error paths), or the stream stays blocked.
+// them beforehand behind a gate costs only the host's turnaround. A gate is one workgroup's
+// bounded spin on a pinned flag followed by a copy of up to 8 scalars from a pinned slot into
+// device memory, where the queued kernels read them: a launch of its own (mam_gate_launch,
+// or mam_gate_queue = arm + launch), or the tail of the kernel that posts the previous
+// round's result (mam_mailbox_post and the lane-parallel tree's post take a GateJob), which
+// saves the gate's own dispatch. mam_gate_arm reserves the calling thread's next gate and
+// returns the device address of its scalars; mam_gate_open writes the scalars and releases.
+// Gates are released in the order they were armed, at most two armed and not yet released.
+// Every armed gate must be opened (mam_gate_open_all on error paths); a spin that has not
+// seen its release after MAM_GATE_LIMIT_S seconds gives up (the stream then runs on with
+// stale scalars) and records it: mam_gate_take_failure() reports and clears that.
 // A stream waiting at a gate also holds up the other streams that share its hardware queue,
 // so threads that prove side by side with others (the worker pools) do not pre-queue:
 // mam_prequeue_enabled() is false on them (mam_prequeue_set), and MAM_PREQUEUE=0 turns it
 // off everywhere.
+static constexpr double MAM_GATE_LIMIT_S = 2.0;
+struct GateJob {
+    const u64 *flag;  // pinned: released number (nullptr: no gate)
+    u64 seq;
+    const u64 *slot;  // pinned scalars
+    u64 *dst;         // device copy read by the gated kernels
+    int n;
+    u64 *err;    // pinned: set to 1 when the spin gave up
+    u64 limit;   // spin limit in wall-clock ticks
+};
 bool mam_prequeue_enabled();
 void mam_prequeue_set(bool on);
+bool mam_gate_ready();  // the calling thread's gate memory is set up (false: cannot pre-queue)
+const u64 *mam_gate_arm(int nwords, GateJob *g, u64 *seq);
+int mam_gate_launch(const GateJob &g, hipStream_t s);
 const u64 *mam_gate_queue(hipStream_t s, int nwords, u64 *seq);
 void mam_gate_open(u64 seq, const u64 *vals, int nwords);
 void mam_gate_open_all();
+bool mam_gate_take_failure();
+
+// Device side of a gate: one thread spins on the release, then copies the scalars.
+__device__ __forceinline__ void mam_gate_wait(const GateJob &g) {
+    if (!g.flag || threadIdx.x != 0) return;
+    const u64 t0 = (u64)wall_clock64();
+    while (__hip_atomic_load(g.flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) < g.seq) {
+        if ((u64)wall_clock64() - t0 > g.limit) {
+            __hip_atomic_store(g.err, (u64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+    }
+    for (int i = 0; i < g.n; i++) g.dst[i] = __hip_atomic_load(&g.slot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 // A mailbox post handed to a kernel: na words from a, then the kernel's own words.
 struct MailJob {
@@ -122,4 +158,5 @@ struct MailJob {
     int na;
     u64 *box;  // nullptr: no post
     u64 seq;
+    GateJob gate;  // the next round's gate, waited for after the post (gate.flag nullptr: none)
 };

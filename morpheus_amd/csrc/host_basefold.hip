@@ -79,6 +79,9 @@ struct QuerySrc {
     const u64 *fcw[MAX_LEVELS];        // folded codewords 0..nv-2 (Ext2)
     const u64 *ftree[MAX_LEVELS];      // their trees
     int nv, lr;
+    const u64 *fin;  // final codeword (fin_words), copied to fin_out by workgroup 0
+    u64 *fin_out;
+    int fin_words;
 };
 
 // Query indices passed in the kernel's arguments (no copy of their own), up to 128 queries.
@@ -96,6 +99,8 @@ __global__ void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx, u64 *_
     u64 *o = out + (size_t)qi * qlen;
     const int nv = q.nv, lr = q.lr;
     if (threadIdx.x == 0) o[0] = leaf;
+    if (qi == 0)
+        for (int w = threadIdx.x; w < q.fin_words; w += blockDim.x) q.fin_out[w] = q.fin[w];
     u64 *vals = o + 1;
     // values: round 0 from the base codeword, rounds 1..nv-1 from folded codewords
     for (int r = threadIdx.x; r < nv; r += blockDim.x) {
@@ -231,7 +236,7 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
     {
         need += 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) + words(8) + words(nq) +
-                words((size_t)nq * qlen);
+                words(2 * ((size_t)1 << lr) + (size_t)nq * qlen);
         size_t flen = cw_len / 2;
         for (int r = 0; r < nv; r++) {
             need += words(2 * flen);
@@ -274,27 +279,99 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     // scale_K.
     const bool late_eq = !S.eq && S.K < nv - 1;
     ext2 kscale = e2_one();
-    if (S.eq) {
-        if (!rc) rc = mam_open_oracle0(S.eq, S.bh, S.bh_base, n / 2, rs, s);
-        if (!rc) rc = read_back(h, rs.result, 6, s);
-    } else {
-        if (!rc) rc = mam_open_u(S.bh, S.bh_base, false, e2_zero(), nullptr, n / 2, S.pt + 2, nv - 1, rs, s);
+    // Pre-queued rounds (common.h gates): round r's launches, which read their challenge from
+    // gate r's device slot, are queued before the host waits for round r - 1's results, and the
+    // kernel that posts round r - 1's results waits for gate r before it ends, so the device
+    // starts the round as soon as the host releases the challenge.
+    const bool pq = mam_prequeue_enabled() && mam_gate_ready();
+    const bool ride_all = uj_ride();
+    struct Queued {
+        u64 gate = 0, seq = 0;
+        u64 *box = nullptr;
+        GateJob g = {};
+        const u64 *chp = nullptr;
+    };
+    std::vector<Queued> qd(nv);  // rounds 0 .. nv-2 and the final fold (nv - 1)
+    // The device work of round r depends on r alone (and on the challenge, by value or at chp).
+    // next: the gate the round's post waits for (null: none)
+    auto queue_round = [&](int round, ext2 ch, const u64 *chp, const GateJob *next) -> int {
+        const int pp = round & 1;
+        const bool prod = !S.eq && round < S.K;
+        const u64 *eq_in = round == 0 ? S.eq : eqp[pp ^ 1];  // round K: the late eq table
+        const u64 *bh_in = round == 0 ? S.bh : bhp[pp ^ 1];
+        const bool bhb = round == 0 && S.bh_base;
+        const size_t scs = n >> round, fl = (cw_len / 2) >> round;
+        const u64 *cwi = round == 0 ? S.cw : fcw[round - 1];
+        const bool cwb = round == 0 && S.cw_base;
+        const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, round);
+        if (round == nv - 1) return mam_cw_fold(cwi, lvl, ch, fcw[nv - 1], fl, cwb, s, chp);  // last fold, no tree
+        if (prod && (nv - round - 2 > 30 || scs / 4 > ((size_t)1 << (nv - round - 2)))) return -2;
+        // the eq / bh fold with the next oracle, run along with the folded codeword's tree
+        const OracleJob job = {eq_in, bh_in, ch, eqp[pp], bhp[pp], scs / 4, rs.partials, rs.counter, rs.result, 0,
+                               bhb ? 1 : 0, chp};
+        // product form: the bh fold and U sums ride along the tree's first launch (or run on a
+        // launch of their own before it with MAM_UJ_RIDE=0)
+        const UJob ujob = {bh_in, bhp[pp], ch, scs / 4, S.pt + 2 * (round + 2), nv - round - 2, rs.partials,
+                           rs.result, rs.counter, chp};
+        const bool ride = prod && ride_all;
+        if (prod && !ride) {
+            const int e = mam_open_u(bh_in, bhb, true, ch, bhp[pp], scs / 4, S.pt + 2 * (round + 2), nv - round - 2,
+                                     rs, s, chp);
+            if (e) return e;
+        }
+        // the oracle (3 Ext2) and the new root come back in one mailbox round trip, posted by
+        // the tree's final launch where it can
+        Queued &q = qd[round];
+        q.box = mam_mailbox_arm(&q.seq);
+        if (!q.box) return -1;
+        const int nres = prod ? 4 : 6;  // U0, U1 or the oracle
+        const MailJob mail = {rs.result, nres, q.box, q.seq, next ? *next : GateJob{}};
+        bool posted = false;
+        const int e = mam_fold_and_tree(cwi, cwb, lvl, ch, fcw[round], ftree[round], fl, S.hash, s,
+                                        prod ? nullptr : &job, &mail, &posted, ride ? &ujob : nullptr, chp);
+        if (e) return e;
+        if (!posted &&
+            mam_mailbox_post(rs.result, nres, ftree[round] + mam_merkle_root_offset(fl / 2), 4, q.box, q.seq, s,
+                             next))
+            return -1;
+        return 0;
+    };
+    auto arm = [&](int round) -> bool {
+        qd[round].chp = mam_gate_arm(2, &qd[round].g, &qd[round].gate);
+        return qd[round].chp != nullptr;
+    };
+    // pre-queue round r (behind gate r, armed before): a tree round's post carries gate r + 1
+    auto queue_gated = [&](int round) -> int {
+        if (round == nv - 1) return queue_round(round, e2_zero(), qd[round].chp, nullptr);
+        if (!arm(round + 1)) return -1;
+        return queue_round(round, e2_zero(), qd[round].chp, &qd[round + 1].g);
+    };
+    auto release = [&](int round, ext2 ch) {
+        const u64 v[2] = {ch.c0, ch.c1};
+        mam_gate_open(qd[round].gate, v, 2);
+    };
+    {
         u64 seq0 = 0;
         u64 *box0 = rc ? nullptr : mam_mailbox_arm(&seq0);
         if (!rc && !box0) rc = -1;
-        if (!rc) rc = mam_mailbox_post(rs.result, 4, nullptr, 0, box0, seq0, s);
+        const int n0 = S.eq ? 6 : 4;
+        if (!rc)
+            rc = S.eq ? mam_open_oracle0(S.eq, S.bh, S.bh_base, n / 2, rs, s)
+                      : mam_open_u(S.bh, S.bh_base, false, e2_zero(), nullptr, n / 2, S.pt + 2, nv - 1, rs, s);
+        if (!rc) rc = mam_mailbox_post(rs.result, n0, nullptr, 0, box0, seq0, s);
         if (!rc && late_eq) {
             const ext2 rk = point_var(S.K);
             rc = mam_eq_table_scaled(S.pt + 2 * S.K, nv - S.K, e2_sub(e2_one(), rk), rk, eqp[(S.K & 1) ^ 1], s);
         }
-        if (!rc) rc = mam_mailbox_wait(box0, seq0, 4, h, s);
-        if (!rc) oracle_from_u(h, e2_one(), point_var(0), h);
+        if (!rc && pq) rc = arm(0) ? mam_gate_launch(qd[0].g, s) : -1;
+        if (!rc && pq) rc = queue_gated(0);
+        if (!rc) rc = mam_mailbox_wait(box0, seq0, n0, h, s);
+        if (!rc && !S.eq) oracle_from_u(h, e2_one(), point_var(0), h);
     }
     u64 *o_eval = out;
     u64 *o_oracles = out + 2;
     u64 *o_roots = o_oracles + 6L * nv;
-    u64 *o_final = o_roots + 4L * (nv - 1);
-    u64 *o_queries = o_final + 2L * (1L << lr);
+    u64 *o_final = o_roots + 4L * (nv - 1);  // the queries follow the final codeword
     if (!rc) {
         memcpy(o_oracles, h, 6 * sizeof(u64));
         // eval = P(0) + P(1) = 2 c0 + c1 + c2
@@ -304,66 +381,26 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         o_eval[1] = ev.c1;
         for (int k = 0; k < 3; k++) observe_ext2(t, o_oracles + 2 * k);
     }
-    const u64 *eq_cur = S.eq, *bh_cur = S.bh;
-    bool bh_base = S.bh_base;
-    size_t sc_size = n;
-    size_t flen = cw_len / 2;  // folded codeword length of this round
-    const u64 *cw_in = S.cw;
-    bool cw_base = S.cw_base;
     ext2 scale = e2_one();  // product form: eq of this round = scale * eq(r_round, ...)
     for (int round = 0; round < nv - 1 && !rc; round++) {
-        ext2 ch = sample_ext2(t);
-        const int pp = round & 1;
+        const ext2 ch = sample_ext2(t);
         const bool prod = !S.eq && round < S.K;
         ext2 next_scale = scale;
         if (!S.eq) {
             const ext2 rr = point_var(round);
             next_scale = e2_mul(scale, e2_add(e2_mul(e2_sub(e2_one(), rr), e2_sub(e2_one(), ch)), e2_mul(rr, ch)));
-            if (round == S.K) {
-                // the rounds from here ride along tree launches on the eq table built above
-                kscale = scale;
-                eq_cur = eqp[pp ^ 1];
-            }
+            // the rounds from K on ride along tree launches on the eq table built above
+            if (round == S.K) kscale = scale;
         }
-        // the eq / bh fold with the next oracle, run along with the folded codeword's tree
-        const OracleJob job = {eq_cur, bh_cur, ch, eqp[pp], bhp[pp], sc_size / 4, rs.partials,
-                               rs.counter, rs.result, 0, bh_base ? 1 : 0};
-        // product form: the bh fold and U sums ride along the tree's first launch (or run on a
-        // launch of their own before it with MAM_UJ_RIDE=0)
-        const UJob ujob = {bh_cur, bhp[pp], ch, sc_size / 4, S.pt + 2 * (round + 2), nv - round - 2,
-                           rs.partials, rs.result, rs.counter};
-        if (prod && (nv - round - 2 > 30 || sc_size / 4 > ((size_t)1 << (nv - round - 2)))) {
-            rc = -2;
-            break;
+        if (pq) {
+            release(round, ch);
+            rc = queue_gated(round + 1);  // the next round (or the final fold) waits behind its gate
+        } else {
+            rc = queue_round(round, ch, nullptr, nullptr);
         }
-        const bool ride = prod && uj_ride();
-        if (prod && !ride)
-            rc = mam_open_u(bh_cur, bh_base, true, ch, bhp[pp], sc_size / 4, S.pt + 2 * (round + 2), nv - round - 2,
-                            rs, s);
         if (rc) break;
-        eq_cur = eqp[pp];
-        bh_cur = bhp[pp];
-        bh_base = false;
-        sc_size /= 2;
-        const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, round);
-        // the oracle (3 Ext2) and the new root come back in one mailbox round trip, posted by
-        // the tree's final launch where it can
-        u64 seq = 0;
-        u64 *box = mam_mailbox_arm(&seq);
-        if (!box) {
-            rc = -1;
-            break;
-        }
-        const int nres = prod ? 4 : 6;  // U0, U1 or the oracle
-        const MailJob mail = {rs.result, nres, box, seq};
-        bool posted = false;
-        rc = mam_fold_and_tree(cw_in, cw_base, lvl, ch, fcw[round], ftree[round], flen, S.hash, s, prod ? nullptr : &job,
-                               &mail, &posted, ride ? &ujob : nullptr);
-        if (rc) break;
-        const size_t leaves = flen / 2;
-        if ((!posted && mam_mailbox_post(rs.result, nres, ftree[round] + mam_merkle_root_offset(leaves), 4, box, seq,
-                                         s)) ||
-            mam_mailbox_wait(box, seq, nres + 4, h, s)) {
+        const int nres = prod ? 4 : 6;
+        if (mam_mailbox_wait(qd[round].box, qd[round].seq, nres + 4, h, s)) {
             rc = -1;
             break;
         }
@@ -383,24 +420,23 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         memcpy(o_roots + 4L * round, h + nres, 4 * sizeof(u64));
         scale = next_scale;
         observe_hash(t, o_roots + 4L * round);
-        cw_in = fcw[round];
-        cw_base = false;
-        flen /= 2;
     }
     if (!rc) {
-        // last fold, no tree
-        ext2 ch = sample_ext2(t);
-        const u64 *lvl = d_table + 2 * table_level_offset(table_nv, lr, nv - 1);
-        rc = mam_cw_fold(cw_in, lvl, ch, fcw[nv - 1], flen, cw_base, s);
-        if (!rc) rc = read_back(h, fcw[nv - 1], 2 * flen, s);
-        if (!rc) memcpy(o_final, h, 2 * flen * sizeof(u64));
+        // last fold, no tree; the query indices follow its challenge directly (the final
+        // codeword is not observed), so the gather is queued with it and one copy brings back
+        // the final codeword and the queries
+        const ext2 ch = sample_ext2(t);
+        if (pq) release(nv - 1, ch);
+        else rc = queue_round(nv - 1, ch, nullptr, nullptr);
     }
+    const size_t fin_words = 2 * ((size_t)1 << lr);
+    u64 *d_out = alloc(fin_words + (size_t)nq * qlen);
+    if (!rc && !d_out) rc = -1;
     if (!rc && nq > 0) {
         std::vector<u64> idx(nq);
         for (int q = 0; q < nq; q++) idx[q] = mam::tr_challenge_scalar(t, nullptr, 0) % (u64)(cw_len / 2);
         u64 *d_idx = alloc(nq);
-        u64 *d_out = alloc((size_t)nq * qlen);
-        if (!d_idx || !d_out) rc = -1;
+        if (!d_idx) rc = -1;
         QuerySrc qs;
         memset(&qs, 0, sizeof(qs));
         qs.cw = S.cw;
@@ -412,6 +448,9 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         }
         qs.nv = nv;
         qs.lr = lr;
+        qs.fin = fcw[nv - 1];
+        qs.fin_out = d_out;
+        qs.fin_words = (int)fin_words;
         QueryIdx qin;
         const bool inl = nq <= QIDX_INLINE;
         if (inl)
@@ -419,14 +458,20 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         else if (!rc && hipMemcpyAsync(d_idx, idx.data(), nq * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
             rc = -1;
         if (!rc) {
-            k_gather_queries<<<nq, 64, 0, s>>>(qs, inl ? nullptr : d_idx, d_out, qlen, qin);
+            k_gather_queries<<<nq, 64, 0, s>>>(qs, inl ? nullptr : d_idx, d_out + fin_words, qlen, qin);
             if (hipGetLastError() != hipSuccess) rc = -1;
         }
-        if (!rc && hipMemcpyAsync(o_queries, d_out, (size_t)nq * qlen * sizeof(u64), hipMemcpyDeviceToHost, s) !=
-                       hipSuccess)
-            rc = -1;
-        if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = -1;
+    } else if (!rc && hipMemcpyAsync(d_out, fcw[nv - 1], fin_words * sizeof(u64), hipMemcpyDeviceToDevice, s) !=
+                          hipSuccess) {
+        rc = -1;
     }
+    if (!rc && hipMemcpyAsync(o_final, d_out, (fin_words + (size_t)nq * qlen) * sizeof(u64), hipMemcpyDeviceToHost,
+                              s) != hipSuccess)
+        rc = -1;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = -1;
+    // a failure may leave a queued round behind its gate: release it so the stream drains
+    if (rc && pq) mam_gate_open_all();
+    if (pq && mam_gate_take_failure() && !rc) rc = -1;  // a gate gave up: stale challenges
     return rc;
 }
 

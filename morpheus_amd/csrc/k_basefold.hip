@@ -345,9 +345,10 @@ int mam_commit_codeword(const u64 *d_evals, u64 *d_bh_evals, u64 *d_codeword, in
 // ---------------------------------------------------------------------------
 template <int KIND>  // 0: base cw, base alpha; 1: base cw, ext2 alpha; 2: ext2 cw, ext2 alpha
 __global__ void k_cw_fold(const u64 *__restrict__ cw, const u64 *__restrict__ table, ext2 alpha,
-                          u64 *__restrict__ out, size_t pairs) {
+                          u64 *__restrict__ out, size_t pairs, const u64 *alpha_p = nullptr) {
     size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
     if (i >= pairs) return;
+    if (alpha_p) alpha = ext2{alpha_p[0], alpha_p[1]};  // written on the device after the launch
     const u64 x0 = gl_canon(table[2 * i]), w = gl_canon(table[2 * i + 1]);
     if (KIND == 0) {
         u64 v0 = gl_canon(cw[2 * i]), v1 = gl_canon(cw[2 * i + 1]);
@@ -391,11 +392,12 @@ MAM_API int basefold_fold_ext2_ffi(const u64 *d_codeword, const u64 *d_table, u6
     return mam_launch_status();
 }
 
-int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s) {
+int mam_cw_fold(const u64 *cw, const u64 *table, ext2 alpha, u64 *out, size_t pairs, bool base_cw, hipStream_t s,
+                const u64 *alpha_p) {
     if (base_cw)
-        k_cw_fold<1><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs);
+        k_cw_fold<1><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs, alpha_p);
     else
-        k_cw_fold<2><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs);
+        k_cw_fold<2><<<grid_1d(pairs), MAM_BLOCK, 0, s>>>(cw, table, alpha, out, pairs, alpha_p);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -740,10 +742,10 @@ static unsigned open_grid(size_t items) {
 }
 
 int mam_open_u(const u64 *bh, bool bh_base, bool fold, ext2 ch, u64 *bh_out, size_t pairs, const u64 *r, int nb,
-               const RedScratch &rs, hipStream_t s) {
+               const RedScratch &rs, hipStream_t s, const u64 *chp) {
     if (nb < 0 || nb > 30 || pairs == 0 || pairs > ((size_t)1 << nb)) return -2;
     const unsigned g = open_grid(pairs);
-    const UJob j = {bh, bh_out, ch, pairs, r, nb, rs.partials, rs.result, rs.counter};
+    const UJob j = {bh, bh_out, ch, pairs, r, nb, rs.partials, rs.result, rs.counter, chp};
     ProfScope prof(PROF_OPEN_SUMCHECK, (double)pairs * ((bh_base ? 8.0 : 16.0) * (fold ? 4 : 2) + (fold ? 32.0 : 0.0)),
                    s);
     if (fold) {

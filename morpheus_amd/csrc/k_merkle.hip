@@ -58,7 +58,12 @@ struct FoldSrc {
     const u64 *table;  // (x0, w) pairs of this fold level
     ext2 alpha;
     u64 *out;          // folded codeword (Ext2)
+    const u64 *alpha_p;  // non-null: the challenge is read here (written on the device after launch)
 };
+
+__device__ __forceinline__ ext2 fold_alpha(const FoldSrc &fs) {
+    return fs.alpha_p ? ext2{fs.alpha_p[0], fs.alpha_p[1]} : fs.alpha;
+}
 
 // Writes `nlev` consecutive levels. The first written level has `n_first` computed nodes
 // (rep copies of them make the level) and starts at digest index `first_off` of `tree`;
@@ -105,7 +110,7 @@ __global__ __launch_bounds__(TREE_BLOCK) __attribute__((amdgpu_waves_per_eu(4)))
                     v1 = e2_load(src, 2 * k + 1);
                 }
                 const u64 x0 = gl_canon(fs.table[2 * k]), w = gl_canon(fs.table[2 * k + 1]);
-                f[h] = mam_fold_value(v0, v1, e2_sub(fs.alpha, e2_base(x0)), w);
+                f[h] = mam_fold_value(v0, v1, e2_sub(fold_alpha(fs), e2_base(x0)), w);
                 e2_store(fs.out, k, f[h]);
             }
             u64 ab[4] = {f[0].c0, f[0].c1, f[1].c0, f[1].c1};
@@ -194,7 +199,7 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
                 v1 = e2_load(src, 2 * k + 1);
             }
             const u64 x0 = gl_canon(fs.table[2 * k]), w = gl_canon(fs.table[2 * k + 1]);
-            const ext2 f = mam_fold_value(v0, v1, e2_sub(fs.alpha, e2_base(x0)), w);
+            const ext2 f = mam_fold_value(v0, v1, e2_sub(fold_alpha(fs), e2_base(x0)), w);
             x = (j & 1) ? f.c1 : f.c0;
             fs.out[2 * k + (j & 1)] = x;
         }
@@ -226,6 +231,7 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
         const int t = threadIdx.x;
         if (t < mj.na + 4) mj.box[t] = t < mj.na ? mj.a[t] : sh[t - mj.na];
         mam_mailbox_raise(mj.box, mj.seq);
+        mam_gate_wait(mj.gate);  // the next round's challenge, when it is pre-queued
     }
 }
 
@@ -457,10 +463,10 @@ int mam_merkle_build(const u64 *d_cw, u64 *d_tree, size_t num_leaves, int hash, 
 
 int mam_fold_and_tree(const u64 *cw, bool base_cw, const u64 *table, ext2 alpha, u64 *fold_out, u64 *tree,
                       size_t folded_len, int hash, hipStream_t s, const OracleJob *oj, const MailJob *mj,
-                      bool *mail_posted, const UJob *uj) {
+                      bool *mail_posted, const UJob *uj, const u64 *alpha_p) {
     if (mail_posted) *mail_posted = false;
     if (folded_len < 2) return -2;
-    FoldSrc fs = {table, alpha, fold_out};
+    FoldSrc fs = {table, alpha, fold_out, alpha_p};
     int src = base_cw ? SRC_FOLD_GL : SRC_FOLD_EXT2;
     if (hash == MAM_HASH_MONOLITH)
         return build_tree<MonoHash>(cw, tree, folded_len / 2, src, fs, s, oj, mj, mail_posted, 1, uj);

@@ -21,6 +21,7 @@ struct OracleJob {
     u64 *result;    // 6 words: the next oracle (c0, c1, c2) as Ext2
     unsigned blocks;  // workgroups running the job (0: no job)
     int bh_base;
+    const u64 *chp;  // non-null: the challenge is read here instead of ch
 };
 
 // `lds` holds 6 * (blockDim.x / 64) words; bid / nblocks number the job's workgroups.
@@ -30,7 +31,7 @@ struct OracleJob {
 template <bool BH_BASE, bool TICKET>
 __device__ __forceinline__ void open_fold_oracle_run(const OracleJob &j, unsigned bid, unsigned nblocks, u64 *lds) {
     u64 acc[6] = {0, 0, 0, 0, 0, 0};
-    const ext2 ch = j.ch;
+    const ext2 ch = j.chp ? ext2{j.chp[0], j.chp[1]} : j.ch;
     const u64 ch7 = gl_mul7(ch.c1);
     for (size_t y = bid * (size_t)blockDim.x + threadIdx.x; y < j.half_out; y += (size_t)nblocks * blockDim.x) {
         ext2 q0 = e2_load(j.eq, 4 * y), q1 = e2_load(j.eq, 4 * y + 1), q2 = e2_load(j.eq, 4 * y + 2),
@@ -79,6 +80,7 @@ struct UJob {
     int nb;        // E's variables (bits of y), <= 30
     u64 *partials, *result;
     u32 *counter;  // ticket of the one-launch reduction (a job riding along a tree launch)
+    const u64 *chp;  // non-null: the challenge is read here instead of ch
 };
 
 template <bool BH_BASE, bool FOLD, bool TICKET>
@@ -97,7 +99,7 @@ __device__ __forceinline__ void open_u_run(const UJob &j, unsigned bid, unsigned
     }
     mam_barrier();
     const int ng = j.nb > 0 ? (j.nb + 5) / 6 : 1;
-    const ext2 ch = j.ch;
+    const ext2 ch = j.chp ? ext2{j.chp[0], j.chp[1]} : j.ch;
     const u64 ch7 = gl_mul7(ch.c1);
     // A wave takes chunks of 64 * st consecutive y (st steps of 64 lanes, st a power of two <= 16
     // chosen so that every wave of the grid has a chunk). Within a chunk a lane's y keeps its
