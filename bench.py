@@ -333,7 +333,9 @@ def bench_basefold(args, M):
     d_cw, d_bh, d_tree = M.DeviceBuffer(cw_len), M.DeviceBuffer(1 << nv), M.DeviceBuffer((cw_len - 1) * 4)
     root = np.zeros(4, dtype=np.uint64)
     U = ctypes.POINTER(ctypes.c_uint64)
-    proof = np.zeros(M.proof_len(nv, lr, nq), dtype=np.uint64)
+    # the proof lands in a page-locked buffer allocated once (mam_host_alloc)
+    proof_buf = M.PinnedArray(M.proof_len(nv, lr, nq))
+    proof = proof_buf.a
 
     def step():
         M.check(lib.mam_basefold_commit_into(d_ev.ptr, nv, lr, hash_kind, d_cw.ptr, d_bh.ptr, d_tree.ptr,
@@ -623,24 +625,29 @@ def bench_batch(args, M):
     total = int(sizes.sum())
     perms = sum(merkle_perms(int(v), lr) for v in nvs)
 
-    def run(w):
+    # proofs land in page-locked buffers allocated once (one set per form, compared below)
+    def pinned_set():
+        return [M.PinnedArray(M.proof_len(int(v), lr, nq)) for v in nvs]
+    pb_seq, pb_bat, pb_g = pinned_set(), pinned_set(), pinned_set()
+
+    def run(w, pb):
         t0 = time.perf_counter()
-        out = prove_batch(insts, table, lr, 0, nq, workers=w)
+        out = prove_batch(insts, table, lr, 0, nq, workers=w, proof_bufs=[p.a for p in pb])
         return time.perf_counter() - t0, out
 
     # worker count autotuned on every fourth instance (untimed)
     cands = (args.workers,) if args.workers else (1, 2, 4, 8, 12, 16)
     best, tune = autotune_workers(insts, table, lr, 0, nq, candidates=cands)
     for _ in range(max(args.warmup, 1)):
-        run(best)
-        run(1)
+        run(best, pb_bat)
+        run(1, pb_seq)
     seq_t, seq_lat, bat_t, bat_lat = [], [], [], []
     seq_out = bat_out = None
     for _ in range(args.steps):
-        dt, seq_out = run(1)
+        dt, seq_out = run(1, pb_seq)
         seq_t.append(dt)
         seq_lat += [o["end_ms"] - o["start_ms"] for o in seq_out]
-        dt, bat_out = run(best)
+        dt, bat_out = run(best, pb_bat)
         bat_t.append(dt)
         bat_lat += [o["end_ms"] - o["start_ms"] for o in bat_out]
     seq_s, bat_s = float(np.mean(seq_t)), float(np.mean(bat_t))
@@ -651,7 +658,8 @@ def bench_batch(args, M):
     g_out = None
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        g_out = prove_batch_graph(insts, table, lr, 0, nq, gset["workers"], gset["branches"], gset["max_wg"])
+        g_out = prove_batch_graph(insts, table, lr, 0, nq, gset["workers"], gset["branches"], gset["max_wg"],
+                                  proof_bufs=[p.a for p in pb_g])
         g_t.append(time.perf_counter() - t0)
         g_lat += [o["end_ms"] - o["start_ms"] for o in g_out]
         g_phase.append(graph_phase_ms())

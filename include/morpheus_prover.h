@@ -126,6 +126,12 @@ int mam_basefold_commitment_buffers(void *handle, uint64_t **d_codeword, uint64_
 int mam_basefold_commitment_free(void *handle);
 /* Free the calling thread's cached scratch (re-created on demand). */
 int mam_release_scratch(void);
+/* No reference counterpart: page-locked host memory for outputs written again and again (a
+ * proof buffer). Device-to-host copies into it are direct; into pageable memory they pass
+ * through a staging buffer and a host copy (about 40 us per 2^22 proof's 0.9 MB of queries).
+ * For callers that reuse their output buffers. */
+int mam_host_alloc(size_t bytes, void **p);
+int mam_host_free(void *p);
 /* BasefoldCommitment::open_ext2 (:578-863) with extract_ext2_queries (:2310-2402).
  * point2: num_vars Ext2 (host); d_table: device table generated for table_nv >= num_vars;
  * proof: mam_basefold_proof_len() words in the flat layout

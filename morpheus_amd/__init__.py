@@ -6,7 +6,7 @@ Importing requires the built HIP library (morpheus_amd/libmorpheus_amd.so); ther
 CPU fallback.
 """
 from ._lib import EXPORTED, LIB_PATH, lib  # noqa: F401
-from .device import DeviceBuffer, GpuError, check, device_count, synchronize  # noqa: F401
+from .device import DeviceBuffer, GpuError, PinnedArray, check, device_count, synchronize  # noqa: F401
 from .prover import (HASH_MONOLITH, HASH_POSEIDON2, BasefoldCommitment, BasefoldTable,  # noqa: F401
                      GpuSumcheckStateExt2, Transcript, proof_len, split_proof, sumcheck_prove, sumcheck_verify,
                      verify_ext2)

@@ -260,6 +260,8 @@ _SIGS = {
     "mam_profile_enable": (ci, [ci]), "mam_profile_enable_mask": (ci, [ctypes.c_uint]),
     "mam_valu_probe": (ci, [ci, ci, ci, ctypes.POINTER(ctypes.c_double)]),
     "mam_release_scratch": (ci, []),
+    "mam_host_alloc": (ci, [ctypes.c_size_t, ctypes.POINTER(vp)]),
+    "mam_host_free": (ci, [vp]),
     "mam_qtranscript_init": (None, [qtrp, ctypes.c_char_p, cl]),
     "mam_qtranscript_append_u64s": (None, [qtrp, ctypes.c_char_p, cl, u64p, cl]),
     "mam_qtranscript_append_ext2": (None, [qtrp, ctypes.c_char_p, cl, u64p]),

@@ -25,24 +25,38 @@ class Instance:
         self.d_fg = d_fg if d_fg is not None else DeviceBuffer.from_numpy(np.concatenate([f, g]))
 
 
-def prove_batch(instances, table, log_rate=1, hash_kind=0, num_queries=100, workers=1):
-    """Returns one dict per instance (round_msgs, challenges, final_evals, root, proof,
-    start_ms, end_ms) in input order."""
-    d_table = table.upload()
-    n = len(instances)
-    arr = (BatchInstance_t * max(n, 1))()
+def _outputs(instances, arr, log_rate, num_queries, proof_bufs):
+    """Output arrays per instance, pointed to by arr. proof_bufs (optional): one array per
+    instance, at least proof_len words, that receives the proof (e.g. PinnedArray.a, which the
+    device copies into directly; fresh pageable arrays cost a staging copy and page faults)."""
     outs = []
     for i, inst in enumerate(instances):
         nv = inst.num_vars
+        plen = proof_len(nv, log_rate, num_queries)
+        if proof_bufs is not None:
+            pb = proof_bufs[i]
+            assert pb.dtype == np.uint64 and pb.flags.c_contiguous and pb.size >= plen
+            proof = pb[:plen]
+        else:
+            proof = np.zeros(plen, dtype=np.uint64)
         o = {"round_msgs": np.zeros((nv, 3, 2), dtype=np.uint64), "challenges": np.zeros((nv, 2), dtype=np.uint64),
-             "final_evals": np.zeros((2, 2), dtype=np.uint64), "root": np.zeros(4, dtype=np.uint64),
-             "proof": np.zeros(proof_len(nv, log_rate, num_queries), dtype=np.uint64)}
+             "final_evals": np.zeros((2, 2), dtype=np.uint64), "root": np.zeros(4, dtype=np.uint64), "proof": proof}
         outs.append(o)
         a = arr[i]
         a.d_fg, a.num_vars = inst.d_fg.ptr, nv
         a.label, a.label_len = inst.label, len(inst.label)
         for k in ("round_msgs", "challenges", "final_evals", "root", "proof"):
             setattr(a, k, o[k].ctypes.data_as(_u64p))
+    return outs
+
+
+def prove_batch(instances, table, log_rate=1, hash_kind=0, num_queries=100, workers=1, proof_bufs=None):
+    """Returns one dict per instance (round_msgs, challenges, final_evals, root, proof,
+    start_ms, end_ms) in input order. proof_bufs: see _outputs."""
+    d_table = table.upload()
+    n = len(instances)
+    arr = (BatchInstance_t * max(n, 1))()
+    outs = _outputs(instances, arr, log_rate, num_queries, proof_bufs)
     check(lib.mam_prove_batch(arr, n, d_table.ptr, table.num_vars, log_rate, hash_kind, num_queries, workers),
           "mam_prove_batch")
     for i, o in enumerate(outs):
@@ -51,7 +65,7 @@ def prove_batch(instances, table, log_rate=1, hash_kind=0, num_queries=100, work
 
 
 def prove_batch_graph(instances, table, log_rate=1, hash_kind=0, num_queries=100, workers=8, branches=2,
-                      max_wg=256):
+                      max_wg=256, proof_bufs=None):
     """The graph form (mam_prove_batch_graph): every instance's sumcheck and commitment in one
     hipGraph launch with the transcripts on the device, then the openings on `workers`
     threads. Same outputs as prove_batch, word for word. The graph is kept for the next call
@@ -59,18 +73,7 @@ def prove_batch_graph(instances, table, log_rate=1, hash_kind=0, num_queries=100
     d_table = table.upload()
     n = len(instances)
     arr = (BatchInstance_t * max(n, 1))()
-    outs = []
-    for i, inst in enumerate(instances):
-        nv = inst.num_vars
-        o = {"round_msgs": np.zeros((nv, 3, 2), dtype=np.uint64), "challenges": np.zeros((nv, 2), dtype=np.uint64),
-             "final_evals": np.zeros((2, 2), dtype=np.uint64), "root": np.zeros(4, dtype=np.uint64),
-             "proof": np.zeros(proof_len(nv, log_rate, num_queries), dtype=np.uint64)}
-        outs.append(o)
-        a = arr[i]
-        a.d_fg, a.num_vars = inst.d_fg.ptr, nv
-        a.label, a.label_len = inst.label, len(inst.label)
-        for k in ("round_msgs", "challenges", "final_evals", "root", "proof"):
-            setattr(a, k, o[k].ctypes.data_as(_u64p))
+    outs = _outputs(instances, arr, log_rate, num_queries, proof_bufs)
     check(lib.mam_prove_batch_graph(arr, n, d_table.ptr, table.num_vars, log_rate, hash_kind, num_queries, workers,
                                     branches, max_wg), "mam_prove_batch_graph")
     for i, o in enumerate(outs):

@@ -368,6 +368,17 @@ u64 *mam_host_arena(size_t words) {
     return (u64 *)a.p;
 }
 
+// Page-locked host memory for caller-owned outputs (no counterpart in the reference).
+MAM_API int mam_host_alloc(size_t bytes, void **p) {
+    if (!p) return -2;
+    *p = nullptr;
+    return hipHostMalloc(p, bytes ? bytes : 8, hipHostMallocDefault) == hipSuccess ? 0 : -1;
+}
+MAM_API int mam_host_free(void *p) {
+    if (!p) return 0;
+    return hipHostFree(p) == hipSuccess ? 0 : -1;
+}
+
 // Free the calling thread's scratch arenas (they are re-created on demand).
 MAM_API int mam_release_scratch() {
     (void)hipStreamSynchronize(mam_stream());

@@ -145,10 +145,27 @@ __global__ void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx, u64 *_
     }
 }
 
+// 128 pinned words per thread: [0, 64) read-back words, [64, 128) host-to-device staging
 u64 *pinned_scratch() {
     static thread_local u64 *buf = nullptr;
-    if (!buf && hipHostMalloc((void **)&buf, 64 * sizeof(u64), hipHostMallocDefault) != hipSuccess) buf = nullptr;
+    if (!buf && hipHostMalloc((void **)&buf, 128 * sizeof(u64), hipHostMallocDefault) != hipSuccess) buf = nullptr;
     return buf;
+}
+
+// The opening's ticket counters (grid_finish_n): device words of the calling thread, zeroed
+// once; every ticketed reduction leaves its counter at zero again, so no clear per opening.
+u32 *ticket_counters() {
+    static thread_local u32 *c = nullptr;
+    if (!c) {
+        u32 *p = nullptr;
+        if (hipMalloc((void **)&p, 64) != hipSuccess) return nullptr;
+        if (hipMemset(p, 0, 64) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+            (void)hipFree(p);
+            return nullptr;
+        }
+        c = p;
+    }
+    return c;
 }
 
 int read_back(u64 *host, const u64 *dev, size_t words, hipStream_t s) {
@@ -235,7 +252,7 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     size_t need = 0;
     auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
     {
-        need += 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) + words(8) + words(nq) +
+        need += 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) + words(nq) +
                 words(2 * ((size_t)1 << lr) + (size_t)nq * qlen);
         size_t flen = cw_len / 2;
         for (int r = 0; r < nv; r++) {
@@ -256,7 +273,6 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     u64 *eqp[2] = {alloc(n), alloc(n / 2)};   // Ext2 ping-pong: n/2 then n/4 elements
     u64 *bhp[2] = {alloc(n), alloc(n / 2)};
     u64 *red = alloc(RED_BLOCKS * 6 + 6);
-    u64 *cnt = alloc(8);
     std::vector<u64 *> fcw(nv), ftree(nv - 1);
     {
         size_t flen = cw_len / 2;
@@ -269,8 +285,9 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     if (!arena) rc = -1;
     u64 *h = pinned_scratch();
     if (!h) rc = -1;
-    RedScratch rs = {red, (u32 *)cnt, red + RED_BLOCKS * 6};
-    if (!rc && hipMemsetAsync(cnt, 0, 64, s) != hipSuccess) rc = -1;
+    u32 *tickets = ticket_counters();
+    if (!tickets) rc = -1;
+    RedScratch rs = {red, tickets, red + RED_BLOCKS * 6};
     auto point_var = [&](int k) { return ext2{gl_canon(S.pt_host[2 * k]), gl_canon(S.pt_host[2 * k + 1])}; };
     // The rounds from K on ride along tree launches and fold an eq table. Their table is
     // scale_K * eq(r_K, ..., r_{nv-1}; .); the unscaled table depends on the point only and is
@@ -503,7 +520,12 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     u64 *arena = mam_dev_arena(words(2 * nv) + words(n) + eq_words, 3);
     if (!arena) return -1;
     u64 *d_point = arena, *bh = arena + words(2 * nv), *eq = K == 0 ? bh + words(n) : nullptr;
-    if (hipMemcpyAsync(d_point, point2, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+    // the point goes through the thread's pinned staging words (a copy from pageable memory
+    // would be staged and waited for by the runtime)
+    u64 *stage = pinned_scratch();
+    if (!stage || 2 * nv > 64) return -1;
+    memcpy(stage + 64, point2, 2 * nv * sizeof(u64));
+    if (hipMemcpyAsync(d_point, stage + 64, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
     int rc = K == 0 ? mam_eq_table(d_point, nv, eq, true, s) : 0;
     if (!rc) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
     if (rc) return rc;

@@ -55,6 +55,34 @@ class DeviceBuffer:
             pass
 
 
+class PinnedArray:
+    """Page-locked host array of ``n`` words of ``dtype`` (mam_host_alloc): device-to-host
+    copies into it skip the staging copy that pageable memory takes. ``.a`` is the numpy
+    view; the memory lives as long as this object."""
+
+    def __init__(self, n, dtype=np.uint64):
+        dt = np.dtype(dtype)
+        self.n = int(n)
+        p = ctypes.c_void_p()
+        check(lib.mam_host_alloc(max(self.n, 1) * dt.itemsize, ctypes.byref(p)), "mam_host_alloc")
+        self.ptr = p.value
+        buf = (ctypes.c_char * (max(self.n, 1) * dt.itemsize)).from_address(self.ptr)
+        self.a = np.frombuffer(buf, dtype=dt, count=self.n)
+        self.a[:] = 0
+
+    def free(self):
+        if self.ptr:
+            self.a = None
+            lib.mam_host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def synchronize():
     check(lib.cuda_device_synchronize(), "cuda_device_synchronize")
 

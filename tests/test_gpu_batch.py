@@ -55,6 +55,28 @@ def test_batch_matches_oracle_and_sequential():
             assert np.array_equal(mix[i][k], seq[i][k]), (i, k)
 
 
+def test_batch_into_pinned_buffers():
+    """Proofs written into caller-owned page-locked buffers (PinnedArray, larger than needed)
+    equal the library-allocated ones; the words past the proof stay untouched."""
+    print("batch pinned", flush=True)
+    nvs = [6, 9, 4]
+    lr, nq, tnv = 1, 10, 9
+    table = M.BasefoldTable.generate(tnv, lr, tnv, 42)
+    _, insts = make(nvs, 3)
+    ref = prove_batch(insts, table, lr, 0, nq, workers=2)
+    bufs = [M.PinnedArray(M.proof_len(nv, lr, nq) + 5) for nv in nvs]
+    for b in bufs:
+        b.a[:] = 0xABCD
+    got = prove_batch(insts, table, lr, 0, nq, workers=2, proof_bufs=[b.a for b in bufs])
+    for i, nv in enumerate(nvs):
+        plen = M.proof_len(nv, lr, nq)
+        assert np.array_equal(got[i]["proof"], ref[i]["proof"]), i
+        assert np.array_equal(bufs[i].a[:plen], ref[i]["proof"]), i
+        assert np.all(bufs[i].a[plen:] == 0xABCD), i
+    for b in bufs:
+        b.free()
+
+
 def test_batch_uneven_sizes_many_workers():
     print("batch uneven", flush=True)
     nvs = [12, 2, 10, 2, 11, 8, 12, 3, 9, 6, 7, 5]
