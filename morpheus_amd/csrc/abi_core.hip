@@ -485,14 +485,18 @@ int mam_gate_launch(const GateJob &g, hipStream_t s) {
 }
 
 const u64 *mam_gate_queue(hipStream_t s, int nwords, u64 *seq) {
-    GateJob g;
-    const u64 *d = mam_gate_arm(nwords, &g, seq);
-    if (!d) return nullptr;
-    if (mam_gate_launch(g, s)) {
+    if (nwords < 1 || nwords > 8 || !gate_ready()) return nullptr;
+    const u64 n = t_gate.armed + 1;
+    const int k = (int)(n & 1);
+    if (hipStreamWaitValue64(s, t_gate.pin, n, hipStreamWaitValueGte, ~(u64)0) != hipSuccess) return nullptr;
+    t_gate.armed = n;
+    if (hipMemcpyAsync(t_gate.dev + 8 * k, t_gate.pin + 8 + 8 * k, nwords * sizeof(u64), hipMemcpyHostToDevice, s) !=
+        hipSuccess) {
         mam_gate_open_all();
         return nullptr;
     }
-    return d;
+    *seq = n;
+    return t_gate.dev + 8 * k;
 }
 
 void mam_gate_open(u64 seq, const u64 *vals, int nwords) {

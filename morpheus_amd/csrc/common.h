@@ -102,18 +102,21 @@ __device__ __forceinline__ void mam_mailbox_raise(u64 *box, u64 seq) {
 
 // Pre-queued rounds. A host-driven proving round needs values the host computes from the
 // previous round's result (Fiat-Shamir challenges). Launching the round's kernels only after
-// that costs an idle-queue launch (~13 us until the first kernel runs on the box); queuing
-// them beforehand behind a gate costs only the host's turnaround. A gate is one workgroup's
-// bounded spin on a pinned flag followed by a copy of up to 8 scalars from a pinned slot into
-// device memory, where the queued kernels read them: a launch of its own (mam_gate_launch,
-// or mam_gate_queue = arm + launch), or the tail of the kernel that posts the previous
-// round's result (mam_mailbox_post and the lane-parallel tree's post take a GateJob), which
-// saves the gate's own dispatch. mam_gate_arm reserves the calling thread's next gate and
-// returns the device address of its scalars; mam_gate_open writes the scalars and releases.
-// Gates are released in the order they were armed, at most two armed and not yet released.
-// Every armed gate must be opened (mam_gate_open_all on error paths); a spin that has not
-// seen its release after MAM_GATE_LIMIT_S seconds gives up (the stream then runs on with
-// stale scalars) and records it: mam_gate_take_failure() reports and clears that.
+// that costs an idle-queue launch (~10-15 us until the first kernel runs on the box); queuing
+// them beforehand behind a gate costs only the host's turnaround. A gate releases up to 8
+// scalars, which the host writes to a pinned slot, into device memory where the queued
+// kernels read them. Two forms:
+//  - mam_gate_queue: a stream wait on the pinned release flag (hipStreamWaitValue64) and an
+//    asynchronous copy of the slot. The sumcheck's passes use it (measured faster there).
+//  - mam_gate_arm + a spin: one wave polls the flag (bounded) and copies the slot, either as
+//    a launch of its own (mam_gate_launch) or as the tail of the kernel that posts the
+//    previous round's result (mam_mailbox_post and the lane-parallel tree's post take a
+//    GateJob), which saves the gate's own dispatches. The Basefold opening uses it.
+// mam_gate_open writes the scalars and releases. Gates are released in the order they were
+// armed or queued, at most two outstanding. Every gate must be opened (mam_gate_open_all on
+// error paths); a spin that has not seen its release after MAM_GATE_LIMIT_S seconds gives up
+// (the stream then runs on with stale scalars) and records it: mam_gate_take_failure()
+// reports and clears that.
 // A stream waiting at a gate also holds up the other streams that share its hardware queue,
 // so threads that prove side by side with others (the worker pools) do not pre-queue:
 // mam_prequeue_enabled() is false on them (mam_prequeue_set), and MAM_PREQUEUE=0 turns it
@@ -138,18 +141,23 @@ void mam_gate_open(u64 seq, const u64 *vals, int nwords);
 void mam_gate_open_all();
 bool mam_gate_take_failure();
 
-// Device side of a gate: one thread spins on the release, then copies the scalars.
+// Device side of a gate, run by the first wave of the workgroup (the other waves return):
+// lane 0 spins on the release flag, then lane i < n copies scalar i (one host read for all).
 __device__ __forceinline__ void mam_gate_wait(const GateJob &g) {
-    if (!g.flag || threadIdx.x != 0) return;
-    const u64 t0 = (u64)wall_clock64();
-    while (__hip_atomic_load(g.flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM) < g.seq) {
-        if ((u64)wall_clock64() - t0 > g.limit) {
-            __hip_atomic_store(g.err, (u64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-            break;
+    if (!g.flag || threadIdx.x >= 64) return;
+    const int t = threadIdx.x;
+    if (t == 0) {
+        const u64 t0 = (u64)wall_clock64();
+        while (__hip_atomic_load(g.flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) < g.seq) {
+            if ((u64)wall_clock64() - t0 > g.limit) {
+                __hip_atomic_store(g.err, (u64)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_s_sleep(2);
     }
-    for (int i = 0; i < g.n; i++) g.dst[i] = __hip_atomic_load(&g.slot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __atomic_thread_fence(__ATOMIC_ACQUIRE);  // the slot reads follow the flag
+    if (t < g.n) g.dst[t] = __hip_atomic_load(&g.slot[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // A mailbox post handed to a kernel: na words from a, then the kernel's own words.

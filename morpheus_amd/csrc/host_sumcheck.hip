@@ -317,7 +317,6 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
     if (!pre) rc = mam_sc_launch_pass_final(np, fsrc, fstride, fbase, last.look ? 2 : 1, ca, cb, S.finals, S.st);
     if (!rc) rc = mam_publish_wait(S.finals, 2 * np, nullptr, 0, S.host, S.st);
     if (!rc) memcpy(final_evals, S.host, 2 * np * sizeof(u64));
-    if (pre && mam_gate_take_failure() && !rc) rc = -1;  // a gate gave up: stale challenges
     S.finished = true;
     return rc;
 }
