@@ -105,6 +105,8 @@ def parse():
                     help="Merkle hash of the basefold workload (the reference default is Monolith-w12)")
     ap.add_argument("--cpu-sample-nv", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-sweep", action="store_true",
+                    help="only time a plain device copy at 1 MiB .. 512 MiB (the achievable HBM rate by size)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy (PMC byte-counter calibration)")
     return ap.parse_args()
@@ -1014,6 +1016,15 @@ def main():
         print("no usable GPU", file=sys.stderr)
         sys.exit(1)
     valu_peak(M)
+    if args.copy_sweep:
+        for mib in (1, 2, 4, 8, 16, 32, 64, 96, 128, 256, 512):
+            ms = ctypes.c_double()
+            M.check(M.lib.mam_copy_probe(mib << 20, 20, ctypes.byref(ms)), "mam_copy_probe")
+            moved = 2 * (mib << 20)
+            print(json.dumps({"copy_mib_each_way": mib, "bytes_moved": moved, "ms": ms.value,
+                              "gb_s": moved / ms.value / 1e6, "frac_of_8tb": moved / ms.value / 1e6 / 8000.0}),
+                  flush=True)
+        return
     if args.calibrate_copy:
         n = 1 << 25
         a, b = M.DeviceBuffer(n), M.DeviceBuffer(n)

@@ -302,6 +302,10 @@ int mam_profile_enable_mask(unsigned mask);
  * 3 = v_add_u32; each thread of `blocks` x 256 runs iters x 8 single-instruction steps.
  * *ms = the timed launch. */
 int mam_valu_probe(int kind, int blocks, int iters, double *ms);
+/* No reference counterpart: median launch time (ms) of a plain device-to-device copy of
+ * `bytes` (a multiple of 4096), the best of three grid shapes: the bandwidth a streaming
+ * kernel of that size can reach, launch and ramp included. */
+int mam_copy_probe(size_t bytes, int reps, double *ms);
 int mam_profile_collect(double *ms, long *count, double *work);
 
 #ifdef __cplusplus

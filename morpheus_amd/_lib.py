@@ -259,6 +259,7 @@ _SIGS = {
     "agl_fused_permute_partial_eval_ffi": (ci, [vp, vp, vp, vp, vp, ci, ci, ci, ci, ci]),
     "mam_profile_enable": (ci, [ci]), "mam_profile_enable_mask": (ci, [ctypes.c_uint]),
     "mam_valu_probe": (ci, [ci, ci, ci, ctypes.POINTER(ctypes.c_double)]),
+    "mam_copy_probe": (ci, [ctypes.c_size_t, ci, ctypes.POINTER(ctypes.c_double)]),
     "mam_release_scratch": (ci, []),
     "mam_host_alloc": (ci, [ctypes.c_size_t, ctypes.POINTER(vp)]),
     "mam_host_free": (ci, [vp]),

@@ -164,14 +164,16 @@ __global__ __launch_bounds__(WIDE_GROUPS * 16) void k_tree_wide(const u64 *__res
                                                                 FoldSrc fs, OracleJob oj, MailJob mj, int rep) {
     __shared__ u64 sh[WIDE_GROUPS * 4];
     __shared__ __align__(16) u64 wst[WIDE_GROUPS][16];  // per-group permutation slots
-    if (oj.blocks) {
-        const unsigned tree_blocks = gridDim.x - oj.blocks;
-        if (blockIdx.x >= tree_blocks) {  // workgroup-uniform
-            if (oj.bh_base)
-                open_fold_oracle_run<true, true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
-            else
-                open_fold_oracle_run<false, true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
-            return;
+    if constexpr (SRC == SRC_FOLD_GL || SRC == SRC_FOLD_EXT2) {  // only folded trees carry the job
+        if (oj.blocks) {
+            const unsigned tree_blocks = gridDim.x - oj.blocks;
+            if (blockIdx.x >= tree_blocks) {  // workgroup-uniform
+                if (oj.bh_base)
+                    open_fold_oracle_run<true, true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
+                else
+                    open_fold_oracle_run<false, true>(oj, blockIdx.x - tree_blocks, oj.blocks, sh);
+                return;
+            }
         }
     }
     const int g = threadIdx.x >> 4, j = threadIdx.x & 15;
