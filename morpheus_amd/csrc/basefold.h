@@ -63,9 +63,9 @@ size_t mam_merkle_narrow_min();
 // -> folded pair y, written to bh_out) at ch; then U0 = sum_y E(y) b(2y), U1 = sum_y E(y)
 // b(2y+1) over `pairs` pairs y, E(y) = eq(r, y) over nb variables (r: device Ext2 point
 // words). result: U0, U1 (4 words); partials: RedScratch partials. chp (optional): ch is read
-// from device memory there.
+// from device memory there. rev: bh in bit-reversed order and r reversed (UJob::rev).
 int mam_open_u(const u64 *bh, bool bh_base, bool fold, ext2 ch, u64 *bh_out, size_t pairs, const u64 *r, int nb,
-               const RedScratch &rs, hipStream_t s, const u64 *chp = nullptr);
+               const RedScratch &rs, hipStream_t s, const u64 *chp = nullptr, bool rev = false);
 int mam_open_oracle0(const u64 *eq, const u64 *bh, bool bh_base, size_t pairs, const RedScratch &rs,
                      hipStream_t s);
 int mam_open_fold_oracle(const u64 *eq, const u64 *bh, bool bh_base, ext2 ch, u64 *eq_out, u64 *bh_out,

@@ -145,10 +145,10 @@ __global__ void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx, u64 *_
     }
 }
 
-// 128 pinned words per thread: [0, 64) read-back words, [64, 128) host-to-device staging
+// 192 pinned words per thread: [0, 64) read-back words, [64, 192) host-to-device staging
 u64 *pinned_scratch() {
     static thread_local u64 *buf = nullptr;
-    if (!buf && hipHostMalloc((void **)&buf, 128 * sizeof(u64), hipHostMallocDefault) != hipSuccess) buf = nullptr;
+    if (!buf && hipHostMalloc((void **)&buf, 192 * sizeof(u64), hipHostMallocDefault) != hipSuccess) buf = nullptr;
     return buf;
 }
 
@@ -220,6 +220,11 @@ struct OpenSrc {
     // mam_open_u, round K builds the scaled eq table the later (ride-along) rounds fold.
     const u64 *pt = nullptr, *pt_host = nullptr;
     int K = 0;
+    // Product form only: bh (and every table the rounds fold) in bit-reversed order, as the
+    // commitment stores it, with pt_rev the point's coordinates in reversed order on the
+    // device (UJob::rev, OracleJob::rev); no bit reversal ahead of the rounds.
+    bool rev = false;
+    const u64 *pt_rev = nullptr;
 };
 
 // The round's oracle (c0, c1, c2) from U0, U1 when eq = sc * eq(r_k, r_k+1, ...) (product form).
@@ -233,6 +238,14 @@ static void oracle_from_u(const u64 *U, ext2 sc, ext2 rk, u64 *o6) {
         o6[2 * k] = c[k].c0;
         o6[2 * k + 1] = c[k].c1;
     }
+}
+
+static bool open_rev() {
+    static const bool v = [] {
+        const char *e = getenv("MAM_OPEN_REV");
+        return !(e && e[0] == '0');
+    }();
+    return v;
 }
 
 static bool uj_ride() {
@@ -325,15 +338,15 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         if (prod && (nv - round - 2 > 30 || scs / 4 > ((size_t)1 << (nv - round - 2)))) return -2;
         // the eq / bh fold with the next oracle, run along with the folded codeword's tree
         const OracleJob job = {eq_in, bh_in, ch, eqp[pp], bhp[pp], scs / 4, rs.partials, rs.counter, rs.result, 0,
-                               bhb ? 1 : 0, chp};
+                               bhb ? 1 : 0, chp, S.rev ? 1 : 0};
         // product form: the bh fold and U sums ride along the tree's first launch (or run on a
         // launch of their own before it with MAM_UJ_RIDE=0)
-        const UJob ujob = {bh_in, bhp[pp], ch, scs / 4, S.pt + 2 * (round + 2), nv - round - 2, rs.partials,
-                           rs.result, rs.counter, chp};
+        const u64 *ur = S.rev ? S.pt_rev : S.pt + 2 * (round + 2);  // E's coordinates
+        const UJob ujob = {bh_in, bhp[pp], ch, scs / 4, ur, nv - round - 2, rs.partials,
+                           rs.result, rs.counter, chp, S.rev ? 1 : 0};
         const bool ride = prod && ride_all;
         if (prod && !ride) {
-            const int e = mam_open_u(bh_in, bhb, true, ch, bhp[pp], scs / 4, S.pt + 2 * (round + 2), nv - round - 2,
-                                     rs, s, chp);
+            const int e = mam_open_u(bh_in, bhb, true, ch, bhp[pp], scs / 4, ur, nv - round - 2, rs, s, chp, S.rev);
             if (e) return e;
         }
         // the oracle (3 Ext2) and the new root come back in one mailbox round trip, posted by
@@ -374,11 +387,17 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         const int n0 = S.eq ? 6 : 4;
         if (!rc)
             rc = S.eq ? mam_open_oracle0(S.eq, S.bh, S.bh_base, n / 2, rs, s)
-                      : mam_open_u(S.bh, S.bh_base, false, e2_zero(), nullptr, n / 2, S.pt + 2, nv - 1, rs, s);
+                      : mam_open_u(S.bh, S.bh_base, false, e2_zero(), nullptr, n / 2, S.rev ? S.pt_rev : S.pt + 2,
+                                   nv - 1, rs, s, nullptr, S.rev);
         if (!rc) rc = mam_mailbox_post(rs.result, n0, nullptr, 0, box0, seq0, s);
         if (!rc && late_eq) {
-            const ext2 rk = point_var(S.K);
-            rc = mam_eq_table_scaled(S.pt + 2 * S.K, nv - S.K, e2_sub(e2_one(), rk), rk, eqp[(S.K & 1) ^ 1], s);
+            if (S.rev) {
+                // coordinates K .. nv-1 in reversed order: the table in bit-reversed order
+                rc = mam_eq_table(S.pt_rev, nv - S.K, eqp[(S.K & 1) ^ 1], true, s);
+            } else {
+                const ext2 rk = point_var(S.K);
+                rc = mam_eq_table_scaled(S.pt + 2 * S.K, nv - S.K, e2_sub(e2_one(), rk), rk, eqp[(S.K & 1) ^ 1], s);
+            }
         }
         if (!rc && pq) rc = arm(0) ? mam_gate_launch(qd[0].g, s) : -1;
         if (!rc && pq) rc = queue_gated(0);
@@ -514,26 +533,37 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
             }
         }
     }
-    // point, natural-order bh and (K == 0 only) the eq table in the thread's arena slot 3
+    // Product-form openings run on the commitment's bit-reversed bh with the point's
+    // coordinates reversed (OpenSrc::rev; MAM_OPEN_REV=0: reverse bh to natural order first).
+    const bool rev = K > 0 && open_rev();
+    // point (and its reversed coordinates), natural-order bh (unless rev) and (K == 0 only)
+    // the eq table in the thread's arena slot 3
     auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
     const size_t eq_words = K == 0 ? words(2 * n) : 0;
-    u64 *arena = mam_dev_arena(words(2 * nv) + words(n) + eq_words, 3);
+    u64 *arena = mam_dev_arena(words(4 * nv) + (rev ? 0 : words(n)) + eq_words, 3);
     if (!arena) return -1;
-    u64 *d_point = arena, *bh = arena + words(2 * nv), *eq = K == 0 ? bh + words(n) : nullptr;
+    u64 *d_point = arena, *d_pt_rev = arena + 2 * nv, *bh = arena + words(4 * nv);
+    u64 *eq = K == 0 ? bh + words(n) : nullptr;
     // the point goes through the thread's pinned staging words (a copy from pageable memory
     // would be staged and waited for by the runtime)
     u64 *stage = pinned_scratch();
-    if (!stage || 2 * nv > 64) return -1;
+    if (!stage || 4 * nv > 128) return -1;
     memcpy(stage + 64, point2, 2 * nv * sizeof(u64));
-    if (hipMemcpyAsync(d_point, stage + 64, 2 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
+    for (int i = 0; i < nv; i++) {
+        stage[64 + 2 * nv + 2 * i] = point2[2 * (nv - 1 - i)];
+        stage[64 + 2 * nv + 2 * i + 1] = point2[2 * (nv - 1 - i) + 1];
+    }
+    if (hipMemcpyAsync(d_point, stage + 64, 4 * nv * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess) return -1;
     int rc = K == 0 ? mam_eq_table(d_point, nv, eq, true, s) : 0;
-    if (!rc) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
+    if (!rc && !rev) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
     if (rc) return rc;
-    OpenSrc S{K == 0 ? eq : nullptr, bh, true, C.codeword, true, C.tree, nv, C.lr, C.hash};
+    OpenSrc S{K == 0 ? eq : nullptr, rev ? C.bh_evals : bh, true, C.codeword, true, C.tree, nv, C.lr, C.hash};
     if (K > 0) {
         S.pt = d_point;
         S.pt_host = point2;
         S.K = K;
+        S.rev = rev;
+        S.pt_rev = d_pt_rev;
     }
     return open_core(S, d_table, table_nv, t, nq, proof);
 }

@@ -742,10 +742,10 @@ static unsigned open_grid(size_t items) {
 }
 
 int mam_open_u(const u64 *bh, bool bh_base, bool fold, ext2 ch, u64 *bh_out, size_t pairs, const u64 *r, int nb,
-               const RedScratch &rs, hipStream_t s, const u64 *chp) {
+               const RedScratch &rs, hipStream_t s, const u64 *chp, bool rev) {
     if (nb < 0 || nb > 30 || pairs == 0 || pairs > ((size_t)1 << nb)) return -2;
     const unsigned g = open_grid(pairs);
-    const UJob j = {bh, bh_out, ch, pairs, r, nb, rs.partials, rs.result, rs.counter, chp};
+    const UJob j = {bh, bh_out, ch, pairs, r, nb, rs.partials, rs.result, rs.counter, chp, rev ? 1 : 0};
     ProfScope prof(PROF_OPEN_SUMCHECK, (double)pairs * ((bh_base ? 8.0 : 16.0) * (fold ? 4 : 2) + (fold ? 32.0 : 0.0)),
                    s);
     if (fold) {

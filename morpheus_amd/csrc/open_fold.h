@@ -22,6 +22,9 @@ struct OracleJob {
     unsigned blocks;  // workgroups running the job (0: no job)
     int bh_base;
     const u64 *chp;  // non-null: the challenge is read here instead of ch
+    // 1: eq and bh in bit-reversed order (index k holds natural index brev(k)); output pair
+    // k <- inputs k, k + 2Q and k + Q, k + 3Q (Q = half_out), written at k and k + Q
+    int rev;
 };
 
 // `lds` holds 6 * (blockDim.x / 64) words; bid / nblocks number the job's workgroups.
@@ -33,27 +36,30 @@ __device__ __forceinline__ void open_fold_oracle_run(const OracleJob &j, unsigne
     u64 acc[6] = {0, 0, 0, 0, 0, 0};
     const ext2 ch = j.chp ? ext2{j.chp[0], j.chp[1]} : j.ch;
     const u64 ch7 = gl_mul7(ch.c1);
+    const size_t Q = j.half_out;
     for (size_t y = bid * (size_t)blockDim.x + threadIdx.x; y < j.half_out; y += (size_t)nblocks * blockDim.x) {
-        ext2 q0 = e2_load(j.eq, 4 * y), q1 = e2_load(j.eq, 4 * y + 1), q2 = e2_load(j.eq, 4 * y + 2),
-             q3 = e2_load(j.eq, 4 * y + 3);
+        // inputs of output pair y: a0, a1 fold to its first element, a2, a3 to its second
+        const size_t i0 = j.rev ? y : 4 * y, i1 = j.rev ? y + 2 * Q : 4 * y + 1;
+        const size_t i2 = j.rev ? y + Q : 4 * y + 2, i3 = j.rev ? y + 3 * Q : 4 * y + 3;
+        const size_t o0 = j.rev ? y : 2 * y, o1 = j.rev ? y + Q : 2 * y + 1;
+        ext2 q0 = e2_load(j.eq, i0), q1 = e2_load(j.eq, i1), q2 = e2_load(j.eq, i2), q3 = e2_load(j.eq, i3);
         ext2 ea = e2_add(q0, e2_mul_fixed(ch, ch7, e2_sub(q1, q0)));
         ext2 eb = e2_add(q2, e2_mul_fixed(ch, ch7, e2_sub(q3, q2)));
         ext2 ba, bb;
         if (BH_BASE) {
-            u64 a0 = gl_canon(j.bh[4 * y]), a1 = gl_canon(j.bh[4 * y + 1]);
-            u64 a2 = gl_canon(j.bh[4 * y + 2]), a3 = gl_canon(j.bh[4 * y + 3]);
+            u64 a0 = gl_canon(j.bh[i0]), a1 = gl_canon(j.bh[i1]);
+            u64 a2 = gl_canon(j.bh[i2]), a3 = gl_canon(j.bh[i3]);
             ba = e2_add(e2_base(a0), e2_scale(gl_sub(a1, a0), ch));
             bb = e2_add(e2_base(a2), e2_scale(gl_sub(a3, a2), ch));
         } else {
-            ext2 b0 = e2_load(j.bh, 4 * y), b1 = e2_load(j.bh, 4 * y + 1), b2 = e2_load(j.bh, 4 * y + 2),
-                 b3 = e2_load(j.bh, 4 * y + 3);
+            ext2 b0 = e2_load(j.bh, i0), b1 = e2_load(j.bh, i1), b2 = e2_load(j.bh, i2), b3 = e2_load(j.bh, i3);
             ba = e2_add(b0, e2_mul_fixed(ch, ch7, e2_sub(b1, b0)));
             bb = e2_add(b2, e2_mul_fixed(ch, ch7, e2_sub(b3, b2)));
         }
-        e2_store(j.eq_out, 2 * y, ea);
-        e2_store(j.eq_out, 2 * y + 1, eb);
-        e2_store(j.bh_out, 2 * y, ba);
-        e2_store(j.bh_out, 2 * y + 1, bb);
+        e2_store(j.eq_out, o0, ea);
+        e2_store(j.eq_out, o1, eb);
+        e2_store(j.bh_out, o0, ba);
+        e2_store(j.bh_out, o1, bb);
         mam_oracle_accumulate_ends(acc, ea, eb, ba, bb);
     }
     block_sum<6>(acc, lds);
@@ -81,6 +87,11 @@ struct UJob {
     u64 *partials, *result;
     u32 *counter;  // ticket of the one-launch reduction (a job riding along a tree launch)
     const u64 *chp;  // non-null: the challenge is read here instead of ch
+    // 1: bh in bit-reversed order (index k holds natural index brev(k)) and r the point's
+    // coordinates in reversed order: pair y reads k = y and y + Q (y + 2Q, y + Q, y + 3Q with
+    // the fold, Q = pairs) and writes its folded pair at y and y + Q; E(y) is then the
+    // natural E at brev(y), which is what the reversed coordinates give
+    int rev;
 };
 
 template <bool BH_BASE, bool FOLD, bool TICKET>
@@ -121,23 +132,27 @@ __device__ __forceinline__ void open_u_run(const UJob &j, unsigned bid, unsigned
             const size_t y = (c << lc) + ((size_t)i << 6) + lane;
             if (y >= j.pairs) break;
             ext2 b0, b1;
+            const size_t Q = j.pairs;
             if (FOLD) {
+                const size_t i0 = j.rev ? y : 4 * y, i1 = j.rev ? y + 2 * Q : 4 * y + 1;
+                const size_t i2 = j.rev ? y + Q : 4 * y + 2, i3 = j.rev ? y + 3 * Q : 4 * y + 3;
                 if (BH_BASE) {
-                    const u64 a0 = gl_canon(j.bh[4 * y]), a1 = gl_canon(j.bh[4 * y + 1]);
-                    const u64 a2 = gl_canon(j.bh[4 * y + 2]), a3 = gl_canon(j.bh[4 * y + 3]);
+                    const u64 a0 = gl_canon(j.bh[i0]), a1 = gl_canon(j.bh[i1]);
+                    const u64 a2 = gl_canon(j.bh[i2]), a3 = gl_canon(j.bh[i3]);
                     b0 = e2_add(e2_base(a0), e2_scale(gl_sub(a1, a0), ch));
                     b1 = e2_add(e2_base(a2), e2_scale(gl_sub(a3, a2), ch));
                 } else {
-                    const ext2 a0 = e2_load(j.bh, 4 * y), a1 = e2_load(j.bh, 4 * y + 1);
-                    const ext2 a2 = e2_load(j.bh, 4 * y + 2), a3 = e2_load(j.bh, 4 * y + 3);
+                    const ext2 a0 = e2_load(j.bh, i0), a1 = e2_load(j.bh, i1);
+                    const ext2 a2 = e2_load(j.bh, i2), a3 = e2_load(j.bh, i3);
                     b0 = e2_add(a0, e2_mul_fixed(ch, ch7, e2_sub(a1, a0)));
                     b1 = e2_add(a2, e2_mul_fixed(ch, ch7, e2_sub(a3, a2)));
                 }
-                e2_store(j.bh_out, 2 * y, b0);
-                e2_store(j.bh_out, 2 * y + 1, b1);
+                e2_store(j.bh_out, j.rev ? y : 2 * y, b0);
+                e2_store(j.bh_out, j.rev ? y + Q : 2 * y + 1, b1);
             } else {
-                b0 = BH_BASE ? e2_base(gl_canon(j.bh[2 * y])) : e2_load(j.bh, 2 * y);
-                b1 = BH_BASE ? e2_base(gl_canon(j.bh[2 * y + 1])) : e2_load(j.bh, 2 * y + 1);
+                const size_t i0 = j.rev ? y : 2 * y, i1 = j.rev ? y + Q : 2 * y + 1;
+                b0 = BH_BASE ? e2_base(gl_canon(j.bh[i0])) : e2_load(j.bh, i0);
+                b1 = BH_BASE ? e2_base(gl_canon(j.bh[i1])) : e2_load(j.bh, i1);
             }
             const size_t m1 = (y >> 6) & 63;
             const ext2 t1{tab[1][m1][0], tab[1][m1][1]};
