@@ -17,10 +17,13 @@
 //   num_queries x { index | values[nv][2][2] | initial path[nv+lr-1][4] |
 //                   folded path i = 0..nv-2: [nv+lr-2-i][4] }
 //
-// Device pipeline of the opening: eq table (one pass), bit reversal of bh_evals, round-0
-// oracle, then per round one fused fold+oracle launch for (eq, bh) and one fused
-// fold+Merkle pass for the codeword; the host reads back 10 words per round (oracle and
-// root) to run the transcript. Query values and paths are gathered by one kernel.
+// Device pipeline of the opening (single point): the round-0 oracle from the stored
+// bit-reversed bh, then per round the codeword fold and its Merkle tree with the round's
+// bh fold and sums (product form) or eq / bh fold and oracle riding along the tree's
+// launches; each round is queued behind a gate before the host has read the previous
+// round's oracle and root, which come back through the pinned mailbox (10 words) for the
+// transcript. Query values and paths are gathered by one kernel and copied back with the
+// final codeword.
 #include <stdlib.h>
 #include <string.h>
 
