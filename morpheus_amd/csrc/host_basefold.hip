@@ -94,57 +94,59 @@ struct QueryIdx {
 };
 
 // One workgroup per query: gathers values and authentication paths into the flat layout.
-// The indices come from `idx` in device memory, or from qi_inline when idx is null.
-__global__ void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx, u64 *__restrict__ out, long qlen,
-                                 QueryIdx qi_inline) {
+// The indices come from `idx` in device memory, or from qi_inline when idx is null. Every
+// 4-word item of the query (nv values, then the path entries of the nv trees, tree k at
+// depth nv + lr - 1 - k) goes to its own thread, so all of a query's reads are in flight
+// at once. `out` may be page-locked host memory (the writes then cross the bus directly).
+static constexpr int GATHER_THREADS = 256;
+__global__ __launch_bounds__(GATHER_THREADS) void k_gather_queries(QuerySrc q, const u64 *__restrict__ idx,
+                                                                    u64 *__restrict__ out, long qlen,
+                                                                    QueryIdx qi_inline) {
     const int qi = blockIdx.x;
     const u64 leaf = idx ? idx[qi] : qi_inline.v[qi];
     u64 *o = out + (size_t)qi * qlen;
     const int nv = q.nv, lr = q.lr;
+    const int D = nv + lr - 1;                     // depth of the initial tree
+    const int entries = nv * D - nv * (nv - 1) / 2;  // path entries of the nv trees
     if (threadIdx.x == 0) o[0] = leaf;
     if (qi == 0)
         for (int w = threadIdx.x; w < q.fin_words; w += blockDim.x) q.fin_out[w] = q.fin[w];
     u64 *vals = o + 1;
-    // values: round 0 from the base codeword, rounds 1..nv-1 from folded codewords
-    for (int r = threadIdx.x; r < nv; r += blockDim.x) {
-        u64 *v = vals + 4 * r;
-        if (r == 0 && q.cw_ext2) {
-            v[0] = q.cw[4 * leaf];
-            v[1] = q.cw[4 * leaf + 1];
-            v[2] = q.cw[4 * leaf + 2];
-            v[3] = q.cw[4 * leaf + 3];
-        } else if (r == 0) {
-            v[0] = q.cw[2 * leaf];
-            v[1] = 0;
-            v[2] = q.cw[2 * leaf + 1];
-            v[3] = 0;
+    for (int it = threadIdx.x; it < nv + entries; it += blockDim.x) {
+        const u64 *d;
+        u64 v[4];
+        if (it < nv) {
+            // value of round r: round 0 from the base codeword, rounds 1..nv-1 from folded codewords
+            const int r = it;
+            if (r == 0 && q.cw_ext2) {
+                d = q.cw + 4 * leaf;
+                v[0] = d[0], v[1] = d[1], v[2] = d[2], v[3] = d[3];
+            } else if (r == 0) {
+                v[0] = q.cw[2 * leaf], v[1] = 0, v[2] = q.cw[2 * leaf + 1], v[3] = 0;
+            } else {
+                d = q.fcw[r - 1] + 4 * (leaf >> r);
+                v[0] = d[0], v[1] = d[1], v[2] = d[2], v[3] = d[3];
+            }
         } else {
-            const u64 pi = leaf >> r;
-            const u64 *f = q.fcw[r - 1];
-            v[0] = f[4 * pi];
-            v[1] = f[4 * pi + 1];
-            v[2] = f[4 * pi + 2];
-            v[3] = f[4 * pi + 3];
-        }
-    }
-    // paths: tree k (k = 0 initial, k >= 1 folded tree k-1) at leaf index leaf >> k
-    u64 *p = vals + 4 * nv;
-    for (int k = 0; k < nv; k++) {
-        const int depth = nv + lr - 1 - k;  // levels below the root
-        const u64 *tr = k == 0 ? q.tree0 : q.ftree[k - 1];
-        const u64 leaves = (u64)1 << depth;
-        for (int l = threadIdx.x; l < depth; l += blockDim.x) {
-            // level l starts at digest 2*leaves - 2*(leaves >> l)
-            const u64 lvl_off = 2 * leaves - 2 * (leaves >> l);
+            // path entry e: tree k (k = 0 initial, k >= 1 folded tree k-1) at level l, leaf >> k
+            int e = it - nv, k = 0;
+            while (k < nv - 1 && e >= D - k) {
+                e -= D - k;
+                k++;
+            }
+            const int l = e, depth = D - k;
+            const u64 *tr = k == 0 ? q.tree0 : q.ftree[k - 1];
+            const u64 leaves = (u64)1 << depth;
+            const u64 lvl_off = 2 * leaves - 2 * (leaves >> l);  // level l starts at this digest
             const u64 node = ((leaf >> k) >> l) ^ 1;
-            const u64 *d = tr + 4 * (lvl_off + node);
-            u64 *w = p + 4 * l;
-            w[0] = d[0];
-            w[1] = d[1];
-            w[2] = d[2];
-            w[3] = d[3];
+            d = tr + 4 * (lvl_off + node);
+            v[0] = d[0], v[1] = d[1], v[2] = d[2], v[3] = d[3];
         }
-        p += 4 * depth;
+        u64 *w = vals + 4 * (size_t)it;
+        w[0] = v[0];
+        w[1] = v[1];
+        w[2] = v[2];
+        w[3] = v[3];
     }
 }
 
@@ -497,7 +499,7 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         else if (!rc && hipMemcpyAsync(d_idx, idx.data(), nq * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
             rc = -1;
         if (!rc) {
-            k_gather_queries<<<nq, 64, 0, s>>>(qs, inl ? nullptr : d_idx, d_out + fin_words, qlen, qin);
+            k_gather_queries<<<nq, GATHER_THREADS, 0, s>>>(qs, inl ? nullptr : d_idx, d_out + fin_words, qlen, qin);
             if (hipGetLastError() != hipSuccess) rc = -1;
         }
     } else if (!rc && hipMemcpyAsync(d_out, fcw[nv - 1], fin_words * sizeof(u64), hipMemcpyDeviceToDevice, s) !=
