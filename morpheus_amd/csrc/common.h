@@ -142,7 +142,8 @@ void mam_gate_open_all();
 bool mam_gate_take_failure();
 
 // Device side of a gate, run by the first wave of the workgroup (the other waves return):
-// lane 0 spins on the release flag, then lane i < n copies scalar i (one host read for all).
+// lane 0 spins on the release flag, then the lanes copy the n scalars (one host read for up
+// to 64; a gate armed with mam_gate_arm may be pointed at a longer pinned source).
 __device__ __forceinline__ void mam_gate_wait(const GateJob &g) {
     if (!g.flag || threadIdx.x >= 64) return;
     const int t = threadIdx.x;
@@ -157,7 +158,7 @@ __device__ __forceinline__ void mam_gate_wait(const GateJob &g) {
         }
     }
     __atomic_thread_fence(__ATOMIC_ACQUIRE);  // the slot reads follow the flag
-    if (t < g.n) g.dst[t] = __hip_atomic_load(&g.slot[t], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    for (int i = t; i < g.n; i += 64) g.dst[i] = __hip_atomic_load(&g.slot[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 // A mailbox post handed to a kernel: na words from a, then the kernel's own words.

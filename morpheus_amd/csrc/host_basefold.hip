@@ -150,10 +150,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void k_gather_queries(QuerySrc q, c
     }
 }
 
-// 192 pinned words per thread: [0, 64) read-back words, [64, 192) host-to-device staging
+// 320 pinned words per thread: [0, 64) read-back words, [64, 192) host-to-device staging,
+// [192, 320) the query indices of a pre-queued gather
+static constexpr int PIN_IDX = 192;
 u64 *pinned_scratch() {
     static thread_local u64 *buf = nullptr;
-    if (!buf && hipHostMalloc((void **)&buf, 192 * sizeof(u64), hipHostMallocDefault) != hipSuccess) buf = nullptr;
+    if (!buf && hipHostMalloc((void **)&buf, 320 * sizeof(u64), hipHostMallocDefault) != hipSuccess) buf = nullptr;
     return buf;
 }
 
@@ -270,7 +272,7 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     size_t need = 0;
     auto words = [](size_t w) { return (w + 31) & ~(size_t)31; };
     {
-        need += 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) + words(nq) +
+        need += 2 * (words(n) + words(n / 2)) + words(RED_BLOCKS * 6 + 6) + words(nq > 0 ? nq : 1) +
                 words(2 * ((size_t)1 << lr) + (size_t)nq * qlen);
         size_t flen = cw_len / 2;
         for (int r = 0; r < nv; r++) {
@@ -385,6 +387,74 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         const u64 v[2] = {ch.c0, ch.c1};
         mam_gate_open(qd[round].gate, v, 2);
     };
+    // The end of the proof: query gather (indices at idx, or in qin when idx is null) and one
+    // copy of the final codeword and the queries into `out`.
+    const size_t fin_words = 2 * ((size_t)1 << lr);
+    u64 *d_out = alloc(fin_words + (size_t)nq * qlen);
+    u64 *d_idx = alloc(nq > 0 ? nq : 1);
+    if (!d_out || !d_idx) rc = -1;
+    u64 *o_final_ptr = out + 2 + 6L * nv + 4L * (nv - 1);  // the queries follow the final codeword
+    auto queue_copy = [&]() -> int {
+        return hipMemcpyAsync(o_final_ptr, d_out, (fin_words + (size_t)nq * qlen) * sizeof(u64),
+                              hipMemcpyDeviceToHost, s) == hipSuccess
+                   ? 0
+                   : -1;
+    };
+    auto queue_gather = [&](const u64 *idx, const QueryIdx &qin) -> int {
+        if (nq > 0) {
+            QuerySrc qs;
+            memset(&qs, 0, sizeof(qs));
+            qs.cw = S.cw;
+            qs.cw_ext2 = S.cw_base ? 0 : 1;
+            qs.tree0 = S.tree0;
+            for (int r = 0; r < nv - 1; r++) {
+                qs.fcw[r] = fcw[r];
+                qs.ftree[r] = ftree[r];
+            }
+            qs.nv = nv;
+            qs.lr = lr;
+            qs.fin = fcw[nv - 1];
+            qs.fin_out = d_out;
+            qs.fin_words = (int)fin_words;
+            k_gather_queries<<<nq, GATHER_THREADS, 0, s>>>(qs, idx, d_out + fin_words, qlen, qin);
+            if (hipGetLastError() != hipSuccess) return -1;
+        } else if (hipMemcpyAsync(d_out, fcw[nv - 1], fin_words * sizeof(u64), hipMemcpyDeviceToDevice, s) !=
+                   hipSuccess) {
+            return -1;
+        }
+        return 0;
+    };
+    // Pre-queued: the gather reads its indices from device memory, copied there by a gate
+    // that the host releases once it has drawn them (at most 128 queries). The copy back is
+    // queued with it only into page-locked memory: a copy into pageable memory waits on the
+    // host for its data, which would stand behind the unreleased gate.
+    u64 *pin_idx = h ? h + PIN_IDX : nullptr;
+    const bool tail_pq = pq && nq > 0 && nq <= QIDX_INLINE && pin_idx;
+    bool tail_queued = false, copy_queued = false;
+    u64 idx_gate = 0;
+    auto out_pinned = [&]() {
+        hipPointerAttribute_t a;
+        if (hipPointerGetAttributes(&a, o_final_ptr) != hipSuccess) {
+            (void)hipGetLastError();  // pageable memory: not an error of this call's stream
+            return false;
+        }
+        return a.type == hipMemoryTypeHost;
+    };
+    auto queue_tail = [&]() -> int {
+        GateJob ig;
+        if (!mam_gate_arm(1, &ig, &idx_gate)) return -1;
+        ig.slot = pin_idx;
+        ig.dst = d_idx;
+        ig.n = nq;
+        if (mam_gate_launch(ig, s)) return -1;
+        tail_queued = true;
+        int e = queue_gather(d_idx, QueryIdx{});
+        if (!e && out_pinned()) {
+            e = queue_copy();
+            copy_queued = !e;
+        }
+        return e;
+    };
     {
         u64 seq0 = 0;
         u64 *box0 = rc ? nullptr : mam_mailbox_arm(&seq0);
@@ -412,7 +482,6 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     u64 *o_eval = out;
     u64 *o_oracles = out + 2;
     u64 *o_roots = o_oracles + 6L * nv;
-    u64 *o_final = o_roots + 4L * (nv - 1);  // the queries follow the final codeword
     if (!rc) {
         memcpy(o_oracles, h, 6 * sizeof(u64));
         // eval = P(0) + P(1) = 2 c0 + c1 + c2
@@ -436,6 +505,7 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         if (pq) {
             release(round, ch);
             rc = queue_gated(round + 1);  // the next round (or the final fold) waits behind its gate
+            if (!rc && round + 1 == nv - 1 && tail_pq) rc = queue_tail();
         } else {
             rc = queue_round(round, ch, nullptr, nullptr);
         }
@@ -464,51 +534,33 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
     }
     if (!rc) {
         // last fold, no tree; the query indices follow its challenge directly (the final
-        // codeword is not observed), so the gather is queued with it and one copy brings back
+        // codeword is not observed), so the gather follows the fold and one copy brings back
         // the final codeword and the queries
         const ext2 ch = sample_ext2(t);
         if (pq) release(nv - 1, ch);
         else rc = queue_round(nv - 1, ch, nullptr, nullptr);
     }
-    const size_t fin_words = 2 * ((size_t)1 << lr);
-    u64 *d_out = alloc(fin_words + (size_t)nq * qlen);
-    if (!rc && !d_out) rc = -1;
-    if (!rc && nq > 0) {
-        std::vector<u64> idx(nq);
-        for (int q = 0; q < nq; q++) idx[q] = mam::tr_challenge_scalar(t, nullptr, 0) % (u64)(cw_len / 2);
-        u64 *d_idx = alloc(nq);
-        if (!d_idx) rc = -1;
-        QuerySrc qs;
-        memset(&qs, 0, sizeof(qs));
-        qs.cw = S.cw;
-        qs.cw_ext2 = S.cw_base ? 0 : 1;
-        qs.tree0 = S.tree0;
-        for (int r = 0; r < nv - 1; r++) {
-            qs.fcw[r] = fcw[r];
-            qs.ftree[r] = ftree[r];
-        }
-        qs.nv = nv;
-        qs.lr = lr;
-        qs.fin = fcw[nv - 1];
-        qs.fin_out = d_out;
-        qs.fin_words = (int)fin_words;
+    if (!rc && tail_queued) {
+        // the gather and the copy wait behind the index gate (queued with the final fold)
+        for (int q = 0; q < nq; q++) pin_idx[q] = mam::tr_challenge_scalar(t, nullptr, 0) % (u64)(cw_len / 2);
+        mam_gate_open(idx_gate, nullptr, 0);
+        if (!copy_queued) rc = queue_copy();
+    } else if (!rc) {
         QueryIdx qin;
-        const bool inl = nq <= QIDX_INLINE;
-        if (inl)
-            for (int q = 0; q < nq; q++) qin.v[q] = idx[q];
-        else if (!rc && hipMemcpyAsync(d_idx, idx.data(), nq * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
-            rc = -1;
-        if (!rc) {
-            k_gather_queries<<<nq, GATHER_THREADS, 0, s>>>(qs, inl ? nullptr : d_idx, d_out + fin_words, qlen, qin);
-            if (hipGetLastError() != hipSuccess) rc = -1;
+        if (nq > 0) {
+            std::vector<u64> idx(nq);
+            for (int q = 0; q < nq; q++) idx[q] = mam::tr_challenge_scalar(t, nullptr, 0) % (u64)(cw_len / 2);
+            const bool inl = nq <= QIDX_INLINE;
+            if (inl)
+                for (int q = 0; q < nq; q++) qin.v[q] = idx[q];
+            else if (hipMemcpyAsync(d_idx, idx.data(), nq * sizeof(u64), hipMemcpyHostToDevice, s) != hipSuccess)
+                rc = -1;
+            if (!rc) rc = queue_gather(inl ? nullptr : d_idx, qin);
+        } else {
+            rc = queue_gather(nullptr, qin);
         }
-    } else if (!rc && hipMemcpyAsync(d_out, fcw[nv - 1], fin_words * sizeof(u64), hipMemcpyDeviceToDevice, s) !=
-                          hipSuccess) {
-        rc = -1;
+        if (!rc) rc = queue_copy();
     }
-    if (!rc && hipMemcpyAsync(o_final, d_out, (fin_words + (size_t)nq * qlen) * sizeof(u64), hipMemcpyDeviceToHost,
-                              s) != hipSuccess)
-        rc = -1;
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = -1;
     // a failure may leave a queued round behind its gate: release it so the stream drains
     if (rc && pq) mam_gate_open_all();
