@@ -418,7 +418,11 @@ def bench_basefold(args, M):
                                  "frac": (mk_perm / (mk_ms / 1e3) * ALG_OPS_PER_PERM / 1e12 / VALU_NOMINAL_TOPS)
                                  if mk_ms > 0 else 0.0,
                                  "avg_build_ms": mk_ms / max(mk_n, 1), "builds_per_step": mk_n / 3,
-                                 "from": "3 profiled steps after the timed region"}
+                                 "from": "3 profiled steps after the timed region",
+                                 "note": "events around each tree's launches: an opening round's time also holds "
+                                         "the product-form job riding on its first launch and, when the rounds are "
+                                         "pre-queued, the wait for the host's next challenge at the end of its "
+                                         "last launch"}
     hbm = {}
     for cat, name in ((1, "open_fold_oracle"), (2, "commit_transform")):
         ms, n, by = prof_all[cat]
