@@ -339,15 +339,18 @@ def bench_basefold(args, M):
     proof_buf = M.PinnedArray(M.proof_len(nv, lr, nq))
     proof = proof_buf.a
 
+    # argument objects made once (the calls themselves are the same as a caller's)
+    root_p, pt_p, proof_p = root.ctypes.data_as(U), pt.ctypes.data_as(U), proof.ctypes.data_as(U)
+    h = ctypes.c_void_p()
+    h_ref = ctypes.byref(h)
+
     def step():
-        M.check(lib.mam_basefold_commit_into(d_ev.ptr, nv, lr, hash_kind, d_cw.ptr, d_bh.ptr, d_tree.ptr,
-                                             root.ctypes.data_as(U)), "commit")
-        h = ctypes.c_void_p()
-        M.check(lib.mam_basefold_commitment_wrap(d_cw.ptr, d_bh.ptr, d_tree.ptr, root.ctypes.data_as(U), nv, lr,
-                                                 hash_kind, ctypes.byref(h)), "wrap")
+        M.check(lib.mam_basefold_commit_into(d_ev.ptr, nv, lr, hash_kind, d_cw.ptr, d_bh.ptr, d_tree.ptr, root_p),
+                "commit")
+        M.check(lib.mam_basefold_commitment_wrap(d_cw.ptr, d_bh.ptr, d_tree.ptr, root_p, nv, lr, hash_kind, h_ref),
+                "wrap")
         t = M.Transcript(b"bench")
-        M.check(lib.mam_basefold_open_ext2(h.value, pt.ctypes.data_as(U), d_table.ptr, nv, ctypes.byref(t.t), nq,
-                                           proof.ctypes.data_as(U)), "open")
+        M.check(lib.mam_basefold_open_ext2(h.value, pt_p, d_table.ptr, nv, ctypes.byref(t.t), nq, proof_p), "open")
         lib.mam_basefold_commitment_free(h.value)
 
     for _ in range(args.warmup):

@@ -232,6 +232,9 @@ struct OpenSrc {
     // device (UJob::rev, OracleJob::rev); no bit reversal ahead of the rounds.
     bool rev = false;
     const u64 *pt_rev = nullptr;
+    // Transcript preamble (root, then the point's nv Ext2 words) absorbed by open_core after
+    // queuing the first launches, before it reads the first oracle (null: none)
+    const u64 *pre_root = nullptr, *pre_point = nullptr;
 };
 
 // The round's oracle (c0, c1, c2) from U0, U1 when eq = sc * eq(r_k, r_k+1, ...) (product form).
@@ -476,6 +479,10 @@ int open_core(const OpenSrc &S, const u64 *d_table, int table_nv, mam_transcript
         }
         if (!rc && pq) rc = arm(0) ? mam_gate_launch(qd[0].g, s) : -1;
         if (!rc && pq) rc = queue_gated(0);
+        if (S.pre_root) {
+            observe_hash(t, S.pre_root);
+            for (int i = 0; i < nv; i++) observe_ext2(t, S.pre_point + 2 * i);
+        }
         if (!rc) rc = mam_mailbox_wait(box0, seq0, n0, h, s);
         if (!rc && !S.eq) oracle_from_u(h, e2_one(), point_var(0), h);
     }
@@ -575,9 +582,8 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     hipStream_t s = mam_stream();
     mam_stream_pool_setup();
     const size_t n = (size_t)1 << nv;
-    // transcript preamble
-    observe_hash(t, C.root);
-    for (int i = 0; i < nv; i++) observe_ext2(t, point2 + 2 * i);
+    // the transcript preamble (root, point) is absorbed by open_core once the first launches
+    // are queued
     // rounds whose eq / bh fold runs on a launch of its own take the product form (no eq
     // table); from the first round that rides along a lane-parallel tree launch on, the
     // scaled eq table is built and folded as usual
@@ -615,6 +621,8 @@ int open_ext2(const Commitment &C, const u64 *point2, const u64 *d_table, int ta
     if (!rc && !rev) rc = mam_bitrev(C.bh_evals, bh, nv, 1, s);
     if (rc) return rc;
     OpenSrc S{K == 0 ? eq : nullptr, rev ? C.bh_evals : bh, true, C.codeword, true, C.tree, nv, C.lr, C.hash};
+    S.pre_root = C.root;
+    S.pre_point = point2;
     if (K > 0) {
         S.pt = d_point;
         S.pt_host = point2;
