@@ -373,6 +373,12 @@ __device__ __forceinline__ ext2 pass_load(const u64 *p, size_t j, ext2 ca, u64 c
     return e2_add(f0, e2_mul_fixed(cb, cb7, e2_sub(f1, f0)));
 }
 
+// LDS words of sc_pass_body's reduction (block_sum_lds up to 24 words, shuffles above)
+template <int W>
+struct PassLds {
+    static constexpr int value = W <= 24 ? block_sum_lds_words(W, MAM_BLOCK) : W * (MAM_BLOCK / WAVE);
+};
+
 // The pass as workgroup `bid` of `nblk` (the grid-stride loop runs over the nblk workgroups
 // given); the workgroup's W reduced words are left in acc of thread 0 after block_sum.
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
@@ -524,7 +530,22 @@ __device__ __forceinline__ void sc_pass_body(const u64 *__restrict__ in, size_t 
             }
         }
     }
-    block_sum<W>(acc, lds);
+    if constexpr (BASE_IN && FOLDS == 0) {
+        // base products only: the c1 words are zero, so only the c0 words are reduced
+        u64 c0w[W / 2];
+#pragma unroll
+        for (int k = 0; k < W / 2; k++) c0w[k] = acc[2 * k];
+        block_sum_lds<W / 2>(c0w, lds);
+#pragma unroll
+        for (int k = 0; k < W / 2; k++) {
+            acc[2 * k] = c0w[k];
+            acc[2 * k + 1] = 0;
+        }
+    } else if constexpr (W <= 24) {
+        block_sum_lds<W>(acc, lds);
+    } else {
+        block_sum<W>(acc, lds);  // three polynomials: the LDS rows would take 80 KB
+    }
 }
 
 template <int NP, bool BASE_IN, int FOLDS, bool LOOK>
@@ -532,7 +553,7 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_sc_pass(const u64 *__restrict__ i
                                                       u64 *__restrict__ out, size_t out_stride, size_t items,
                                                       ext2 ca, ext2 cb, const u64 *chp, u64 *partials, u64 *res) {
     constexpr int W = PassWords<NP, LOOK>::value;
-    __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
+    __shared__ u64 lds[PassLds<W>::value];
     u64 acc[W];
     if (chp) pass_challenges(chp, ca, cb);
     sc_pass_body<NP, BASE_IN, FOLDS, LOOK>(in, in_stride, out, out_stride, items, ca, cb, blockIdx.x, gridDim.x, acc,
@@ -810,10 +831,10 @@ __device__ __forceinline__ int bsc_find(const BscJob *jobs, int njobs, unsigned 
     return lo;
 }
 
+// lds: PassLds<PassWords<2, true>::value>::value words (the largest of the pass shapes)
 template <bool BASE_IN, int FOLDS, bool LOOK>
-__device__ __forceinline__ void bsc_body(const BscJob &J, ext2 ca, ext2 cb, unsigned local, u64 *slot) {
+__device__ __forceinline__ void bsc_body(const BscJob &J, ext2 ca, ext2 cb, unsigned local, u64 *slot, u64 *lds) {
     constexpr int W = PassWords<2, LOOK>::value;
-    __shared__ u64 lds[W * (MAM_BLOCK / WAVE)];
     u64 acc[W];
     sc_pass_body<2, BASE_IN, FOLDS, LOOK>(J.in, J.in_stride, J.out, J.out_stride, J.items, ca, cb, local, J.nblk, acc,
                                           lds);
@@ -832,17 +853,18 @@ __global__ __launch_bounds__(MAM_BLOCK) void k_bsc_pass(const BscJob *__restrict
     const u64 *c = insts[J.inst].ch;
     const ext2 ca{gl_canon(c[0]), gl_canon(c[1])}, cb{gl_canon(c[2]), gl_canon(c[3])};
     u64 *slot = partials + (size_t)blockIdx.x * MAM_BSC_WORDS;
+    __shared__ u64 lds[PassLds<PassWords<2, true>::value>::value];
     // the pass shapes of a two-rounds-per-pass schedule: the first pass reads the base inputs
     // as they are, the second reads them with two folds applied, later passes read Ext2 arrays
     if (J.base_in && J.folds == 0) {
-        if (J.look) bsc_body<true, 0, true>(J, ca, cb, local, slot);
-        else bsc_body<true, 0, false>(J, ca, cb, local, slot);
+        if (J.look) bsc_body<true, 0, true>(J, ca, cb, local, slot, lds);
+        else bsc_body<true, 0, false>(J, ca, cb, local, slot, lds);
     } else if (J.base_in) {
-        if (J.look) bsc_body<true, 2, true>(J, ca, cb, local, slot);
-        else bsc_body<true, 2, false>(J, ca, cb, local, slot);
+        if (J.look) bsc_body<true, 2, true>(J, ca, cb, local, slot, lds);
+        else bsc_body<true, 2, false>(J, ca, cb, local, slot, lds);
     } else {
-        if (J.look) bsc_body<false, 2, true>(J, ca, cb, local, slot);
-        else bsc_body<false, 2, false>(J, ca, cb, local, slot);
+        if (J.look) bsc_body<false, 2, true>(J, ca, cb, local, slot, lds);
+        else bsc_body<false, 2, false>(J, ca, cb, local, slot, lds);
     }
 }
 

@@ -53,6 +53,41 @@ __device__ __forceinline__ void block_sum(u64 (&v)[M], u64 *lds) {
     }
 }
 
+// The same for many words (M up to blockDim.x) through LDS instead of lane shuffles: every
+// thread stores its M words (word k of all threads in row k, rows blockDim.x + 1 words apart
+// against bank conflicts), blockDim.x / M threads per word add a segment of its row each, and
+// M threads add the segments. A thread does ~M stores and ~2 M loads instead of 6 shuffle
+// steps per word. The result is valid in thread 0 only; `lds` holds
+// block_sum_lds_words(M, blockDim.x) words.
+__host__ __device__ constexpr int block_sum_lds_words(int M, int B) { return M * (B + 1) + B; }
+template <int M, class F = FieldGL>
+__device__ __forceinline__ void block_sum_lds(u64 (&v)[M], u64 *lds) {
+    const int t = threadIdx.x, B = blockDim.x, stride = B + 1;
+    const int S = B / M;  // segments per word
+    u64 *part = lds + M * stride;
+#pragma unroll
+    for (int k = 0; k < M; k++) lds[k * stride + t] = v[k];
+    mam_barrier();
+    if (t < S * M) {
+        const int k = t % M, seg = t / M;
+        const int lo = seg * B / S, hi = (seg + 1) * B / S;
+        u64 acc = 0;
+        for (int i = lo; i < hi; i++) acc = F::add(acc, lds[k * stride + i]);
+        part[seg * M + k] = acc;
+    }
+    mam_barrier();
+    if (t < M) {
+        u64 acc = 0;
+        for (int sg = 0; sg < S; sg++) acc = F::add(acc, part[sg * M + t]);
+        lds[t] = acc;  // row 0 is no longer read
+    }
+    mam_barrier();
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < M; k++) v[k] = lds[k];
+    }
+}
+
 // Completion of a reduction over `nblocks` workgroups numbered `bid` = 0 .. nblocks - 1
 // (grid_finish: every workgroup of the launch; grid_finish_n also serves a subset of a
 // launch whose other workgroups do other work). Every thread of each of those workgroups
