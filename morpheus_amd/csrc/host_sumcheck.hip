@@ -215,8 +215,13 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
     }
     const int P = (int)passes.size();
     auto words = [&](const Pass &p) { return 2 * dp1 * (p.look ? 1 + dp1 : 1); };
-    auto draw = [&](const u64 *m, int r) {
-        for (int c = 0; c < dp1; c++) mam::tr_append_ext2(t, l_msg, l_msg_len, m + 2 * c);
+    // labelled: the label of the round's first message is already absorbed (the host does it
+    // while the device computes the message; the absorbed sequence is the same)
+    auto draw = [&](const u64 *m, int r, bool labelled = false) {
+        for (int c = 0; c < dp1; c++) {
+            const bool skip = labelled && c == 0;
+            mam::tr_append_ext2(t, skip ? nullptr : l_msg, skip ? 0 : l_msg_len, m + 2 * c);
+        }
         u64 ch[2];
         mam::tr_challenge_ext2(t, l_ch, l_ch_len, ch);
         challenges[2 * r] = ch[0];
@@ -279,6 +284,9 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
                 rc = -1;
             }
         }
+        // the first message's label does not depend on the message: absorb it (its full
+        // blocks run the permutation) before waiting for the pass
+        if (!rc && l_msg_len > 0) mam::tr_append_scalars(t, l_msg, l_msg_len, nullptr, 0);
         if (!rc) rc = mam_mailbox_wait(box, seq, words(p), S.host, S.st);
         if (rc) {
             mam_gate_open_all();
@@ -286,7 +294,7 @@ int prove_lookahead(ScState &S, mam_transcript *t, const uint8_t *l_msg, long l_
         }
         u64 *m0 = round_msgs + (size_t)p.r * dp1 * 2;
         memcpy(m0, S.host, dp1 * 2 * sizeof(u64));
-        ca = draw(m0, p.r);
+        ca = draw(m0, p.r, l_msg_len > 0);
         if (p.look) {
             // round r + 1: msg(c) = sum_j ch_r^j sym[c][j] (Horner)
             u64 *m1 = round_msgs + (size_t)(p.r + 1) * dp1 * 2;
