@@ -76,11 +76,14 @@ MAM_API int mam_q_sumcheck_prove(mam_qtranscript *t, const u64 *d_polys, int num
             cur_base = false;
             nxt ^= 1;
         }
+        // the first message's label does not depend on the message: absorb it (its full block
+        // runs the permutation) while the device computes the message; the sequence is the same
+        if (!rc) mam::qtr_append_u64s(t, L_ROUND_MSG, 13, nullptr, 0);
         if (!rc) rc = mam_publish_wait(s.msg, 2 * dp1, nullptr, 0, host, st);
         if (rc) return rc;
         u64 *m = round_msgs + (size_t)r * dp1 * 2;
         memcpy(m, host, dp1 * 2 * sizeof(u64));
-        for (int c = 0; c < dp1; c++) mam::qtr_append_ext2(t, L_ROUND_MSG, 13, m + 2 * c);
+        for (int c = 0; c < dp1; c++) mam::qtr_append_ext2(t, c == 0 ? nullptr : L_ROUND_MSG, c == 0 ? 0 : 13, m + 2 * c);
         u64 chw[2];
         mam::qtr_challenge_scalars(t, L_CHALLENGE, 9, chw, 2);
         challenges[2 * r] = chw[0];
