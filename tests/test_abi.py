@@ -71,6 +71,25 @@ def test_host_transcript_matches_oracle():
         assert np.array_equal(a.state(), b.state())
 
 
+def test_label_absorbed_ahead_is_the_same_transcript():
+    """The sumcheck provers absorb a round's first message label before the message arrives
+    (host_sumcheck.hip prove_lookahead, host_qsumcheck.hip): the label appended on its own,
+    then the values appended without a label, leave both transcripts where the labelled
+    append does."""
+    rng = np.random.default_rng(11)
+    for make in (M.Transcript, M.QTranscript):
+        for lab in (b"round_message", b"x", b"abcdefgh", b"abcdefghijklmno"):
+            a, b = make(b"t"), make(b"t")
+            for _ in range(5):
+                for _ in range(3):
+                    v = rand_field(rng, 2)
+                    a.append_ext2(lab, v)
+                    b.append_scalars(lab, np.zeros(0, dtype=np.uint64))
+                    b.append_ext2(b"", v)
+                assert np.array_equal(a.challenge_ext2(b"challenge"), b.challenge_ext2(b"challenge"))
+            assert np.array_equal(np.array(a.t.state, dtype=np.uint64), np.array(b.t.state, dtype=np.uint64))
+
+
 def test_host_permutations_match_oracle():
     import ctypes
     rng = np.random.default_rng(8)
